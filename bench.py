@@ -1,0 +1,218 @@
+#!/usr/bin/env python3
+"""Benchmark: frontier transitions/s and residues/hour, batched vs one-residue,
+on one MI355X, for the corner-to-corner simple-path count at n=22.
+
+A *step* is one row sweep (cells W-1..1, W = n+1) of the interleaved frontier
+state: every main and blocked word, all k CRT moduli, in HBM.  With the
+defaults (--warmup 1 --steps 22) warmup + timed steps are the W = 23 rows of
+one complete n=22 solve, so the residues are read afterwards and checked
+against the reference's published value for the first prime.
+
+Workload: n=22, k = number of primes the checkerboard-strip bound needs (14),
+the largest primes below 2^32 -- an exact-count (CRT) run.  Inputs: the
+solve's own initial state (synthetic in the sense that nothing is read from
+disk).  Also measured in the same process:
+  * plain: the same sweep with k=1 (one residue at a time) on a few rows,
+    extrapolated to a full solve -> residues/hour of the plain form;
+  * roofline of the cell kernel: declared algorithmic bytes per launch
+    (8 B per state per modulus: read + write of one uint32 residue) divided by
+    the average launch duration from HIP events on the library's stream;
+  * cpu_baseline: the CPU oracle's pull-form step timed on the host cores on
+    a bounded sample of the same n=22 cell step.
+
+Prints ONE JSON line on stdout.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "oneesan-gpu_amd"))
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+GOLDEN_FIRST = {22: (4294967291, 1245069298), 21: (4294967291, 998035516),
+                20: (4294967291, 2308006916), 18: (4294967291, 503411004)}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(n: int, k: int, seconds: float) -> dict:
+    """Time the CPU oracle's pull-form cell step on a bounded sample of the
+    n-sized state (consecutive target windows of a middle cell), at one
+    modulus; rate = target states processed per second."""
+    import ctypes
+
+    import numpy as np
+
+    from oracle import binding
+    lib = binding.load()
+    fn = lib.or_cell_pull_range
+    fn.restype = None
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    fn.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint32, u32p, u32p, u32p, u32p,
+                   ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64]
+    nm, nb = int(lib.or_main_count(n)), int(lib.or_blocked_count(n))
+    libc = ctypes.CDLL("libc.so.6")
+    libc.calloc.restype = ctypes.c_void_p
+    libc.calloc.argtypes = [ctypes.c_size_t, ctypes.c_size_t]
+    libc.free.argtypes = [ctypes.c_void_p]
+    bufs = [libc.calloc(nm, 4), libc.calloc(nb, 4), libc.calloc(nm, 4), libc.calloc(nb, 4)]
+    if not all(bufs):
+        for b in bufs:
+            if b:
+                libc.free(b)
+        return {"value": None, "unit": "transitions/s", "cores": 0, "kind": "port",
+                "sample": "allocation failed"}
+    mi, bi, mo, bo = (ctypes.cast(b, u32p) for b in bufs)
+    W = n + 1
+    p = W // 2
+    win_m, win_b = 1 << 20, 1 << 19
+    t0m, t0b = nm // 3, nb // 3
+    done = 0
+    t_start = time.perf_counter()
+    windows = 0
+    while time.perf_counter() - t_start < seconds and t0m + win_m <= nm and t0b + win_b <= nb:
+        fn(n, p, 4294967291, mi, bi, mo, bo, t0m, t0m + win_m, t0b, t0b + win_b)
+        done += win_m + win_b
+        t0m += win_m
+        t0b += win_b
+        windows += 1
+    el = time.perf_counter() - t_start
+    for b in bufs:
+        libc.free(b)
+    cores = int(lib.or_num_threads())
+    rate = done / el if el > 0 else None
+    return {"value": rate, "unit": "transitions/s", "cores": cores, "kind": "port",
+            "sample": (f"oracle pull-form step (oracle/gridfp_oracle.c or_cell_pull_range), n={n} cell "
+                       f"p={p}, {windows} windows of {win_m} main + {win_b} blocked target states from "
+                       f"rank 1/3 on, one modulus, {el:.1f} s on {cores} threads; per-residue CPU "
+                       f"time for k moduli = k x this"),
+            "residues_per_hour_extrapolated": (3600.0 * rate / (W * (W - 1) * (nm + nb))) if rate else None}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=22, help="timed row sweeps")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed row sweeps before timing")
+    ap.add_argument("--n", type=int, default=22)
+    ap.add_argument("--k", type=int, default=0, help="moduli (0 = as many as the CRT bound needs)")
+    ap.add_argument("--plain-rows", type=int, default=2, help="rows timed for the k=1 plain form")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build measures one GPU: the frontier state of one solve "
+                                   "lives in one GPU's HBM; run with --gpus 1"}), flush=True)
+        return 2
+
+    import ofp
+    from ofp.path_bound import PRIMES, primes_for_n
+
+    n = args.n
+    W = n + 1
+    mods = primes_for_n(n) if args.k == 0 else PRIMES[:args.k]
+    k = len(mods)
+    nm, nb = ofp.state_counts(n)
+    S = nm + nb
+    log(f"bench: n={n} W={W} k={k} states={S} ({nm} main + {nb} blocked)")
+
+    ctx = ofp.Context(device=0)
+    plan = ctx.plan(n, k)
+    log(f"bench: state {plan.state_bytes / 2**30:.1f} GiB, free {plan.free_bytes / 2**30:.1f} GiB")
+    ctx.prepare(n, mods)
+    if args.warmup:
+        ctx.row_sweep(args.warmup)
+        log(f"bench: warmup {args.warmup} rows done")
+    # The library synchronises its stream before returning from row_sweep, so the
+    # timed region is bracketed by device synchronisation on both sides.
+    t0 = time.perf_counter()
+    st = ctx.row_sweep(args.steps)
+    t1 = time.perf_counter()
+    el = t1 - t0
+    log(f"bench: {args.steps} rows in {el:.3f} s (kernel {st['kernel_s']:.3f} s, {st['launches']} launches)")
+    verified = None
+    residues = None
+    if args.warmup + args.steps == W:
+        residues = ctx.read_main(0, k)
+        if n in GOLDEN_FIRST and GOLDEN_FIRST[n][0] == mods[0]:
+            verified = residues[0] == GOLDEN_FIRST[n][1]
+        log(f"bench: residues {residues} verified_first={verified}")
+
+    transitions = st["transitions"]
+    value = transitions / el
+    ms_per_step = el / args.steps * 1e3
+    solve_s = el / args.steps * W
+    residues_per_hour = 3600.0 * k / solve_s
+    launches = st["launches"]
+    avg_launch_s = st["kernel_s"] / launches
+    bytes_per_launch = 8.0 * S * min(k, 16)  # one launch carries up to 16 moduli
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+
+    # plain one-residue-at-a-time form (k=1), a few rows, extrapolated to a solve
+    plain = None
+    if args.plain_rows > 0:
+        ctx.prepare(n, [mods[0]])
+        ctx.row_sweep(1)
+        tp0 = time.perf_counter()
+        sp = ctx.row_sweep(args.plain_rows)
+        tp = time.perf_counter() - tp0
+        plain_solve_s = tp / args.plain_rows * W
+        plain = {"rows_timed": args.plain_rows, "ms_per_row": tp / args.plain_rows * 1e3,
+                 "solve_s_extrapolated": plain_solve_s,
+                 "residues_per_hour": 3600.0 / plain_solve_s,
+                 "transitions_per_s": sp["transitions"] / tp,
+                 "avg_launch_ms": sp["kernel_s"] / sp["launches"] * 1e3}
+        log(f"bench: plain k=1 {plain['ms_per_row']:.1f} ms/row -> {plain['residues_per_hour']:.0f} residues/h")
+    ctx.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(n, k, args.cpu_seconds)
+        log(f"bench: cpu baseline {cpu['value']:.3e} transitions/s on {cpu['cores']} threads")
+
+    out = {
+        "metric": "frontier transitions/s & residues/hour, batched vs 1-residue, n=22, bit-exact",
+        "value": value,
+        "unit": "transitions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (the solve's own initial frontier; nothing loaded)",
+        "config": {"workload": f"exact CRT count, n={n} ({W}x{W} vertices), {k} moduli interleaved, "
+                               f"one row sweep per step", "n": n, "moduli": k, "states": S,
+                   "main_states": nm, "blocked_states": nb, "state_bytes": plan.state_bytes,
+                   "parallelism": "single GPU"},
+        "residues_per_hour": residues_per_hour,
+        "solve_s": solve_s,
+        "verified_first_residue": verified,
+        "residues": residues,
+        "plain_k1": plain,
+        "batched_vs_plain_residues_per_hour": (residues_per_hour / plain["residues_per_hour"]) if plain else None,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "cell_kernel", "bytes_per_launch": bytes_per_launch,
+                     "avg_launch_ms": avg_launch_s * 1e3},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

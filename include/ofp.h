@@ -1,0 +1,123 @@
+/*
+ * ofp.h -- C ABI of the MI355X frontier transfer-matrix solver (libofp.so).
+ *
+ * Counts simple corner-to-corner paths on an (n+1)x(n+1) vertex grid
+ * (OEIS A007764) modulo one or more uint32 moduli, with the whole
+ * frontier state (main + blocked words, all moduli interleaved per state)
+ * resident in one GPU's HBM.
+ *
+ * The reference has no library boundary: its GPU path is a process whose
+ * argv is `n target_mib max_window ngpu mod...` and whose stdout carries one
+ * `backend=... n= residue= modulus= ... wall_s=` line per modulus
+ * (src/cuda/b300/oneesan_cuda_gridfp_b300_hbm32_factorized_reverse2_row6crt20_batch.cu:1690-1701,1841;
+ *  src/cuda/b300/oneesan_cuda_gridfp_b300_hbm32_fullmate_dropN_batch.cu:467-514).
+ * Each entry point below names the part of that process it replaces.  The
+ * CLI in oneesan-gpu_amd/csrc/ofp_cli.cpp reproduces the process contract on
+ * top of this ABI.
+ *
+ * Conventions: every function returns an ofp_status (0 = OFP_OK, negative on
+ * error) and never exits the process; ofp_last_error() gives the message.
+ * The caller owns host buffers; the library owns device memory.  A context
+ * is single-threaded (one host thread at a time).
+ */
+#ifndef OFP_H
+#define OFP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t ofp_status;
+#define OFP_OK 0
+#define OFP_E_ARG (-1)      /* bad argument (reference exit code 1) */
+#define OFP_E_DEVICE (-2)   /* no usable GPU (reference exit code 2) */
+#define OFP_E_MEMORY (-5)   /* state does not fit HBM (reference exit code 5) */
+#define OFP_E_RUNTIME (-6)  /* HIP runtime error (reference ck() -> exit 1) */
+#define OFP_E_RANGE (-7)    /* n outside the supported range */
+
+#define OFP_MAX_N 27        /* W = n+1 <= 28, as the reference's MAXW */
+
+typedef struct ofp_ctx ofp_ctx;
+
+typedef struct ofp_config {
+    int device;             /* HIP device ordinal */
+    int threads;            /* threads per workgroup, 0 = default (256) */
+    int blocks_per_cu;      /* workgroups per CU for the cell kernel, 0 = default */
+    size_t reserve_bytes;   /* HBM to leave free (reference GRIDFP_VRAM_RESERVE_MIB), 0 = default */
+    int use_graphs;         /* capture each row sweep in a hipGraph (default 1) */
+} ofp_config;
+
+typedef struct ofp_plan_info {
+    uint64_t main_states;     /* H_DP[W][1] = M_{n+2} - M_{n+1}  (row6crt20.cu:1748) */
+    uint64_t blocked_states;  /* H_DP[W-1][1] */
+    uint64_t state_bytes;     /* device bytes of the interleaved state for k moduli */
+    uint64_t transitions;     /* W*(W-1)*(main+blocked)*k: frontier transitions per solve */
+    uint64_t free_bytes;      /* free HBM on the device right now */
+    int fits;                 /* state_bytes + reserve <= free_bytes */
+} ofp_plan_info;
+
+typedef struct ofp_stats {
+    double wall_s;            /* whole solve, host clock */
+    double kernel_s;          /* sum of cell-kernel time (HIP events), 0 if not timed */
+    uint64_t transitions;     /* frontier transitions performed */
+    uint64_t main_states, blocked_states;
+    int rows, cells_per_row, launches;
+} ofp_stats;
+
+/* Create a context on cfg->device (NULL = defaults).  Replaces the
+ * reference's device discovery and DeviceCtx::init
+ * (row6crt20.cu:1716-1750; fullmate_dropN_batch.cu:483-497). */
+ofp_status ofp_create(const ofp_config* cfg, ofp_ctx** out);
+
+/* Predict state size and work for (n, k).  Replaces the reference's HBM
+ * budget check (row6crt20.cu:1751-1755; fullmate_dropN_batch.cu:495-501);
+ * the serving loop admits requests with it. */
+ofp_status ofp_plan(ofp_ctx* ctx, int n, int k, ofp_plan_info* out);
+
+/* Full solve: residues_out[j] = A007764(n) mod moduli[j] for j < k, all k
+ * moduli advanced together as one interleaved residue vector per state.
+ * Replaces the reference's residue loop (row6crt20.cu:1784-1841;
+ * fullmate_dropN_batch.cu:509-515), which ran the moduli one after another.
+ * moduli[j] must be in [2, 2^32-1] (row6crt20.cu:1697-1698). */
+ofp_status ofp_solve(ofp_ctx* ctx, int n, const uint32_t* moduli, int k,
+                     uint32_t* residues_out, ofp_stats* stats);
+
+/* Apply cells p_hi..p_lo (W-1 >= p_hi >= p_lo >= 1) of one row to caller
+ * state, on the device, in the product layout.  main_io holds
+ * main_states*k residues and blocked_io blocked_states*k residues, both
+ * state-major (index*k + j) on the host; they are updated in place.  If
+ * blocked_valid is 0, the blocked input of the first cell is taken as zero
+ * (as at the start of a row).  This is the unit the reference's own GPU
+ * tests drive (tests/factor_transfer_batch_test.cu:44-96: one window of
+ * cells on seeded random vectors, compared with a CPU forward recurrence). */
+ofp_status ofp_apply_cells(ofp_ctx* ctx, int n, const uint32_t* moduli, int k,
+                           uint32_t* main_io, uint32_t* blocked_io,
+                           int p_hi, int p_lo, int blocked_valid);
+
+/* One timed row sweep (cells W-1..1) over the device-resident state of the
+ * last ofp_prepare() call; used by the benchmark and the autotuner.  The
+ * state is not re-initialised, so repeated calls advance it row by row. */
+ofp_status ofp_prepare(ofp_ctx* ctx, int n, const uint32_t* moduli, int k);
+ofp_status ofp_row_sweep(ofp_ctx* ctx, int rows, ofp_stats* stats);
+/* Read residues of main state `index` (k values) from the prepared state. */
+ofp_status ofp_read_main(ofp_ctx* ctx, uint64_t index, uint32_t* out_k);
+
+/* Kernel launch tuning (threads per workgroup, workgroups per CU); set by
+ * the autotuner, 0 = default.  Reference analogue: GRIDFP_FRONTIER_THREADS /
+ * GRIDFP_FRONTIER_BLOCKS (row6crt20.cu:1182,1192,1306-1307). */
+ofp_status ofp_set_launch(ofp_ctx* ctx, int threads, int blocks_per_cu);
+
+/* Counts only, no device needed: main / blocked word counts for n. */
+ofp_status ofp_state_counts(int n, uint64_t* main_states, uint64_t* blocked_states);
+
+const char* ofp_last_error(const ofp_ctx* ctx);
+const char* ofp_version(void);
+void ofp_destroy(ofp_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OFP_H */

@@ -1,0 +1,119 @@
+// ofp_cell.hip -- the per-cell frontier transfer step for gfx950.
+//
+// One cell p of the Grid-FP sweep (reference semantics:
+// src/common/gridfp_transition.hpp:113-146, src/common/gridfp_reverse.hpp:39-98,
+// applied as in tests/factor_transfer_batch_test.cu:60-72) is performed IN
+// PLACE on the whole device-resident state with one thread per blocked word.
+//
+// Why one thread per blocked word: write a blocked word of width W-1 at cell p
+// as b = [P, x, S] (P = positions above p-1, x = position p-1, S = positions
+// below).  The main words touched by cell p pair up with b exactly:
+//   x = N : main [P,N,N,S] (NN) and [P,L,R,S] (LR);  NN --include--> LR
+//   x = R : main [P,N,R,S] (NR) and [P,R,N,S] (RN);  RN --include--> NR
+//   x = L : main [P,N,L,S] (NL) and [P,L,N,S] (LN);  LN --include--> NL
+// and the blocked input consumed at cell p (blocked_exclude = minsert(b,p,N))
+// lands on [P,N,x,S], the blocked output produced at cell p has the same index
+// b.  Every main word written at cell p (NN, LR, NR, NL; at p=1 also RN) is
+// owned by exactly one thread, and every main word read by a foreign thread
+// (RL, LL, RR: sources of closures) is never written at cell p.  Hence one
+// launch updates main and blocked in place with no atomics and no second copy
+// of the state; at n=22 with 14 moduli that is 156 GB instead of 313 GB.
+//
+// Index math: the thread decodes b by walking the height table from the top
+// (lexicographic rank, N<R<L, entered at height 1), accumulating along the way
+// the rank contribution the prefix P has in the main numbering.  Every main
+// index it needs differs from that base by the contributions of the two cell
+// positions (and, for closure predecessors, by one flipped label), so each is
+// a few table lookups.  The closure predecessors of an x=N target are the
+// reference's reverse_block_predecessors (LL: top-level L's of S; RR:
+// top-level R's of P), enumerated with running rank sums so that each costs
+// O(1) beyond the scan.
+//
+// Residues: uint32 mod p, additions only (every transfer coefficient is 1);
+// a + b mod p is computed with one compare-and-subtract.
+#include "ofp_cell_impl.h"
+
+namespace ofp {
+
+namespace {
+
+using namespace cell;
+
+// Work distribution: the blocked range is cut into chunks of blockDim.x words;
+// workgroup g runs on XCD g%8 (round-robin dispatch), so the chunks are split
+// into 8 contiguous slabs and XCD x's workgroups stride through slab x.  Each
+// XCD's L2 then sees one contiguous region of the state.
+template <int KC, typename Idx>
+__global__ __launch_bounds__(256) void cell_kernel(CellLaunch a) {
+    __shared__ Idx tab[kDpRows][kDpCols];
+    const Idx* g = (const Idx*)a.dp;
+    for (int t = threadIdx.x; t < kDpRows * kDpCols; t += blockDim.x) (&tab[0][0])[t] = g[t];
+    __syncthreads();
+    Tab<Idx> T{tab};
+    const uint64_t chunks = (a.nblk + blockDim.x - 1) / blockDim.x;
+    const uint32_t nwg = gridDim.x;  // multiple of 8
+    const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3, nx = nwg >> 3;
+    const uint64_t c0 = (chunks * xcd) >> 3, c1 = (chunks * (xcd + 1)) >> 3;
+    for (uint64_t c = c0 + slot; c < c1; c += nx) {
+        const uint64_t b = c * blockDim.x + threadIdx.x;
+        if (b < a.nblk) cell_one<KC, Idx>(a, T, a.first + b);
+    }
+}
+
+__global__ void set_one_kernel(uint32_t* main, uint64_t index, int K, const uint32_t* mods) {
+    int j = threadIdx.x;
+    if (j < K) main[aosoa(index, K, j)] = 1u % mods[j];
+}
+
+template <typename Idx, int KC>
+hipError_t launch_kc(const CellLaunch& a, int threads, int blocks, hipStream_t s) {
+    hipLaunchKernelGGL((cell_kernel<KC, Idx>), dim3(blocks), dim3(threads), 0, s, a);
+    return hipGetLastError();
+}
+
+template <typename Idx>
+hipError_t launch_idx(const CellLaunch& a, int threads, int blocks, hipStream_t s) {
+    switch (a.kc) {
+#define OFP_KC(n) case n: return launch_kc<Idx, n>(a, threads, blocks, s);
+        OFP_KC(1) OFP_KC(2) OFP_KC(3) OFP_KC(4) OFP_KC(5) OFP_KC(6) OFP_KC(7) OFP_KC(8)
+        OFP_KC(9) OFP_KC(10) OFP_KC(11) OFP_KC(12) OFP_KC(13) OFP_KC(14) OFP_KC(15) OFP_KC(16)
+#undef OFP_KC
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_cell(const CellLaunch& a, bool wide_index, int threads, int blocks,
+                       hipStream_t s) {
+    if (threads <= 0 || threads > 256 || (threads & 63) || blocks <= 0 || (blocks & 7))
+        return hipErrorInvalidConfiguration;
+    if (a.W < 2 || a.W > kMaxW || a.p < 1 || a.p > a.W - 1 || a.kc < 1 || a.kc > kMaxKC ||
+        a.off < 0 || a.off + a.kc > a.K)
+        return hipErrorInvalidValue;
+    return wide_index ? launch_idx<uint64_t>(a, threads, blocks, s)
+                      : launch_idx<uint32_t>(a, threads, blocks, s);
+}
+
+hipError_t launch_set_one(uint32_t* main, uint64_t index, int K, const uint32_t* mods_dev,
+                          hipStream_t s) {
+    if (K < 1 || K > 1024) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(set_one_kernel, dim3(1), dim3(((K + 63) / 64) * 64), 0, s, main, index, K,
+                       mods_dev);
+    return hipGetLastError();
+}
+
+void build_height_table(uint64_t (*d)[kDpCols]) {
+    for (int w = 0; w < kDpRows; ++w)
+        for (int h = 0; h < kDpCols; ++h) d[w][h] = 0;
+    d[0][0] = 1;
+    for (int w = 1; w < kDpRows; ++w)
+        for (int h = 0; h < kDpCols; ++h) {
+            uint64_t x = d[w - 1][h];
+            if (h > 0) x += d[w - 1][h - 1];
+            if (h + 1 < kDpCols) x += d[w - 1][h + 1];
+            d[w][h] = x;
+        }
+}
+
+}  // namespace ofp

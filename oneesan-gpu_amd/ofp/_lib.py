@@ -1,0 +1,226 @@
+"""ctypes binding of libofp.so (include/ofp.h).
+
+The product path is the HIP library; there is no CPU fallback.  Loading fails
+loudly when the in-tree library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+from pathlib import Path
+from typing import Sequence
+
+PKG = Path(__file__).resolve().parents[1]
+LIB_PATH = Path(os.environ.get("OFP_LIB", PKG / "lib" / "libofp.so"))
+
+OFP_OK = 0
+OFP_E_ARG = -1
+OFP_E_DEVICE = -2
+OFP_E_MEMORY = -5
+OFP_E_RUNTIME = -6
+OFP_E_RANGE = -7
+OFP_MAX_N = 27
+
+# Exported names, in the order include/ofp.h declares them.
+EXPORTS = (
+    "ofp_create", "ofp_plan", "ofp_solve", "ofp_apply_cells", "ofp_prepare",
+    "ofp_row_sweep", "ofp_read_main", "ofp_set_launch", "ofp_state_counts",
+    "ofp_last_error", "ofp_version", "ofp_destroy",
+)
+
+
+class OfpError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"ofp status {status}: {message}")
+        self.status = status
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("device", ctypes.c_int),
+        ("threads", ctypes.c_int),
+        ("blocks_per_cu", ctypes.c_int),
+        ("reserve_bytes", ctypes.c_size_t),
+        ("use_graphs", ctypes.c_int),
+    ]
+
+
+class PlanInfo(ctypes.Structure):
+    _fields_ = [
+        ("main_states", ctypes.c_uint64),
+        ("blocked_states", ctypes.c_uint64),
+        ("state_bytes", ctypes.c_uint64),
+        ("transitions", ctypes.c_uint64),
+        ("free_bytes", ctypes.c_uint64),
+        ("fits", ctypes.c_int),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("wall_s", ctypes.c_double),
+        ("kernel_s", ctypes.c_double),
+        ("transitions", ctypes.c_uint64),
+        ("main_states", ctypes.c_uint64),
+        ("blocked_states", ctypes.c_uint64),
+        ("rows", ctypes.c_int),
+        ("cells_per_row", ctypes.c_int),
+        ("launches", ctypes.c_int),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load libofp.so (once).  Raises if the library has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it first (python __graft_entry__.py or "
+            "python oneesan-gpu_amd/build.py); there is no CPU fallback")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+    vp = ctypes.c_void_p
+    sig = {
+        "ofp_create": (ctypes.c_int32, [ctypes.POINTER(Config), ctypes.POINTER(vp)]),
+        "ofp_plan": (ctypes.c_int32, [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(PlanInfo)]),
+        "ofp_solve": (ctypes.c_int32, [vp, ctypes.c_int, u32p, ctypes.c_int, u32p, ctypes.POINTER(Stats)]),
+        "ofp_apply_cells": (ctypes.c_int32, [vp, ctypes.c_int, u32p, ctypes.c_int, u32p, u32p,
+                                             ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+        "ofp_prepare": (ctypes.c_int32, [vp, ctypes.c_int, u32p, ctypes.c_int]),
+        "ofp_row_sweep": (ctypes.c_int32, [vp, ctypes.c_int, ctypes.POINTER(Stats)]),
+        "ofp_read_main": (ctypes.c_int32, [vp, ctypes.c_uint64, u32p]),
+        "ofp_set_launch": (ctypes.c_int32, [vp, ctypes.c_int, ctypes.c_int]),
+        "ofp_state_counts": (ctypes.c_int32, [ctypes.c_int, u64p, u64p]),
+        "ofp_last_error": (ctypes.c_char_p, [vp]),
+        "ofp_version": (ctypes.c_char_p, []),
+        "ofp_destroy": (None, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _u32_array(values: Sequence[int]):
+    arr = (ctypes.c_uint32 * len(values))()
+    for i, v in enumerate(values):
+        arr[i] = int(v)
+    return arr
+
+
+def state_counts(n: int) -> tuple[int, int]:
+    lib = load()
+    m, b = ctypes.c_uint64(), ctypes.c_uint64()
+    st = lib.ofp_state_counts(n, ctypes.byref(m), ctypes.byref(b))
+    if st != OFP_OK:
+        raise OfpError(st, f"state_counts(n={n})")
+    return m.value, b.value
+
+
+@dataclass
+class Plan:
+    main_states: int
+    blocked_states: int
+    state_bytes: int
+    transitions: int
+    free_bytes: int
+    fits: bool
+
+
+class Context:
+    """One GPU context (a device, a stream, one device-resident state)."""
+
+    def __init__(self, device: int = 0, threads: int = 0, blocks_per_cu: int = 0,
+                 reserve_bytes: int = 0, use_graphs: bool = True):
+        self._lib = load()
+        cfg = Config(device, threads, blocks_per_cu, reserve_bytes, 1 if use_graphs else 0)
+        h = ctypes.c_void_p()
+        st = self._lib.ofp_create(ctypes.byref(cfg), ctypes.byref(h))
+        if st != OFP_OK:
+            raise OfpError(st, "ofp_create failed (no usable GPU?)")
+        self._h = h
+
+    def _check(self, st: int, what: str) -> None:
+        if st != OFP_OK:
+            msg = self._lib.ofp_last_error(self._h)
+            raise OfpError(st, f"{what}: {msg.decode() if msg else ''}")
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.ofp_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def plan(self, n: int, k: int) -> Plan:
+        info = PlanInfo()
+        self._check(self._lib.ofp_plan(self._h, n, k, ctypes.byref(info)), "ofp_plan")
+        return Plan(info.main_states, info.blocked_states, info.state_bytes, info.transitions,
+                    info.free_bytes, bool(info.fits))
+
+    def solve(self, n: int, moduli: Sequence[int]) -> tuple[list[int], dict]:
+        k = len(moduli)
+        mods = _u32_array(moduli)
+        out = (ctypes.c_uint32 * k)()
+        stats = Stats()
+        self._check(self._lib.ofp_solve(self._h, n, mods, k, out, ctypes.byref(stats)), "ofp_solve")
+        return list(out), stats.as_dict()
+
+    def apply_cells(self, n: int, moduli: Sequence[int], main, blocked, p_hi: int, p_lo: int,
+                    blocked_valid: bool = True) -> None:
+        """main / blocked: contiguous numpy uint32 arrays of shape (states, k), updated in place."""
+        import numpy as np
+        k = len(moduli)
+        if main.dtype != np.uint32 or blocked.dtype != np.uint32:
+            raise TypeError("state arrays must be uint32")
+        if not (main.flags.c_contiguous and blocked.flags.c_contiguous):
+            raise ValueError("state arrays must be C-contiguous")
+        nm, nb = state_counts(n)
+        if main.shape != (nm, k) or blocked.shape != (nb, k):
+            raise ValueError(f"expected shapes {(nm, k)} and {(nb, k)}")
+        u32p = ctypes.POINTER(ctypes.c_uint32)
+        st = self._lib.ofp_apply_cells(self._h, n, _u32_array(moduli), k,
+                                       main.ctypes.data_as(u32p), blocked.ctypes.data_as(u32p),
+                                       p_hi, p_lo, 1 if blocked_valid else 0)
+        self._check(st, "ofp_apply_cells")
+
+    def prepare(self, n: int, moduli: Sequence[int]) -> None:
+        self._check(self._lib.ofp_prepare(self._h, n, _u32_array(moduli), len(moduli)), "ofp_prepare")
+
+    def row_sweep(self, rows: int) -> dict:
+        stats = Stats()
+        self._check(self._lib.ofp_row_sweep(self._h, rows, ctypes.byref(stats)), "ofp_row_sweep")
+        return stats.as_dict()
+
+    def read_main(self, index: int, k: int) -> list[int]:
+        out = (ctypes.c_uint32 * k)()
+        self._check(self._lib.ofp_read_main(self._h, index, out), "ofp_read_main")
+        return list(out)
+
+    def set_launch(self, threads: int, blocks_per_cu: int) -> None:
+        self._check(self._lib.ofp_set_launch(self._h, threads, blocks_per_cu), "ofp_set_launch")
+
+
+def version() -> str:
+    return load().ofp_version().decode()

@@ -1,0 +1,111 @@
+"""GPU parity of the HIP cell sweep against the CPU oracle (through the C ABI).
+
+Pattern follows the reference's own GPU tests (tests/factor_transfer_batch_test.cu:44-96):
+seeded random residues in [0, p) on every main and blocked word, one window of
+cells, every coefficient compared with the CPU forward recurrence, moduli
+{2, 4294967291, 4294966997}.  End-to-end residues are compared with the
+reference's published values (tests/golden/reference_residues.json) and with
+the independent vertex/component DP.  The bar is bit-exact.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import load_golden
+
+pytestmark = pytest.mark.gpu
+
+MODS3 = [2, 4294967291, 4294966997]
+
+
+def _oracle_cells(oracle, n, mod, main, blk, p_hi, p_lo, blocked_valid):
+    m = np.ascontiguousarray(main.copy())
+    b = np.ascontiguousarray(blk.copy())
+    oracle.apply_cells(n, mod, m, b, p_hi, p_lo, blocked_valid)
+    return m, b
+
+
+@pytest.mark.parametrize("n", [2, 3, 5, 7, 9])
+def test_every_cell_random_vectors(gpu_ctx, oracle, n):
+    """Each single cell p on random full vectors, k=3 moduli interleaved."""
+    import ofp
+    nm, nb = ofp.state_counts(n)
+    W = n + 1
+    rng = np.random.default_rng(0x62C31F05 + n)
+    for p in range(W - 1, 0, -1):
+        for blocked_valid in (True, False):
+            main = np.stack([rng.integers(0, m, nm, dtype=np.uint64) for m in MODS3], 1).astype(np.uint32)
+            blk = np.stack([rng.integers(0, m, nb, dtype=np.uint64) for m in MODS3], 1).astype(np.uint32)
+            g_main, g_blk = main.copy(), blk.copy()
+            print(f"case n={n} p={p} blocked_valid={blocked_valid}", flush=True)
+            gpu_ctx.apply_cells(n, MODS3, g_main, g_blk, p, p, blocked_valid)
+            for j, mod in enumerate(MODS3):
+                em, eb = _oracle_cells(oracle, n, mod, main[:, j], blk[:, j], p, p, blocked_valid)
+                assert np.array_equal(g_main[:, j], em), (n, p, mod, "main")
+                assert np.array_equal(g_blk[:, j], eb), (n, p, mod, "blocked")
+
+
+@pytest.mark.parametrize("n,k", [(8, 1), (9, 14), (10, 17), (11, 5)])
+def test_whole_row_windows(gpu_ctx, oracle, n, k):
+    """A window of consecutive cells (two windows covering a row), k moduli."""
+    import ofp
+    nm, nb = ofp.state_counts(n)
+    W = n + 1
+    primes = [4294967291, 4294967279, 4294967231, 4294967197, 4294967189, 4294967161,
+              4294967143, 4294967111, 4294967087, 4294967029, 4294966997, 4294966981,
+              4294966943, 4294966927, 4294966909, 4294966877, 4294966829]
+    mods = (primes + [2, 3, 65537])[:k] if k <= 17 else None
+    rng = np.random.default_rng(1234 + n * 31 + k)
+    half = (W - 1) // 2 + 1
+    for p_hi, p_lo in ((W - 1, half), (half - 1, 1)):
+        main = np.stack([rng.integers(0, m, nm, dtype=np.uint64) for m in mods], 1).astype(np.uint32)
+        blk = np.stack([rng.integers(0, m, nb, dtype=np.uint64) for m in mods], 1).astype(np.uint32)
+        g_main, g_blk = main.copy(), blk.copy()
+        print(f"case n={n} k={k} window={p_hi}..{p_lo}", flush=True)
+        gpu_ctx.apply_cells(n, mods, g_main, g_blk, p_hi, p_lo, True)
+        for j in sorted({0, k // 2, k - 1}):
+            em, eb = _oracle_cells(oracle, n, mods[j], main[:, j], blk[:, j], p_hi, p_lo, True)
+            assert np.array_equal(g_main[:, j], em), (n, k, j, p_hi, p_lo)
+            assert np.array_equal(g_blk[:, j], eb), (n, k, j, p_hi, p_lo)
+
+
+def test_small_n_against_exact_counts(gpu_ctx, oracle):
+    """n=1..12: GPU residues equal the exact vertex/component DP count mod p."""
+    mods = [4294967291, 4294966997, 2147483647, 2, 3, 1000003]
+    for n in range(1, 13):
+        print(f"case n={n}", flush=True)
+        got, _ = gpu_ctx.solve(n, mods)
+        exact = int(load_golden("exact_counts.json")["counts"][str(n)])
+        assert got == [exact % m for m in mods], n
+
+
+def test_reference_residues(gpu_ctx):
+    """Every published reference residue with n <= 21 (n=22/23 are bench-sized)."""
+    g = load_golden("reference_residues.json")
+    by_n: dict[int, list[tuple[int, int]]] = {}
+    for r in g["residues"]:
+        if r["n"] <= 21:
+            by_n.setdefault(r["n"], []).append((r["modulus"], r["residue"]))
+    for n, pairs in sorted(by_n.items()):
+        print(f"case reference n={n}", flush=True)
+        got, _ = gpu_ctx.solve(n, [m for m, _ in pairs])
+        assert got == [r for _, r in pairs], n
+
+
+def test_batched_equals_one_at_a_time(gpu_ctx):
+    """k moduli interleaved give the same residues as k separate one-modulus runs."""
+    mods = [4294967291, 4294967279, 4294967231, 4294967197, 4294967189, 4294967161,
+            4294967143, 4294967111, 4294967087, 4294967029, 4294966997, 4294966981,
+            4294966943, 4294966927, 4294966909, 4294966877, 4294966829, 4294966813]
+    n = 14
+    batched, _ = gpu_ctx.solve(n, mods)
+    single = [gpu_ctx.solve(n, [m])[0][0] for m in mods]
+    assert batched == single
+
+
+def test_n12_config_residue(gpu_ctx, oracle):
+    """Config 1 (n=12, p=4294967291): GPU == Grid-FP oracle == vertex DP."""
+    got, _ = gpu_ctx.solve(12, [4294967291])
+    assert got[0] == oracle.solve(12, 4294967291)
+    assert got[0] == int(load_golden("exact_counts.json")["counts"]["12"]) % 4294967291
