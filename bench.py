@@ -98,6 +98,30 @@ def cpu_baseline(n: int, k: int, seconds: float) -> dict:
             "residues_per_hour_extrapolated": (3600.0 * rate / (W * (W - 1) * (nm + nb))) if rate else None}
 
 
+def copy_calibration(nbytes: int) -> int:
+    """One device-to-device copy of nbytes (HIP runtime blit kernel), so the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE counters can be scaled: a copy of B bytes
+    must read and write B bytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    for ptr in (src, dst):
+        if hip.hipMalloc(ctypes.byref(ptr), ctypes.c_size_t(nbytes)) != 0:
+            log("copy calibration: hipMalloc failed")
+            return 1
+    hip.hipMemset(src, 1, ctypes.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    rc = hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    el = time.perf_counter() - t0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"copy_bytes": nbytes, "seconds": el, "GBps_read_plus_write": 2 * nbytes / el / 1e9,
+                      "rc": rc}), flush=True)
+    return 0 if rc == 0 else 1
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,7 +132,11 @@ def main() -> int:
     ap.add_argument("--plain-rows", type=int, default=2, help="rows timed for the k=1 plain form")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run one device-to-device copy of BYTES (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -127,7 +155,10 @@ def main() -> int:
     S = nm + nb
     log(f"bench: n={n} W={W} k={k} states={S} ({nm} main + {nb} blocked)")
 
+    from ofp import autotune
     ctx = ofp.Context(device=0)
+    tuned = autotune.apply(ctx, n, k)
+    log(f"bench: launch config {tuned}")
     plan = ctx.plan(n, k)
     log(f"bench: state {plan.state_bytes / 2**30:.1f} GiB, free {plan.free_bytes / 2**30:.1f} GiB")
     ctx.prepare(n, mods)
@@ -162,6 +193,7 @@ def main() -> int:
     # plain one-residue-at-a-time form (k=1), a few rows, extrapolated to a solve
     plain = None
     if args.plain_rows > 0:
+        autotune.apply(ctx, n, 1)
         ctx.prepare(n, [mods[0]])
         ctx.row_sweep(1)
         tp0 = time.perf_counter()
@@ -196,6 +228,7 @@ def main() -> int:
         "data": "synthetic (the solve's own initial frontier; nothing loaded)",
         "config": {"workload": f"exact CRT count, n={n} ({W}x{W} vertices), {k} moduli interleaved, "
                                f"one row sweep per step", "n": n, "moduli": k, "states": S,
+                   "launch": tuned,
                    "main_states": nm, "blocked_states": nb, "state_bytes": plan.state_bytes,
                    "parallelism": "single GPU"},
         "residues_per_hour": residues_per_hour,

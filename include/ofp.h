@@ -48,6 +48,7 @@ typedef struct ofp_config {
     int blocks_per_cu;      /* workgroups per CU for the cell kernel, 0 = default */
     size_t reserve_bytes;   /* HBM to leave free (reference GRIDFP_VRAM_RESERVE_MIB), 0 = default */
     int use_graphs;         /* capture each row sweep in a hipGraph (default 1) */
+    int layout;             /* state layout: 0 = default, 1 = 64-state tiles, 2 = state-major */
 } ofp_config;
 
 typedef struct ofp_plan_info {
@@ -109,6 +110,13 @@ ofp_status ofp_read_main(ofp_ctx* ctx, uint64_t index, uint32_t* out_k);
  * the autotuner, 0 = default.  Reference analogue: GRIDFP_FRONTIER_THREADS /
  * GRIDFP_FRONTIER_BLOCKS (row6crt20.cu:1182,1192,1306-1307). */
 ofp_status ofp_set_launch(ofp_ctx* ctx, int threads, int blocks_per_cu);
+
+/* State layout for the next prepare/solve: 0 = tiles of 64 states (one
+ * modulus of 64 consecutive states is 256 contiguous bytes), 1 = state-major
+ * (the k residues of a state contiguous, stride padded to even).  No
+ * reference counterpart (the reference runs one modulus at a time); set by
+ * the autotuner.  Invalidates the prepared state. */
+ofp_status ofp_set_layout(ofp_ctx* ctx, int layout);
 
 /* Counts only, no device needed: main / blocked word counts for n. */
 ofp_status ofp_state_counts(int n, uint64_t* main_states, uint64_t* blocked_states);

@@ -26,19 +26,20 @@ struct ofp_ctx {
     int threads = 256;
     int blocks_per_cu = 8;
     int use_graphs = 1;
+    int layout = 1;  // 0: 64-state tiles, 1: state-major with even stride
     size_t reserve = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    void* dp32 = nullptr;  // uint32 height table (n <= 22)
-    void* dp64 = nullptr;  // uint64 height table
-    uint64_t host_dp[kDpRows][kDpCols];
+    void* dp32 = nullptr;  // packed uint32 height table (n <= 22)
+    void* dp64 = nullptr;  // packed uint64 height table
     std::string err;
     // device-resident state
     uint32_t* main = nullptr;
     uint32_t* blk = nullptr;
     size_t main_cap = 0, blk_cap = 0;  // bytes allocated
     uint32_t* mods_dev = nullptr;
-    int n = 0, W = 0, K = 0;
+    int n = 0, W = 0, K = 0, Kp = 0;  // k moduli, interleave stride Kp
+    int state_layout = -1;           // layout the current buffers were written in
     std::vector<uint32_t> mods;
     uint64_t nmain = 0, nblk = 0;
     // row graph cache
@@ -101,8 +102,9 @@ ofp_status ensure_state(ofp_ctx* c, int n, const uint32_t* moduli, int k) {
     if (k < 1 || k > 1024) return fail(c, OFP_E_ARG, "number of moduli must be in 1..1024");
     for (int j = 0; j < k; ++j)
         if (moduli[j] < 2) return fail(c, OFP_E_ARG, "modulus must be in [2, 4294967295]");
-    size_t mb = padded_states(nm) * (size_t)k * sizeof(uint32_t);
-    size_t bb = padded_states(nb ? nb : 1) * (size_t)k * sizeof(uint32_t);
+    const int Kp = interleave_stride(c->layout, k);
+    size_t mb = padded_states(nm) * (size_t)Kp * sizeof(uint32_t);
+    size_t bb = padded_states(nb ? nb : 1) * (size_t)Kp * sizeof(uint32_t);
     OFP_HIP(c, hipSetDevice(c->device), "hipSetDevice");
     if (mb > c->main_cap || bb > c->blk_cap) {
         drop_graph(c);
@@ -132,6 +134,8 @@ ofp_status ensure_state(ofp_ctx* c, int n, const uint32_t* moduli, int k) {
     c->n = n;
     c->W = n + 1;
     c->K = k;
+    c->Kp = Kp;
+    c->state_layout = c->layout;
     c->nmain = nm;
     c->nblk = nb;
     return OFP_OK;
@@ -149,10 +153,11 @@ ofp_status enqueue_cells(ofp_ctx* c, int p_hi, int p_lo, bool first_reads_blk, i
             a.dp = wide ? c->dp64 : c->dp32;
             a.nblk = c->nblk;
             a.first = 0;
+            a.layout = c->layout;
             a.W = c->W;
             a.p = p;
             a.read_blk = (p == p_hi) ? (first_reads_blk ? 1 : 0) : 1;
-            a.K = c->K;
+            a.K = c->Kp;
             a.off = off;
             a.kc = std::min(kMaxKC, c->K - off);
             for (int j = 0; j < a.kc; ++j) a.mod[j] = c->mods[off + j];
@@ -174,8 +179,8 @@ ofp_status run_rows(ofp_ctx* c, int rows, int* launches) {
         return OFP_OK;
     }
     char key[256];
-    snprintf(key, sizeof key, "%p/%p/%d/%d/%d/%d", (void*)c->main, (void*)c->blk, c->n, c->K,
-             c->threads, c->blocks_per_cu);
+    snprintf(key, sizeof key, "%p/%p/%d/%d/%d/%d/%d", (void*)c->main, (void*)c->blk, c->n, c->K,
+             c->threads, c->blocks_per_cu, c->layout);
     std::string k = key;
     for (uint32_t m : c->mods) k += "/" + std::to_string(m);
     if (k != c->row_key || !c->row_exec) {
@@ -199,11 +204,11 @@ ofp_status run_rows(ofp_ctx* c, int rows, int* launches) {
 }
 
 ofp_status init_state(ofp_ctx* c) {
-    OFP_HIP(c, hipMemsetAsync(c->main, 0, padded_states(c->nmain) * (size_t)c->K * 4, c->stream),
+    OFP_HIP(c, hipMemsetAsync(c->main, 0, padded_states(c->nmain) * (size_t)c->Kp * 4, c->stream),
             "zero main state");
     // start word: R at position W-1, rank = H_DP[W-1][1] (fullmate_dropN_batch.cu:508)
     uint64_t start = dp_at(c->W - 1, 1);
-    hipError_t e = launch_set_one(c->main, start, c->K, c->mods_dev, c->stream);
+    hipError_t e = launch_set_one(c->main, start, c->Kp, c->K, c->layout, c->mods_dev, c->stream);
     if (e != hipSuccess) return hip_fail(c, e, "init kernel");
     return OFP_OK;
 }
@@ -235,6 +240,7 @@ ofp_status ofp_create(const ofp_config* cfg, ofp_ctx** out) {
         if (cfg->blocks_per_cu) c->blocks_per_cu = cfg->blocks_per_cu;
         c->reserve = cfg->reserve_bytes;
         c->use_graphs = cfg->use_graphs;
+        if (cfg->layout == 1 || cfg->layout == 2) c->layout = cfg->layout - 1;
     }
     if (c->threads <= 0 || c->threads > 256 || (c->threads & 63) || c->blocks_per_cu <= 0) {
         delete c;
@@ -263,14 +269,16 @@ ofp_status ofp_create(const ofp_config* cfg, ofp_ctx** out) {
     }
     if (bad(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking))) return OFP_E_RUNTIME;
     if (bad(hipEventCreate(&c->ev0)) || bad(hipEventCreate(&c->ev1))) return OFP_E_RUNTIME;
-    build_height_table(c->host_dp);
-    uint32_t t32[kDpRows][kDpCols];
-    for (int w = 0; w < kDpRows; ++w)
-        for (int h = 0; h < kDpCols; ++h) t32[w][h] = (uint32_t)c->host_dp[w][h];
-    if (bad(hipMalloc(&c->dp32, sizeof t32)) || bad(hipMalloc(&c->dp64, sizeof c->host_dp)))
+    std::vector<unsigned char> q32(64 * 1024), q64(64 * 1024);
+    size_t b32 = pack_height_table(false, q32.data(), q32.size());
+    size_t b64 = pack_height_table(true, q64.data(), q64.size());
+    if (!b32 || !b64) {
+        delete c;
         return OFP_E_RUNTIME;
-    if (bad(hipMemcpy(c->dp32, t32, sizeof t32, hipMemcpyHostToDevice)) ||
-        bad(hipMemcpy(c->dp64, c->host_dp, sizeof c->host_dp, hipMemcpyHostToDevice)))
+    }
+    if (bad(hipMalloc(&c->dp32, b32)) || bad(hipMalloc(&c->dp64, b64))) return OFP_E_RUNTIME;
+    if (bad(hipMemcpy(c->dp32, q32.data(), b32, hipMemcpyHostToDevice)) ||
+        bad(hipMemcpy(c->dp64, q64.data(), b64, hipMemcpyHostToDevice)))
         return OFP_E_RUNTIME;
     *out = c;
     return OFP_OK;
@@ -288,6 +296,17 @@ ofp_status ofp_set_launch(ofp_ctx* c, int threads, int blocks_per_cu) {
     return OFP_OK;
 }
 
+ofp_status ofp_set_layout(ofp_ctx* c, int layout) {
+    if (!c) return OFP_E_ARG;
+    if (layout != 0 && layout != 1) return fail(c, OFP_E_ARG, "layout must be 0 (64-state tiles) or 1 (state-major)");
+    if (layout != c->layout) {
+        drop_graph(c);
+        c->layout = layout;
+        c->state_layout = -1;  // current contents no longer readable; prepare again
+    }
+    return OFP_OK;
+}
+
 ofp_status ofp_plan(ofp_ctx* c, int n, int k, ofp_plan_info* out) {
     if (!c || !out) return OFP_E_ARG;
     uint64_t nm, nb;
@@ -295,7 +314,8 @@ ofp_status ofp_plan(ofp_ctx* c, int n, int k, ofp_plan_info* out) {
     if (k < 1) return fail(c, OFP_E_ARG, "k must be >= 1");
     out->main_states = nm;
     out->blocked_states = nb;
-    out->state_bytes = (padded_states(nm) + padded_states(nb ? nb : 1)) * (uint64_t)k * 4;
+    out->state_bytes = (padded_states(nm) + padded_states(nb ? nb : 1)) *
+                       (uint64_t)interleave_stride(c->layout, k) * 4;
     const uint64_t W = n + 1;
     out->transitions = W * (W - 1) * (nm + nb) * (uint64_t)k;
     size_t fr = 0, tot = 0;
@@ -320,7 +340,7 @@ ofp_status ofp_prepare(ofp_ctx* c, int n, const uint32_t* moduli, int k) {
 
 ofp_status ofp_row_sweep(ofp_ctx* c, int rows, ofp_stats* st) {
     if (!c || rows < 0) return OFP_E_ARG;
-    if (!c->main) return fail(c, OFP_E_ARG, "no prepared state");
+    if (!c->main || c->state_layout != c->layout) return fail(c, OFP_E_ARG, "no prepared state");
     int launches = 0;
     auto t0 = std::chrono::steady_clock::now();
     OFP_HIP(c, hipEventRecord(c->ev0, c->stream), "event record");
@@ -348,7 +368,8 @@ ofp_status ofp_read_main(ofp_ctx* c, uint64_t index, uint32_t* out_k) {
     if (index >= c->nmain) return fail(c, OFP_E_ARG, "index out of range");
     OFP_HIP(c, hipStreamSynchronize(c->stream), "sync");
     for (int j = 0; j < c->K; ++j)
-        OFP_HIP(c, hipMemcpy(out_k + j, c->main + aosoa(index, c->K, j), 4, hipMemcpyDeviceToHost),
+        OFP_HIP(c, hipMemcpy(out_k + j, c->main + state_elem(c->state_layout, index, c->Kp, j), 4,
+                             hipMemcpyDeviceToHost),
                 "read residue");
     return OFP_OK;
 }
@@ -382,11 +403,13 @@ ofp_status ofp_apply_cells(ofp_ctx* c, int n, const uint32_t* moduli, int k, uin
     ofp_status s = ensure_state(c, n, moduli, k);
     if (s) return s;
     // host state-major -> device tiles
-    std::vector<uint32_t> tm(padded_states(c->nmain) * (size_t)k, 0), tb(padded_states(c->nblk ? c->nblk : 1) * (size_t)k, 0);
+    const int Kp = c->Kp, lay = c->layout;
+    std::vector<uint32_t> tm(padded_states(c->nmain) * (size_t)Kp, 0),
+        tb(padded_states(c->nblk ? c->nblk : 1) * (size_t)Kp, 0);
     for (uint64_t i = 0; i < c->nmain; ++i)
-        for (int j = 0; j < k; ++j) tm[aosoa(i, k, j)] = main_io[i * k + j];
+        for (int j = 0; j < k; ++j) tm[state_elem(lay, i, Kp, j)] = main_io[i * k + j];
     for (uint64_t i = 0; i < c->nblk; ++i)
-        for (int j = 0; j < k; ++j) tb[aosoa(i, k, j)] = blocked_io[i * k + j];
+        for (int j = 0; j < k; ++j) tb[state_elem(lay, i, Kp, j)] = blocked_io[i * k + j];
     OFP_HIP(c, hipMemcpy(c->main, tm.data(), tm.size() * 4, hipMemcpyHostToDevice), "upload main");
     OFP_HIP(c, hipMemcpy(c->blk, tb.data(), tb.size() * 4, hipMemcpyHostToDevice), "upload blocked");
     s = enqueue_cells(c, p_hi, p_lo, blocked_valid != 0, nullptr);
@@ -395,10 +418,10 @@ ofp_status ofp_apply_cells(ofp_ctx* c, int n, const uint32_t* moduli, int k, uin
     OFP_HIP(c, hipMemcpy(tm.data(), c->main, tm.size() * 4, hipMemcpyDeviceToHost), "download main");
     OFP_HIP(c, hipMemcpy(tb.data(), c->blk, tb.size() * 4, hipMemcpyDeviceToHost), "download blocked");
     for (uint64_t i = 0; i < c->nmain; ++i)
-        for (int j = 0; j < k; ++j) main_io[i * k + j] = tm[aosoa(i, k, j)];
+        for (int j = 0; j < k; ++j) main_io[i * k + j] = tm[state_elem(lay, i, Kp, j)];
     // after cell 1 nothing is blocked; the device buffer is simply not written
     for (uint64_t i = 0; i < c->nblk; ++i)
-        for (int j = 0; j < k; ++j) blocked_io[i * k + j] = (p_lo == 1) ? 0u : tb[aosoa(i, k, j)];
+        for (int j = 0; j < k; ++j) blocked_io[i * k + j] = (p_lo == 1) ? 0u : tb[state_elem(lay, i, Kp, j)];
     return OFP_OK;
 }
 

@@ -43,10 +43,10 @@ using namespace cell;
 // workgroup g runs on XCD g%8 (round-robin dispatch), so the chunks are split
 // into 8 contiguous slabs and XCD x's workgroups stride through slab x.  Each
 // XCD's L2 then sees one contiguous region of the state.
-template <int KC, typename Idx>
+template <int KC, typename Idx, int L>
 __global__ __launch_bounds__(256) void cell_kernel(CellLaunch a) {
-    __shared__ Idx tab[kDpRows][kDpCols];
-    const Idx* g = (const Idx*)a.dp;
+    __shared__ Quad<Idx> tab[kDpRows][kDpCols];
+    const Quad<Idx>* g = (const Quad<Idx>*)a.dp;
     for (int t = threadIdx.x; t < kDpRows * kDpCols; t += blockDim.x) (&tab[0][0])[t] = g[t];
     __syncthreads();
     Tab<Idx> T{tab};
@@ -56,18 +56,22 @@ __global__ __launch_bounds__(256) void cell_kernel(CellLaunch a) {
     const uint64_t c0 = (chunks * xcd) >> 3, c1 = (chunks * (xcd + 1)) >> 3;
     for (uint64_t c = c0 + slot; c < c1; c += nx) {
         const uint64_t b = c * blockDim.x + threadIdx.x;
-        if (b < a.nblk) cell_one<KC, Idx>(a, T, a.first + b);
+        if (b < a.nblk) cell_one<KC, Idx, L>(a, T, a.first + b);
     }
 }
 
-__global__ void set_one_kernel(uint32_t* main, uint64_t index, int K, const uint32_t* mods) {
+__global__ void set_one_kernel(uint32_t* main, uint64_t index, int K, int k, int layout,
+                               const uint32_t* mods) {
     int j = threadIdx.x;
-    if (j < K) main[aosoa(index, K, j)] = 1u % mods[j];
+    if (j < k) main[state_elem(layout, index, K, j)] = 1u % mods[j];
 }
 
 template <typename Idx, int KC>
 hipError_t launch_kc(const CellLaunch& a, int threads, int blocks, hipStream_t s) {
-    hipLaunchKernelGGL((cell_kernel<KC, Idx>), dim3(blocks), dim3(threads), 0, s, a);
+    if (a.layout == kAoS)
+        hipLaunchKernelGGL((cell_kernel<KC, Idx, kAoS>), dim3(blocks), dim3(threads), 0, s, a);
+    else
+        hipLaunchKernelGGL((cell_kernel<KC, Idx, kAoSoA>), dim3(blocks), dim3(threads), 0, s, a);
     return hipGetLastError();
 }
 
@@ -89,18 +93,32 @@ hipError_t launch_cell(const CellLaunch& a, bool wide_index, int threads, int bl
     if (threads <= 0 || threads > 256 || (threads & 63) || blocks <= 0 || (blocks & 7))
         return hipErrorInvalidConfiguration;
     if (a.W < 2 || a.W > kMaxW || a.p < 1 || a.p > a.W - 1 || a.kc < 1 || a.kc > kMaxKC ||
-        a.off < 0 || a.off + a.kc > a.K)
+        a.off < 0 || a.off + a.kc > a.K || (a.layout != kAoSoA && a.layout != kAoS) ||
+        (a.layout == kAoS && a.K > 1 && ((a.K & 1) || (a.off & 1))))
         return hipErrorInvalidValue;
     return wide_index ? launch_idx<uint64_t>(a, threads, blocks, s)
                       : launch_idx<uint32_t>(a, threads, blocks, s);
 }
 
-hipError_t launch_set_one(uint32_t* main, uint64_t index, int K, const uint32_t* mods_dev,
-                          hipStream_t s) {
-    if (K < 1 || K > 1024) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(set_one_kernel, dim3(1), dim3(((K + 63) / 64) * 64), 0, s, main, index, K,
-                       mods_dev);
+hipError_t launch_set_one(uint32_t* main, uint64_t index, int K, int k, int layout,
+                          const uint32_t* mods_dev, hipStream_t s) {
+    if (k < 1 || k > 1024 || K < k) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(set_one_kernel, dim3(1), dim3(((k + 63) / 64) * 64), 0, s, main, index, K, k,
+                       layout, mods_dev);
     return hipGetLastError();
+}
+
+size_t pack_height_table(bool wide, void* out, size_t cap) {
+    uint64_t d[kDpRows][kDpCols];
+    build_height_table(d);
+    if (wide) {
+        if (cap < sizeof(Quad<uint64_t>) * kDpRows * kDpCols) return 0;
+        pack_quads<uint64_t>(d, (Quad<uint64_t>(*)[kDpCols])out);
+        return sizeof(Quad<uint64_t>) * kDpRows * kDpCols;
+    }
+    if (cap < sizeof(Quad<uint32_t>) * kDpRows * kDpCols) return 0;
+    pack_quads<uint32_t>(d, (Quad<uint32_t>(*)[kDpCols])out);
+    return sizeof(Quad<uint32_t>) * kDpRows * kDpCols;
 }
 
 void build_height_table(uint64_t (*d)[kDpCols]) {

@@ -16,61 +16,122 @@ OFP_HD uint32_t addm(uint32_t a, uint32_t b, uint32_t m) {
     return (s < a || s >= m) ? s - m : s;
 }
 
+// One entry of the packed height table, for position pos and height h:
+//   zn = D[pos][h], zr = D[pos][h-1] (0 at h=0)      -- decode at pos
+//   mn = D[pos+1][h], mr = D[pos+1][h-1] (0 at h=0)  -- the same value one position up
+// so that decoding a blocked word (width W-1) and accumulating its main-numbering
+// rank (width W, every prefix position shifted up by one) needs one 16-byte
+// (or 32-byte) table read per position.
+template <typename Idx>
+struct Quad {
+    Idx zn, zr, mn, mr;
+};
+
 template <typename Idx>
 struct Tab {
-    const Idx (*d)[kDpCols];
-    OFP_HD Idx at(int pos, int h) const { return d[pos][h]; }
-    // rank contribution of value v at position pos entered at height h
+    const Quad<Idx> (*q)[kDpCols];
+    OFP_HD Idx at(int pos, int h) const { return q[pos][h].zn; }
+    OFP_HD Idx cL(int pos, int h) const { return q[pos][h].zn + q[pos][h].zr; }
+    // rank contribution of value v (0=N,1=R,2=L) at position pos entered at height h
     OFP_HD Idx c(int pos, int v, int h) const {
-        Idx z = 0;
-        if (v >= 1) z = d[pos][h];
-        if (v == 2 && h > 0) z += d[pos][h - 1];
-        return z;
+        const Quad<Idx> e = q[pos][h];
+        return v == 0 ? Idx(0) : (v == 1 ? e.zn : Idx(e.zn + e.zr));
     }
-    OFP_HD Idx cL(int pos, int h) const {
-        return d[pos][h] + (h > 0 ? d[pos][h - 1] : Idx(0));
-    }
-    // decode one position: consume r, return value (0=N,1=R,2=L) and its contribution
+    // decode one position: consume r, return the value and its contribution
     OFP_HD int step(int pos, int h, Idx& r, Idx& contrib) const {
-        Idx zN = d[pos][h];
-        if (r < zN) { contrib = 0; return 0; }
-        r -= zN;
-        Idx zR = h > 0 ? d[pos][h - 1] : Idx(0);
-        if (r < zR) { contrib = zN; return 1; }
-        r -= zR;
-        contrib = zN + zR;
+        const Quad<Idx> e = q[pos][h];
+        return step_e(e, r, contrib);
+    }
+    OFP_HD static int step_e(const Quad<Idx>& e, Idx& r, Idx& contrib) {
+        if (r < e.zn) { contrib = 0; return 0; }
+        r -= e.zn;
+        if (r < e.zr) { contrib = e.zn; return 1; }
+        r -= e.zr;
+        contrib = e.zn + e.zr;
         return 2;
     }
 };
+
+// State layouts.  kAoSoA: tiles of 64 states, modulus-major inside a tile
+// (a wave reading one modulus of 64 consecutive states reads 256 contiguous
+// bytes).  kAoS: state-major with an even stride, residues of one state
+// contiguous (one scattered state is one or two 64-byte sectors), read two
+// residues per 8-byte load.
+enum Layout : int { kAoSoA = 0, kAoS = 1 };
+
+template <int L>
+OFP_HD uint64_t elem(uint64_t i, int K, int j) {
+    if constexpr (L == kAoSoA) return aosoa(i, K, j);
+    else return i * (uint64_t)K + (uint64_t)j;
+}
 
 template <int KC>
 struct Vals {
     uint32_t v[KC];
 };
 
-template <int KC>
-OFP_HD void load_vals(const uint32_t* __restrict__ base, uint64_t i, int K,
-                                          int off, Vals<KC>& out) {
-    const uint32_t* q = base + aosoa(i, K, off);
+template <int KC, int L>
+OFP_HD void load_vals(const uint32_t* __restrict__ base, uint64_t i, int K, int off, Vals<KC>& out) {
+    const uint32_t* q = base + elem<L>(i, K, off);
+    if constexpr (L == kAoSoA) {
 #pragma unroll
-    for (int j = 0; j < KC; ++j) out.v[j] = q[j * kTile];
+        for (int j = 0; j < KC; ++j) out.v[j] = q[j * kTile];
+    } else {
+        const uint2* q2 = reinterpret_cast<const uint2*>(q);
+#pragma unroll
+        for (int j = 0; j + 1 < KC; j += 2) {
+            uint2 t = q2[j / 2];
+            out.v[j] = t.x;
+            out.v[j + 1] = t.y;
+        }
+        if constexpr (KC & 1) out.v[KC - 1] = q[KC - 1];
+    }
+}
+template <int KC, int L>
+OFP_HD void store_vals(uint32_t* __restrict__ base, uint64_t i, int K, int off, const Vals<KC>& in) {
+    uint32_t* q = base + elem<L>(i, K, off);
+    if constexpr (L == kAoSoA) {
+#pragma unroll
+        for (int j = 0; j < KC; ++j) q[j * kTile] = in.v[j];
+    } else {
+        uint2* q2 = reinterpret_cast<uint2*>(q);
+#pragma unroll
+        for (int j = 0; j + 1 < KC; j += 2) q2[j / 2] = make_uint2(in.v[j], in.v[j + 1]);
+        if constexpr (KC & 1) q[KC - 1] = in.v[KC - 1];
+    }
 }
 template <int KC>
-OFP_HD void store_vals(uint32_t* __restrict__ base, uint64_t i, int K, int off,
-                                           const Vals<KC>& in) {
-    uint32_t* q = base + aosoa(i, K, off);
+OFP_HD void add_into(Vals<KC>& acc, const Vals<KC>& x, const uint32_t* mod) {
 #pragma unroll
-    for (int j = 0; j < KC; ++j) q[j * kTile] = in.v[j];
-}
-template <int KC>
-OFP_HD void acc_vals(Vals<KC>& acc, const uint32_t* __restrict__ base,
-                                         uint64_t i, int K, int off, const uint32_t* mod) {
-    const uint32_t* q = base + aosoa(i, K, off);
-#pragma unroll
-    for (int j = 0; j < KC; ++j) acc.v[j] = addm(acc.v[j], q[j * kTile], mod[j]);
+    for (int j = 0; j < KC; ++j) acc.v[j] = addm(acc.v[j], x.v[j], mod[j]);
 }
 
-template <int KC, typename Idx>
+// Closure predecessors are found one at a time by a scan; their residues are
+// accumulated one candidate late, so the loads of candidate c+1 are issued
+// before the values of candidate c are added (no memory latency per candidate).
+template <int KC, int L>
+struct Gather {
+    Vals<KC> acc, pend;
+    bool have = false;
+    OFP_HD void init() {
+#pragma unroll
+        for (int j = 0; j < KC; ++j) acc.v[j] = 0;
+        have = false;
+    }
+    OFP_HD void add(const uint32_t* __restrict__ base, uint64_t i, int K, int off, const uint32_t* mod) {
+        Vals<KC> cur;
+        load_vals<KC, L>(base, i, K, off, cur);
+        if (have) add_into<KC>(acc, pend, mod);
+        pend = cur;
+        have = true;
+    }
+    OFP_HD void finish(const uint32_t* mod) {
+        if (have) add_into<KC>(acc, pend, mod);
+        have = false;
+    }
+};
+
+template <int KC, typename Idx, int L>
 OFP_HD void cell_one(const CellLaunch& a, const Tab<Idx>& T, uint64_t bidx) {
     const int W = a.W, p = a.p, K = a.K, off = a.off;
     uint32_t* __restrict__ mainp = a.main;
@@ -85,9 +146,10 @@ OFP_HD void cell_one(const CellLaunch& a, const Tab<Idx>& T, uint64_t bidx) {
     Idx mainbase = 0;  // sum of main-numbering contributions of P
     uint64_t word = 0;
     for (int pos = W - 2; pos >= p; --pos) {
+        const Quad<Idx> e = T.q[pos][h];
         Idx cb;
-        int v = T.step(pos, h, r, cb);
-        mainbase += T.c(pos + 1, v, h);
+        const int v = Tab<Idx>::step_e(e, r, cb);
+        mainbase += (v == 0) ? Idx(0) : (v == 1 ? e.mn : Idx(e.mn + e.mr));
         word |= (uint64_t)v << (2 * pos);
         h += (v == 2) - (v == 1);
     }
@@ -100,18 +162,17 @@ OFP_HD void cell_one(const CellLaunch& a, const Tab<Idx>& T, uint64_t bidx) {
         // orbit NN -> LR
         const uint64_t iNN = (uint64_t)(mainbase + sr);
         const uint64_t iLR = (uint64_t)(mainbase + T.cL(p, hP) + T.at(p - 1, hP + 1) + sr);
-        Vals<KC> nn, lr, bc, acc;
-        load_vals<KC>(mainp, iNN, K, off, nn);
-        load_vals<KC>(mainp, iLR, K, off, lr);
-        if (a.read_blk) load_vals<KC>(blkp, bidx, K, off, bc);
-#pragma unroll
-        for (int j = 0; j < KC; ++j) acc.v[j] = 0;
-
+        Vals<KC> nn, lr, bc;
+        load_vals<KC, L>(mainp, iNN, K, off, nn);
+        load_vals<KC, L>(mainp, iLR, K, off, lr);
+        if (a.read_blk) load_vals<KC, L>(blkp, bidx, K, off, bc);
+        Gather<KC, L> g;
+        g.init();
         if (p >= 2) {
             // RL predecessor: [P,R,L,S]
             if (hP > 0) {
                 const uint64_t iRL = (uint64_t)(mainbase + T.at(p, hP) + T.cL(p - 1, hP - 1) + sr);
-                acc_vals<KC>(acc, mainp, iRL, K, off, mod);
+                g.add(mainp, iRL, K, off, mod);
             }
             // LL predecessors: [P,L,L,S*], S* = S with a top-level L at q turned into R,
             // entered at height hP+2 (reverse_block_predecessors, gridfp_reverse.hpp:62-68)
@@ -121,13 +182,12 @@ OFP_HD void cell_one(const CellLaunch& a, const Tab<Idx>& T, uint64_t bidx) {
                 int hh = hP, depth = 1;
                 for (int pos = p - 2; pos >= 0; --pos) {
                     Idx c0;
-                    int v = T.step(pos, hh, rr, c0);
-                    if (v == 2 && depth == 1) {
-                        const uint64_t iLL = (uint64_t)(baseLL + s2 + T.at(pos, hh + 2) + (sr - s0 - c0));
-                        acc_vals<KC>(acc, mainp, iLL, K, off, mod);
-                    }
+                    const int v = T.step(pos, hh, rr, c0);
+                    const Quad<Idx> e2 = T.q[pos][hh + 2];
+                    if (v == 2 && depth == 1)
+                        g.add(mainp, (uint64_t)(baseLL + s2 + e2.zn + (sr - s0 - c0)), K, off, mod);
                     s0 += c0;
-                    s2 += T.c(pos, v, hh + 2);
+                    s2 += (v == 0) ? Idx(0) : (v == 1 ? e2.zn : Idx(e2.zn + e2.zr));
                     if (v == 2) ++depth;
                     else if (v == 1) { if (--depth == 0) break; }
                     hh += (v == 2) - (v == 1);
@@ -143,68 +203,83 @@ OFP_HD void cell_one(const CellLaunch& a, const Tab<Idx>& T, uint64_t bidx) {
             for (int mp = p + 1; mp <= W - 1; ++mp) {
                 const int v = (int)((word >> (2 * (mp - 1))) & 3u);
                 const int H = Ha + (v == 1) - (v == 2);
-                const Idx c0 = T.c(mp, v, H);
-                if (v == 1 && depth == 1) {
-                    const uint64_t iRR = (uint64_t)(mainbase - (u0 + c0) + T.cL(mp, H) + u2 + tail);
-                    acc_vals<KC>(acc, mainp, iRR, K, off, mod);
-                }
+                const Quad<Idx> e0 = T.q[mp][H];
+                const Quad<Idx> e2 = T.q[mp][H + 2];
+                const Idx c0 = (v == 0) ? Idx(0) : (v == 1 ? e0.zn : Idx(e0.zn + e0.zr));
+                if (v == 1 && depth == 1)
+                    g.add(mainp, (uint64_t)(mainbase - (u0 + c0) + e0.zn + e0.zr + u2 + tail), K, off, mod);
                 u0 += c0;
-                u2 += T.c(mp, v, H + 2);
+                u2 += (v == 0) ? Idx(0) : (v == 1 ? e2.zn : Idx(e2.zn + e2.zr));
                 if (v == 2) { if (--depth == 0) break; }
                 else if (v == 1) ++depth;
                 Ha = H;
             }
         }
+        g.finish(mod);
         Vals<KC> onn, olr;
 #pragma unroll
         for (int j = 0; j < KC; ++j) {
             uint32_t t = a.read_blk ? addm(nn.v[j], bc.v[j], mod[j]) : nn.v[j];
             olr.v[j] = addm(lr.v[j], nn.v[j], mod[j]);
-            if (p == 1) t = addm(t, acc.v[j], mod[j]);
+            if (p == 1) t = addm(t, g.acc.v[j], mod[j]);
             onn.v[j] = t;
         }
-        store_vals<KC>(mainp, iNN, K, off, onn);
-        store_vals<KC>(mainp, iLR, K, off, olr);
-        if (p >= 2) store_vals<KC>(blkp, bidx, K, off, acc);
+        store_vals<KC, L>(mainp, iNN, K, off, onn);
+        store_vals<KC, L>(mainp, iLR, K, off, olr);
+        if (p >= 2) store_vals<KC, L>(blkp, bidx, K, off, g.acc);
     } else if (x == 1) {
         // orbit RN -> NR (and at p=1 also NR -> RN)
         const uint64_t iNR = (uint64_t)(mainbase + T.at(p - 1, hP) + sr);
         const uint64_t iRN = (uint64_t)(mainbase + T.at(p, hP) + sr);
         Vals<KC> nr, rn, bc, o;
-        load_vals<KC>(mainp, iNR, K, off, nr);
-        load_vals<KC>(mainp, iRN, K, off, rn);
-        if (a.read_blk) load_vals<KC>(blkp, bidx, K, off, bc);
+        load_vals<KC, L>(mainp, iNR, K, off, nr);
+        load_vals<KC, L>(mainp, iRN, K, off, rn);
+        if (a.read_blk) load_vals<KC, L>(blkp, bidx, K, off, bc);
 #pragma unroll
         for (int j = 0; j < KC; ++j) {
             uint32_t t = addm(nr.v[j], rn.v[j], mod[j]);
             if (a.read_blk) t = addm(t, bc.v[j], mod[j]);
             o.v[j] = t;
         }
-        store_vals<KC>(mainp, iNR, K, off, o);
+        store_vals<KC, L>(mainp, iNR, K, off, o);
         if (p >= 2) {
-            store_vals<KC>(blkp, bidx, K, off, nr);
+            store_vals<KC, L>(blkp, bidx, K, off, nr);
         } else {
 #pragma unroll
             for (int j = 0; j < KC; ++j) o.v[j] = addm(rn.v[j], nr.v[j], mod[j]);
-            store_vals<KC>(mainp, iRN, K, off, o);
+            store_vals<KC, L>(mainp, iRN, K, off, o);
         }
     } else {
         // orbit LN -> NL (cannot occur at p=1)
         const uint64_t iNL = (uint64_t)(mainbase + T.cL(p - 1, hP) + sr);
         const uint64_t iLN = (uint64_t)(mainbase + T.cL(p, hP) + sr);
         Vals<KC> nl, ln, bc, o;
-        load_vals<KC>(mainp, iNL, K, off, nl);
-        load_vals<KC>(mainp, iLN, K, off, ln);
-        if (a.read_blk) load_vals<KC>(blkp, bidx, K, off, bc);
+        load_vals<KC, L>(mainp, iNL, K, off, nl);
+        load_vals<KC, L>(mainp, iLN, K, off, ln);
+        if (a.read_blk) load_vals<KC, L>(blkp, bidx, K, off, bc);
 #pragma unroll
         for (int j = 0; j < KC; ++j) {
             uint32_t t = addm(nl.v[j], ln.v[j], mod[j]);
             if (a.read_blk) t = addm(t, bc.v[j], mod[j]);
             o.v[j] = t;
         }
-        store_vals<KC>(mainp, iNL, K, off, o);
-        if (p >= 2) store_vals<KC>(blkp, bidx, K, off, nl);
+        store_vals<KC, L>(mainp, iNL, K, off, o);
+        if (p >= 2) store_vals<KC, L>(blkp, bidx, K, off, nl);
     }
+}
+
+// Packed table from the plain height table d (kDpRows x kDpCols).
+template <typename Idx>
+inline void pack_quads(const uint64_t (*d)[kDpCols], Quad<Idx> (*q)[kDpCols]) {
+    for (int pos = 0; pos < kDpRows; ++pos)
+        for (int h = 0; h < kDpCols; ++h) {
+            Quad<Idx> e;
+            e.zn = (Idx)d[pos][h];
+            e.zr = h > 0 ? (Idx)d[pos][h - 1] : Idx(0);
+            e.mn = pos + 1 < kDpRows ? (Idx)d[pos + 1][h] : Idx(0);
+            e.mr = (pos + 1 < kDpRows && h > 0) ? (Idx)d[pos + 1][h - 1] : Idx(0);
+            q[pos][h] = e;
+        }
 }
 
 }  // namespace cell

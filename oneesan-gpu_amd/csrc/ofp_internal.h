@@ -12,6 +12,9 @@ constexpr int kDpCols = 32;        // heights 0..31 (h+2 <= W+2 <= 30)
 constexpr int kTile = 64;          // AoSoA tile: 64 states x K moduli
 constexpr int kMaxKC = 16;         // moduli handled by one launch
 
+// Packed height table (cell::Quad entries) for device upload; returns bytes.
+size_t pack_height_table(bool wide, void* out, size_t cap);
+
 // Motzkin-height table: D[w][h] = number of words over w positions entered at
 // height h that never go negative and end at height 0.  Same recurrence as the
 // reference's H_DP (src/cuda/b300/oneesan_cuda_gridfp_b300_hbm32_fullmate_dropN_batch.cu:62).
@@ -25,24 +28,32 @@ __host__ __device__ inline uint64_t aosoa(uint64_t i, int K, int j) {
     return (i >> 6) * (uint64_t)(kTile * K) + (uint64_t)j * kTile + (i & (kTile - 1));
 }
 inline uint64_t padded_states(uint64_t n) { return (n + kTile - 1) / kTile * kTile; }
+// Element (i, j) under either state layout (0 = 64-state tiles, 1 = state-major).
+__host__ __device__ inline uint64_t state_elem(int layout, uint64_t i, int K, int j) {
+    return layout == 0 ? aosoa(i, K, j) : i * (uint64_t)K + (uint64_t)j;
+}
+// Interleave stride for k moduli: state-major layout pads to an even stride so
+// that two residues move per 8-byte access.
+inline int interleave_stride(int layout, int k) { return (layout == 1 && k > 1) ? (k + 1) / 2 * 2 : k; }
 
 struct CellLaunch {
     uint32_t* main;       // interleaved main residues
     uint32_t* blk;        // interleaved blocked residues (read and written in place)
-    const void* dp;       // device height table (uint32 or uint64 entries, kDpRows x kDpCols)
+    const void* dp;       // device packed height table (cell::Quad<uint32 or uint64>, kDpRows x kDpCols)
     uint64_t nblk;        // blocked words processed: [first, first + nblk)
     uint64_t first;       // first blocked word index (0 for a whole sweep)
     int W, p;             // frontier width, cell
     int read_blk;         // blocked input of this cell is valid
-    int K, off, kc;       // interleave width, first modulus, moduli in this launch
+    int K, off, kc;       // interleave stride, first modulus, moduli in this launch
+    int layout;           // 0: 64-state tiles (AoSoA), 1: state-major (AoS)
     uint32_t mod[kMaxKC];
 };
 
 // Launch one cell step (all blocked words, moduli off..off+kc-1).
 hipError_t launch_cell(const CellLaunch& a, bool wide_index, int threads, int blocks,
                        hipStream_t s);
-// Set main[aosoa(index,K,j)] = 1 mod moduli[j] for j < K.
-hipError_t launch_set_one(uint32_t* main, uint64_t index, int K, const uint32_t* mods_dev,
-                          hipStream_t s);
+// Set main[(index, j)] = 1 mod moduli[j] for j < k (interleave stride K).
+hipError_t launch_set_one(uint32_t* main, uint64_t index, int K, int k, int layout,
+                          const uint32_t* mods_dev, hipStream_t s);
 
 }  // namespace ofp

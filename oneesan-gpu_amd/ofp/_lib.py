@@ -25,7 +25,7 @@ OFP_MAX_N = 27
 # Exported names, in the order include/ofp.h declares them.
 EXPORTS = (
     "ofp_create", "ofp_plan", "ofp_solve", "ofp_apply_cells", "ofp_prepare",
-    "ofp_row_sweep", "ofp_read_main", "ofp_set_launch", "ofp_state_counts",
+    "ofp_row_sweep", "ofp_read_main", "ofp_set_launch", "ofp_set_layout", "ofp_state_counts",
     "ofp_last_error", "ofp_version", "ofp_destroy",
 )
 
@@ -43,6 +43,7 @@ class Config(ctypes.Structure):
         ("blocks_per_cu", ctypes.c_int),
         ("reserve_bytes", ctypes.c_size_t),
         ("use_graphs", ctypes.c_int),
+        ("layout", ctypes.c_int),
     ]
 
 
@@ -99,6 +100,7 @@ def load() -> ctypes.CDLL:
         "ofp_row_sweep": (ctypes.c_int32, [vp, ctypes.c_int, ctypes.POINTER(Stats)]),
         "ofp_read_main": (ctypes.c_int32, [vp, ctypes.c_uint64, u32p]),
         "ofp_set_launch": (ctypes.c_int32, [vp, ctypes.c_int, ctypes.c_int]),
+        "ofp_set_layout": (ctypes.c_int32, [vp, ctypes.c_int]),
         "ofp_state_counts": (ctypes.c_int32, [ctypes.c_int, u64p, u64p]),
         "ofp_last_error": (ctypes.c_char_p, [vp]),
         "ofp_version": (ctypes.c_char_p, []),
@@ -142,9 +144,10 @@ class Context:
     """One GPU context (a device, a stream, one device-resident state)."""
 
     def __init__(self, device: int = 0, threads: int = 0, blocks_per_cu: int = 0,
-                 reserve_bytes: int = 0, use_graphs: bool = True):
+                 reserve_bytes: int = 0, use_graphs: bool = True, layout: int | None = None):
         self._lib = load()
-        cfg = Config(device, threads, blocks_per_cu, reserve_bytes, 1 if use_graphs else 0)
+        cfg = Config(device, threads, blocks_per_cu, reserve_bytes, 1 if use_graphs else 0,
+                     0 if layout is None else layout + 1)
         h = ctypes.c_void_p()
         st = self._lib.ofp_create(ctypes.byref(cfg), ctypes.byref(h))
         if st != OFP_OK:
@@ -220,6 +223,10 @@ class Context:
 
     def set_launch(self, threads: int, blocks_per_cu: int) -> None:
         self._check(self._lib.ofp_set_launch(self._h, threads, blocks_per_cu), "ofp_set_launch")
+
+    def set_layout(self, layout: int) -> None:
+        """0 = 64-state tiles, 1 = state-major; invalidates the prepared state."""
+        self._check(self._lib.ofp_set_layout(self._h, layout), "ofp_set_layout")
 
 
 def version() -> str:

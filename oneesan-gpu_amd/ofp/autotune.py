@@ -1,0 +1,135 @@
+"""Launch-shape autotuner for the cell sweep.
+
+For a given (n, k) it prepares the device state once, then times row sweeps
+(cells W-1..1) for each candidate (threads per workgroup, workgroups per CU)
+and keeps the fastest.  Timing runs advance the state, which does not change
+the cost of a row.  Results persist in a JSON cache keyed by device name, n
+and k, so a solver or the serving loop can look them up without re-tuning.
+
+Reference analogue: scripts/run/autotune.py:28 (the reference tunes its
+frontier thread count and batch limits by re-running the solver binary);
+GRIDFP_FRONTIER_THREADS / GRIDFP_FRONTIER_BLOCKS (row6crt20.cu:1182,1192,1306-1307).
+"""
+from __future__ import annotations
+
+import json
+import os
+import time
+from pathlib import Path
+from typing import Iterable
+
+from . import _lib
+
+DEFAULT_CACHE = Path(os.environ.get("OFP_TUNE_CACHE", Path.home() / ".cache" / "ofp" / "tune.json"))
+# Table shipped with the package (written by this module on an MI355X and
+# committed); consulted when the user cache has no entry.
+PACKAGED = Path(__file__).resolve().parent / "tuned.json"
+THREADS = (128, 256)
+BLOCKS_PER_CU = (4, 8, 16, 32)
+LAYOUTS = (0, 1)
+
+
+def _key(n: int, k: int) -> str:
+    return f"n={n},k={k}"
+
+
+def load_cache(path: Path = DEFAULT_CACHE) -> dict:
+    try:
+        return json.loads(path.read_text())
+    except (OSError, ValueError):
+        return {}
+
+
+def save_cache(data: dict, path: Path = DEFAULT_CACHE) -> None:
+    path.parent.mkdir(parents=True, exist_ok=True)
+    tmp = path.with_suffix(".tmp")
+    tmp.write_text(json.dumps(data, indent=1, sort_keys=True))
+    os.replace(tmp, path)
+
+
+def lookup(n: int, k: int, path: Path = DEFAULT_CACHE) -> dict | None:
+    """Best known (threads, blocks_per_cu, layout) for (n, k): user cache first,
+    then the packaged table (exact key, else the nearest n with the same k>1-ness)."""
+    ent = load_cache(path).get(_key(n, k))
+    if ent:
+        return ent
+    pk = load_cache(PACKAGED)
+    if _key(n, k) in pk:
+        return pk[_key(n, k)]
+    cands = [(abs(int(key.split(",")[0][2:]) - n), v) for key, v in pk.items()
+             if (int(key.split(",")[1][2:]) > 1) == (k > 1)]
+    return min(cands, key=lambda t: t[0])[1] if cands else None
+
+
+def apply(ctx: "_lib.Context", n: int, k: int) -> dict | None:
+    """Set the best known launch shape and layout on ctx (before prepare/solve)."""
+    ent = lookup(n, k)
+    if ent:
+        ctx.set_layout(int(ent.get("layout", 1)))
+        ctx.set_launch(int(ent["threads"]), int(ent["blocks_per_cu"]))
+    return ent
+
+
+def tune(ctx: "_lib.Context", n: int, moduli: list[int], rows: int = 1,
+         threads: Iterable[int] = THREADS, blocks_per_cu: Iterable[int] = BLOCKS_PER_CU,
+         layouts: Iterable[int] = LAYOUTS, path: Path | None = DEFAULT_CACHE, log=None) -> dict:
+    """Time every (layout, launch shape); set the best on ctx; return the table."""
+    table = []
+    best = None
+    for lay in layouts:
+        ctx.set_layout(lay)
+        ctx.prepare(n, moduli)
+        for t in threads:
+            for b in blocks_per_cu:
+                ctx.set_launch(t, b)
+                ctx.row_sweep(1)  # capture the row graph for this shape
+                t0 = time.perf_counter()
+                st = ctx.row_sweep(rows)
+                el = time.perf_counter() - t0
+                row_ms = el / rows * 1e3
+                ent = {"layout": lay, "threads": t, "blocks_per_cu": b, "row_ms": row_ms,
+                       "kernel_ms_per_row": st["kernel_s"] / rows * 1e3}
+                table.append(ent)
+                if log:
+                    log(f"autotune n={n} k={len(moduli)} layout={lay} threads={t} blocks/cu={b}: "
+                        f"{row_ms:.2f} ms/row")
+                if best is None or row_ms < best["row_ms"]:
+                    best = ent
+    ctx.set_layout(best["layout"])
+    ctx.set_launch(best["threads"], best["blocks_per_cu"])
+    result = {"best": best, "table": table}
+    if path is not None:
+        data = load_cache(path)
+        data[_key(n, len(moduli))] = {"layout": best["layout"], "threads": best["threads"],
+                                      "blocks_per_cu": best["blocks_per_cu"],
+                                      "row_ms": best["row_ms"], "tuned_at": time.time()}
+        save_cache(data, path)
+    return result
+
+
+def main(argv=None) -> int:
+    import argparse
+    import sys
+    from .path_bound import PRIMES
+    ap = argparse.ArgumentParser(description="tune the cell-sweep launch shape")
+    ap.add_argument("--n", type=int, nargs="+", default=[22])
+    ap.add_argument("--k", type=int, nargs="+", default=[1, 14])
+    ap.add_argument("--rows", type=int, default=1)
+    ap.add_argument("--json", type=str, default="")
+    ap.add_argument("--cache", type=str, default=str(DEFAULT_CACHE))
+    args = ap.parse_args(argv)
+    out = {}
+    with _lib.Context() as ctx:
+        for n in args.n:
+            for k in args.k:
+                res = tune(ctx, n, PRIMES[:k], rows=args.rows, path=Path(args.cache),
+                           log=lambda m: print(m, file=sys.stderr, flush=True))
+                out[_key(n, k)] = res
+                print(json.dumps({"n": n, "k": k, "best": res["best"]}), flush=True)
+    if args.json:
+        Path(args.json).write_text(json.dumps(out, indent=1))
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -46,10 +46,19 @@ def test_every_cell_random_vectors(gpu_ctx, oracle, n):
                 assert np.array_equal(g_blk[:, j], eb), (n, p, mod, "blocked")
 
 
-@pytest.mark.parametrize("n,k", [(8, 1), (9, 14), (10, 17), (11, 5)])
-def test_whole_row_windows(gpu_ctx, oracle, n, k):
-    """A window of consecutive cells (two windows covering a row), k moduli."""
+@pytest.fixture(scope="module", params=[0, 1], ids=["tiles64", "statemajor"])
+def layout_ctx(request):
     import ofp
+    ctx = ofp.Context(layout=request.param)
+    yield ctx
+    ctx.close()
+
+
+@pytest.mark.parametrize("n,k", [(8, 1), (9, 14), (10, 17), (11, 5), (7, 3)])
+def test_whole_row_windows(layout_ctx, oracle, n, k):
+    """A window of consecutive cells (two windows covering a row), k moduli, both layouts."""
+    import ofp
+    gpu_ctx = layout_ctx
     nm, nb = ofp.state_counts(n)
     W = n + 1
     primes = [4294967291, 4294967279, 4294967231, 4294967197, 4294967189, 4294967161,
@@ -80,8 +89,9 @@ def test_small_n_against_exact_counts(gpu_ctx, oracle):
         assert got == [exact % m for m in mods], n
 
 
-def test_reference_residues(gpu_ctx):
-    """Every published reference residue with n <= 21 (n=22/23 are bench-sized)."""
+def test_reference_residues(layout_ctx):
+    """Every published reference residue with n <= 21 (n=22/23 are bench-sized), both layouts."""
+    gpu_ctx = layout_ctx
     g = load_golden("reference_residues.json")
     by_n: dict[int, list[tuple[int, int]]] = {}
     for r in g["residues"]:
