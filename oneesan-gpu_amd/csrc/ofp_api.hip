@@ -24,7 +24,7 @@ struct ofp_ctx {
     int device = 0;
     int cus = 256;
     int threads = 256;
-    int blocks_per_cu = 8;
+    int blocks_per_cu = 32;
     int use_graphs = 1;
     int layout = 1;  // 0: 64-state tiles, 1: state-major with even stride
     size_t reserve = 0;
@@ -287,7 +287,7 @@ ofp_status ofp_create(const ofp_config* cfg, ofp_ctx** out) {
 ofp_status ofp_set_launch(ofp_ctx* c, int threads, int blocks_per_cu) {
     if (!c) return OFP_E_ARG;
     if (threads == 0) threads = 256;
-    if (blocks_per_cu == 0) blocks_per_cu = 8;
+    if (blocks_per_cu == 0) blocks_per_cu = 32;
     if (threads < 64 || threads > 256 || (threads & 63) || blocks_per_cu < 1 || blocks_per_cu > 64)
         return fail(c, OFP_E_ARG, "threads must be 64..256 in steps of 64, blocks_per_cu 1..64");
     if (threads != c->threads || blocks_per_cu != c->blocks_per_cu) drop_graph(c);
