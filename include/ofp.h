@@ -111,12 +111,40 @@ ofp_status ofp_read_main(ofp_ctx* ctx, uint64_t index, uint32_t* out_k);
  * GRIDFP_FRONTIER_BLOCKS (row6crt20.cu:1182,1192,1306-1307). */
 ofp_status ofp_set_launch(ofp_ctx* ctx, int threads, int blocks_per_cu);
 
+/* Moduli per launch on the low cells: cells p <= low_cells advance at most
+ * low_kc moduli per launch (more launches, fewer registers per thread); the
+ * other cells advance up to 16.  Default low_cells = 0.  Set by the autotuner;
+ * no reference counterpart. */
+ofp_status ofp_set_low_cells(ofp_ctx* ctx, int low_cells, int low_kc);
+
 /* State layout for the next prepare/solve: 0 = tiles of 64 states (one
  * modulus of 64 consecutive states is 256 contiguous bytes), 1 = state-major
  * (the k residues of a state contiguous, stride padded to even).  No
  * reference counterpart (the reference runs one modulus at a time); set by
  * the autotuner.  Invalidates the prepared state. */
 ofp_status ofp_set_layout(ofp_ctx* ctx, int layout);
+
+/* ---- jobs: device-resident solves that run concurrently (serving) ----
+ * A job owns its own state (main + blocked, k moduli interleaved) and its own
+ * stream; row sweeps are enqueued asynchronously and the host polls for
+ * completion, so several jobs' kernels overlap on the GPU while the host
+ * admits and retires others.  No reference counterpart (the reference solves
+ * one request per process); used by the serving loop (ofp/serving.py). */
+typedef struct ofp_job ofp_job;
+typedef struct ofp_job_info {
+    int n, k, rows_total, rows_enqueued;
+    int idle;                      /* 1 when all enqueued work has finished */
+    uint64_t state_bytes;          /* device bytes held by the job */
+    uint64_t transitions_per_row;
+} ofp_job_info;
+/* Allocate and initialise the state; fails with OFP_E_MEMORY if it does not fit. */
+ofp_status ofp_job_create(ofp_ctx* ctx, int n, const uint32_t* moduli, int k, ofp_job** out);
+/* Enqueue `rows` more row sweeps (at most W in total); returns immediately. */
+ofp_status ofp_job_enqueue(ofp_job* job, int rows);
+ofp_status ofp_job_info_get(ofp_job* job, ofp_job_info* info);
+/* residues_out[j] = A007764(n) mod moduli[j]; waits for the job's stream. */
+ofp_status ofp_job_result(ofp_job* job, uint32_t* residues_out);
+void ofp_job_destroy(ofp_job* job);
 
 /* Counts only, no device needed: main / blocked word counts for n. */
 ofp_status ofp_state_counts(int n, uint64_t* main_states, uint64_t* blocked_states);

@@ -20,6 +20,25 @@
 
 using namespace ofp;
 
+// One device-resident frontier state: main + blocked arrays for (n, moduli),
+// the stream its sweeps run on, and the row graph captured for it.
+struct Sweep {
+    uint32_t* main = nullptr;
+    uint32_t* blk = nullptr;
+    size_t main_cap = 0, blk_cap = 0;  // bytes allocated
+    uint32_t* mods_dev = nullptr;
+    int mods_cap = 0;
+    int n = 0, W = 0, K = 0, Kp = 0;  // k moduli, interleave stride Kp
+    int layout = -1;                  // layout the arrays are written in
+    std::vector<uint32_t> mods;
+    uint64_t nmain = 0, nblk = 0;
+    hipStream_t stream = nullptr;
+    hipGraphExec_t row_exec = nullptr;
+    hipGraph_t row_graph = nullptr;
+    std::string row_key;
+    bool async_alloc = false;  // buffers come from the stream-ordered allocator (jobs)
+};
+
 struct ofp_ctx {
     int device = 0;
     int cus = 256;
@@ -27,30 +46,28 @@ struct ofp_ctx {
     int blocks_per_cu = 32;
     int use_graphs = 1;
     int layout = 1;  // 0: 64-state tiles, 1: state-major with even stride
+    int low_cells = 0;       // cells p <= low_cells launch at most low_kc moduli at a time
+    int low_kc = kMaxKC;
     size_t reserve = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     void* dp32 = nullptr;  // packed uint32 height table (n <= 22)
     void* dp64 = nullptr;  // packed uint64 height table
     std::string err;
-    // device-resident state
-    uint32_t* main = nullptr;
-    uint32_t* blk = nullptr;
-    size_t main_cap = 0, blk_cap = 0;  // bytes allocated
-    uint32_t* mods_dev = nullptr;
-    int n = 0, W = 0, K = 0, Kp = 0;  // k moduli, interleave stride Kp
-    int state_layout = -1;           // layout the current buffers were written in
-    std::vector<uint32_t> mods;
-    uint64_t nmain = 0, nblk = 0;
-    // row graph cache
-    hipGraphExec_t row_exec = nullptr;
-    hipGraph_t row_graph = nullptr;
-    std::string row_key;
+    Sweep st;              // the context's own state (solve / prepare / row_sweep)
+    int jobs = 0;          // live jobs (serving)
+};
+
+struct ofp_job {
+    ofp_ctx* ctx = nullptr;
+    Sweep sw;
+    int rows_enqueued = 0;
+    hipEvent_t done = nullptr;
 };
 
 namespace {
 
-const char* kVersion = "ofp-mi355x 0.1 (per-cell in-place orbit sweep, gfx950)";
+const char* kVersion = "ofp-mi355x 0.2 (per-cell in-place orbit sweep, gfx950)";
 
 ofp_status fail(ofp_ctx* c, ofp_status code, const std::string& msg) {
     if (c) c->err = msg;
@@ -79,12 +96,46 @@ bool counts(int n, uint64_t* m, uint64_t* b) {
     return true;
 }
 
-void drop_graph(ofp_ctx* c) {
-    if (c->row_exec) (void)hipGraphExecDestroy(c->row_exec);
-    if (c->row_graph) (void)hipGraphDestroy(c->row_graph);
-    c->row_exec = nullptr;
-    c->row_graph = nullptr;
-    c->row_key.clear();
+uint64_t state_bytes(int layout, int n, int k) {
+    uint64_t nm, nb;
+    if (!counts(n, &nm, &nb)) return 0;
+    return (padded_states(nm) + padded_states(nb ? nb : 1)) * (uint64_t)interleave_stride(layout, k) * 4;
+}
+
+void drop_graph(Sweep& s) {
+    if (s.row_exec) (void)hipGraphExecDestroy(s.row_exec);
+    if (s.row_graph) (void)hipGraphDestroy(s.row_graph);
+    s.row_exec = nullptr;
+    s.row_graph = nullptr;
+    s.row_key.clear();
+}
+
+// Device buffers of a sweep: jobs use the stream-ordered allocator so that
+// creating or retiring one job never waits for the kernels of the others.
+hipError_t sweep_alloc(Sweep& s, void** p, size_t bytes) {
+    if (s.async_alloc) {
+        hipError_t e = hipMallocAsync(p, bytes, s.stream);
+        if (e != hipErrorNotSupported) return e;
+        (void)hipGetLastError();
+        s.async_alloc = false;
+    }
+    return hipMalloc(p, bytes);
+}
+void sweep_free(Sweep& s, void* p) {
+    if (!p) return;
+    if (s.async_alloc) (void)hipFreeAsync(p, s.stream);
+    else (void)hipFree(p);
+}
+
+void free_sweep(Sweep& s) {
+    drop_graph(s);
+    sweep_free(s, s.main);
+    sweep_free(s, s.blk);
+    sweep_free(s, s.mods_dev);
+    s.main = s.blk = s.mods_dev = nullptr;
+    s.main_cap = s.blk_cap = 0;
+    s.mods_cap = 0;
+    s.layout = -1;
 }
 
 int grid_blocks(const ofp_ctx* c, uint64_t nblk) {
@@ -95,8 +146,8 @@ int grid_blocks(const ofp_ctx* c, uint64_t nblk) {
     return (int)std::max<uint64_t>(b, 8);
 }
 
-// Ensure device buffers for (n, moduli) and record them as the current state.
-ofp_status ensure_state(ofp_ctx* c, int n, const uint32_t* moduli, int k) {
+// Ensure device buffers of s for (n, moduli) in the context's current layout.
+ofp_status ensure_state(ofp_ctx* c, Sweep& s, int n, const uint32_t* moduli, int k) {
     uint64_t nm, nb;
     if (!counts(n, &nm, &nb)) return fail(c, OFP_E_RANGE, "n must be in 1..27");
     if (k < 1 || k > 1024) return fail(c, OFP_E_ARG, "number of moduli must be in 1..1024");
@@ -106,62 +157,70 @@ ofp_status ensure_state(ofp_ctx* c, int n, const uint32_t* moduli, int k) {
     size_t mb = padded_states(nm) * (size_t)Kp * sizeof(uint32_t);
     size_t bb = padded_states(nb ? nb : 1) * (size_t)Kp * sizeof(uint32_t);
     OFP_HIP(c, hipSetDevice(c->device), "hipSetDevice");
-    if (mb > c->main_cap || bb > c->blk_cap) {
-        drop_graph(c);
-        if (c->main) (void)hipFree(c->main);
-        if (c->blk) (void)hipFree(c->blk);
-        c->main = nullptr;
-        c->blk = nullptr;
-        c->main_cap = c->blk_cap = 0;
+    if (mb > s.main_cap || bb > s.blk_cap) {
+        drop_graph(s);
+        sweep_free(s, s.main);
+        sweep_free(s, s.blk);
+        s.main = s.blk = nullptr;
+        s.main_cap = s.blk_cap = 0;
         size_t fr = 0, tot = 0;
         OFP_HIP(c, hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
         if (mb + bb + c->reserve > fr)
             return fail(c, OFP_E_MEMORY, "insufficient HBM: need " + std::to_string(mb + bb) +
                                              " bytes, free " + std::to_string(fr));
-        OFP_HIP(c, hipMalloc(&c->main, mb), "hipMalloc main state");
-        OFP_HIP(c, hipMalloc(&c->blk, bb), "hipMalloc blocked state");
-        c->main_cap = mb;
-        c->blk_cap = bb;
+        OFP_HIP(c, sweep_alloc(s, (void**)&s.main, mb), "allocate main state");
+        OFP_HIP(c, sweep_alloc(s, (void**)&s.blk, bb), "allocate blocked state");
+        s.main_cap = mb;
+        s.blk_cap = bb;
     }
-    if (c->mods_dev == nullptr || (int)c->mods.size() < k) {
-        if (c->mods_dev) (void)hipFree(c->mods_dev);
-        c->mods_dev = nullptr;
-        OFP_HIP(c, hipMalloc(&c->mods_dev, std::max(k, 16) * sizeof(uint32_t)), "hipMalloc moduli");
+    if (s.mods_cap < k) {
+        sweep_free(s, s.mods_dev);
+        s.mods_dev = nullptr;
+        s.mods_cap = std::max(k, 16);
+        OFP_HIP(c, sweep_alloc(s, (void**)&s.mods_dev, s.mods_cap * sizeof(uint32_t)), "allocate moduli");
     }
-    c->mods.assign(moduli, moduli + k);
-    OFP_HIP(c, hipMemcpy(c->mods_dev, moduli, k * sizeof(uint32_t), hipMemcpyHostToDevice),
+    s.mods.assign(moduli, moduli + k);
+    // stream-ordered copy from pageable memory (staged by the runtime before returning)
+    OFP_HIP(c, hipMemcpyAsync(s.mods_dev, s.mods.data(), k * sizeof(uint32_t), hipMemcpyHostToDevice, s.stream),
             "copy moduli");
-    c->n = n;
-    c->W = n + 1;
-    c->K = k;
-    c->Kp = Kp;
-    c->state_layout = c->layout;
-    c->nmain = nm;
-    c->nblk = nb;
+    OFP_HIP(c, hipStreamSynchronize(s.stream), "copy moduli sync");
+    s.n = n;
+    s.W = n + 1;
+    s.K = k;
+    s.Kp = Kp;
+    s.layout = c->layout;
+    s.nmain = nm;
+    s.nblk = nb;
     return OFP_OK;
 }
 
-// Enqueue cells p_hi..p_lo on the context stream.
-ofp_status enqueue_cells(ofp_ctx* c, int p_hi, int p_lo, bool first_reads_blk, int* launches) {
-    const bool wide = c->W > 23;
-    const int blocks = grid_blocks(c, c->nblk);
+// Enqueue cells p_hi..p_lo of s on s.stream.
+ofp_status enqueue_cells(ofp_ctx* c, Sweep& s, int p_hi, int p_lo, bool first_reads_blk, int* launches) {
+    const bool wide = s.W > 23;
+    const int blocks = grid_blocks(c, s.nblk);
     for (int p = p_hi; p >= p_lo; --p) {
-        for (int off = 0; off < c->K; off += kMaxKC) {
+        // moduli per launch: all (up to 16) at once, or fewer on the low cells,
+        // whose scattered closure reads are latency-bound (fewer registers per
+        // thread -> more waves in flight).  Chunks keep an even offset.
+        int step = kMaxKC;
+        if (p <= c->low_cells && c->low_kc < step) step = c->low_kc;
+        if (s.layout == 1 && s.K > 1 && (step & 1)) step = std::max(2, step - 1);
+        for (int off = 0; off < s.K; off += step) {
             CellLaunch a{};
-            a.main = c->main;
-            a.blk = c->blk;
+            a.main = s.main;
+            a.blk = s.blk;
             a.dp = wide ? c->dp64 : c->dp32;
-            a.nblk = c->nblk;
+            a.nblk = s.nblk;
             a.first = 0;
-            a.layout = c->layout;
-            a.W = c->W;
+            a.layout = s.layout;
+            a.W = s.W;
             a.p = p;
             a.read_blk = (p == p_hi) ? (first_reads_blk ? 1 : 0) : 1;
-            a.K = c->Kp;
+            a.K = s.Kp;
             a.off = off;
-            a.kc = std::min(kMaxKC, c->K - off);
-            for (int j = 0; j < a.kc; ++j) a.mod[j] = c->mods[off + j];
-            hipError_t e = launch_cell(a, wide, c->threads, blocks, c->stream);
+            a.kc = std::min(step, s.K - off);
+            for (int j = 0; j < a.kc; ++j) a.mod[j] = s.mods[off + j];
+            hipError_t e = launch_cell(a, wide, c->threads, blocks, s.stream);
             if (e != hipSuccess) return hip_fail(c, e, "cell kernel launch");
             if (launches) ++*launches;
         }
@@ -169,47 +228,68 @@ ofp_status enqueue_cells(ofp_ctx* c, int p_hi, int p_lo, bool first_reads_blk, i
     return OFP_OK;
 }
 
-// One full row (cells W-1..1, blocked input of cell W-1 is zero), via a cached graph.
-ofp_status run_rows(ofp_ctx* c, int rows, int* launches) {
+// `rows` full rows (cells W-1..1; the blocked input of cell W-1 is zero), via a
+// hipGraph captured once per (state, launch shape) and replayed.
+ofp_status run_rows(ofp_ctx* c, Sweep& s, int rows, int* launches) {
     if (!c->use_graphs) {
         for (int r = 0; r < rows; ++r) {
-            ofp_status s = enqueue_cells(c, c->W - 1, 1, false, launches);
-            if (s) return s;
+            ofp_status st = enqueue_cells(c, s, s.W - 1, 1, false, launches);
+            if (st) return st;
         }
         return OFP_OK;
     }
     char key[256];
-    snprintf(key, sizeof key, "%p/%p/%d/%d/%d/%d/%d", (void*)c->main, (void*)c->blk, c->n, c->K,
-             c->threads, c->blocks_per_cu, c->layout);
+    snprintf(key, sizeof key, "%p/%p/%d/%d/%d/%d/%d/%d/%d", (void*)s.main, (void*)s.blk, s.n, s.K,
+             c->threads, c->blocks_per_cu, s.layout, c->low_cells, c->low_kc);
     std::string k = key;
-    for (uint32_t m : c->mods) k += "/" + std::to_string(m);
-    if (k != c->row_key || !c->row_exec) {
-        drop_graph(c);
-        OFP_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+    for (uint32_t m : s.mods) k += "/" + std::to_string(m);
+    if (k != s.row_key || !s.row_exec) {
+        drop_graph(s);
+        OFP_HIP(c, hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal), "begin capture");
         int dummy = 0;
-        ofp_status s = enqueue_cells(c, c->W - 1, 1, false, &dummy);
+        ofp_status st = enqueue_cells(c, s, s.W - 1, 1, false, &dummy);
         hipGraph_t g = nullptr;
-        hipError_t e = hipStreamEndCapture(c->stream, &g);
-        if (s) { if (g) (void)hipGraphDestroy(g); return s; }
+        hipError_t e = hipStreamEndCapture(s.stream, &g);
+        if (st) {
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+        }
         if (e != hipSuccess) return hip_fail(c, e, "end capture");
-        c->row_graph = g;
-        OFP_HIP(c, hipGraphInstantiate(&c->row_exec, g, nullptr, nullptr, 0), "graph instantiate");
-        c->row_key = k;
+        s.row_graph = g;
+        OFP_HIP(c, hipGraphInstantiate(&s.row_exec, g, nullptr, nullptr, 0), "graph instantiate");
+        s.row_key = k;
+    }
+    int per_row = 0;
+    for (int p = s.W - 1; p >= 1; --p) {
+        int step = (p <= c->low_cells && c->low_kc < kMaxKC) ? c->low_kc : kMaxKC;
+        if (s.layout == 1 && s.K > 1 && (step & 1)) step = std::max(2, step - 1);
+        per_row += (s.K + step - 1) / step;
     }
     for (int r = 0; r < rows; ++r) {
-        OFP_HIP(c, hipGraphLaunch(c->row_exec, c->stream), "graph launch");
-        if (launches) *launches += (c->W - 1) * ((c->K + kMaxKC - 1) / kMaxKC);
+        OFP_HIP(c, hipGraphLaunch(s.row_exec, s.stream), "graph launch");
+        if (launches) *launches += per_row;
     }
     return OFP_OK;
 }
 
-ofp_status init_state(ofp_ctx* c) {
-    OFP_HIP(c, hipMemsetAsync(c->main, 0, padded_states(c->nmain) * (size_t)c->Kp * 4, c->stream),
-            "zero main state");
-    // start word: R at position W-1, rank = H_DP[W-1][1] (fullmate_dropN_batch.cu:508)
-    uint64_t start = dp_at(c->W - 1, 1);
-    hipError_t e = launch_set_one(c->main, start, c->Kp, c->K, c->layout, c->mods_dev, c->stream);
+// Zero main and place the start word R at position W-1 (rank H_DP[W-1][1]) with
+// residue 1 (fullmate_dropN_batch.cu:508-511).  The blocked array needs no
+// initialisation: cell W-1 does not read it and writes every entry.
+ofp_status init_state(ofp_ctx* c, Sweep& s) {
+    OFP_HIP(c, hipMemsetAsync(s.main, 0, padded_states(s.nmain) * (size_t)s.Kp * 4, s.stream), "zero main state");
+    uint64_t start = dp_at(s.W - 1, 1);
+    hipError_t e = launch_set_one(s.main, start, s.Kp, s.K, s.layout, s.mods_dev, s.stream);
     if (e != hipSuccess) return hip_fail(c, e, "init kernel");
+    return OFP_OK;
+}
+
+ofp_status read_main(ofp_ctx* c, Sweep& s, uint64_t index, uint32_t* out_k) {
+    if (!s.main) return fail(c, OFP_E_ARG, "no prepared state");
+    if (index >= s.nmain) return fail(c, OFP_E_ARG, "index out of range");
+    OFP_HIP(c, hipStreamSynchronize(s.stream), "sync");
+    for (int j = 0; j < s.K; ++j)
+        OFP_HIP(c, hipMemcpy(out_k + j, s.main + state_elem(s.layout, index, s.Kp, j), 4, hipMemcpyDeviceToHost),
+                "read residue");
     return OFP_OK;
 }
 
@@ -242,7 +322,7 @@ ofp_status ofp_create(const ofp_config* cfg, ofp_ctx** out) {
         c->use_graphs = cfg->use_graphs;
         if (cfg->layout == 1 || cfg->layout == 2) c->layout = cfg->layout - 1;
     }
-    if (c->threads <= 0 || c->threads > 256 || (c->threads & 63) || c->blocks_per_cu <= 0) {
+    if (c->threads < 64 || c->threads > 256 || (c->threads & 63) || c->blocks_per_cu <= 0) {
         delete c;
         return OFP_E_ARG;
     }
@@ -253,6 +333,10 @@ ofp_status ofp_create(const ofp_config* cfg, ofp_ctx** out) {
     }
     auto bad = [&](hipError_t e) {
         if (e == hipSuccess) return false;
+        if (c->dp32) (void)hipFree(c->dp32);
+        if (c->dp64) (void)hipFree(c->dp64);
+        if (c->ev0) (void)hipEventDestroy(c->ev0);
+        if (c->ev1) (void)hipEventDestroy(c->ev1);
         if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
         return true;
@@ -264,16 +348,16 @@ ofp_status ofp_create(const ofp_config* cfg, ofp_ctx** out) {
     if (!c->reserve) {
         // reference default: clamp(total/32, 256 MiB, 8 GiB) (fullmate_dropN_batch.cu:499)
         size_t tot = prop.totalGlobalMem;
-        size_t r = std::min<size_t>(8ull << 30, std::max<size_t>(256ull << 20, tot / 32));
-        c->reserve = r;
+        c->reserve = std::min<size_t>(8ull << 30, std::max<size_t>(256ull << 20, tot / 32));
     }
     if (bad(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking))) return OFP_E_RUNTIME;
+    c->st.stream = c->stream;
     if (bad(hipEventCreate(&c->ev0)) || bad(hipEventCreate(&c->ev1))) return OFP_E_RUNTIME;
     std::vector<unsigned char> q32(64 * 1024), q64(64 * 1024);
     size_t b32 = pack_height_table(false, q32.data(), q32.size());
     size_t b64 = pack_height_table(true, q64.data(), q64.size());
     if (!b32 || !b64) {
-        delete c;
+        (void)bad(hipErrorInvalidValue);
         return OFP_E_RUNTIME;
     }
     if (bad(hipMalloc(&c->dp32, b32)) || bad(hipMalloc(&c->dp64, b64))) return OFP_E_RUNTIME;
@@ -290,9 +374,19 @@ ofp_status ofp_set_launch(ofp_ctx* c, int threads, int blocks_per_cu) {
     if (blocks_per_cu == 0) blocks_per_cu = 32;
     if (threads < 64 || threads > 256 || (threads & 63) || blocks_per_cu < 1 || blocks_per_cu > 64)
         return fail(c, OFP_E_ARG, "threads must be 64..256 in steps of 64, blocks_per_cu 1..64");
-    if (threads != c->threads || blocks_per_cu != c->blocks_per_cu) drop_graph(c);
+    if (threads != c->threads || blocks_per_cu != c->blocks_per_cu) drop_graph(c->st);
     c->threads = threads;
     c->blocks_per_cu = blocks_per_cu;
+    return OFP_OK;
+}
+
+ofp_status ofp_set_low_cells(ofp_ctx* c, int low_cells, int low_kc) {
+    if (!c) return OFP_E_ARG;
+    if (low_cells < 0 || low_cells > kMaxW || low_kc < 1 || low_kc > kMaxKC)
+        return fail(c, OFP_E_ARG, "low_cells must be 0..28 and low_kc 1..16");
+    if (low_cells != c->low_cells || low_kc != c->low_kc) drop_graph(c->st);
+    c->low_cells = low_cells;
+    c->low_kc = low_kc;
     return OFP_OK;
 }
 
@@ -300,9 +394,9 @@ ofp_status ofp_set_layout(ofp_ctx* c, int layout) {
     if (!c) return OFP_E_ARG;
     if (layout != 0 && layout != 1) return fail(c, OFP_E_ARG, "layout must be 0 (64-state tiles) or 1 (state-major)");
     if (layout != c->layout) {
-        drop_graph(c);
+        drop_graph(c->st);
         c->layout = layout;
-        c->state_layout = -1;  // current contents no longer readable; prepare again
+        c->st.layout = -1;  // current contents no longer readable; prepare again
     }
     return OFP_OK;
 }
@@ -314,25 +408,23 @@ ofp_status ofp_plan(ofp_ctx* c, int n, int k, ofp_plan_info* out) {
     if (k < 1) return fail(c, OFP_E_ARG, "k must be >= 1");
     out->main_states = nm;
     out->blocked_states = nb;
-    out->state_bytes = (padded_states(nm) + padded_states(nb ? nb : 1)) *
-                       (uint64_t)interleave_stride(c->layout, k) * 4;
+    out->state_bytes = state_bytes(c->layout, n, k);
     const uint64_t W = n + 1;
     out->transitions = W * (W - 1) * (nm + nb) * (uint64_t)k;
     size_t fr = 0, tot = 0;
     OFP_HIP(c, hipSetDevice(c->device), "hipSetDevice");
     OFP_HIP(c, hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
-    // memory already held by this context's state can be reused
-    uint64_t reusable = c->main_cap + c->blk_cap;
-    out->free_bytes = fr + reusable;
+    // memory already held by this context's own state can be reused
+    out->free_bytes = fr + c->st.main_cap + c->st.blk_cap;
     out->fits = out->state_bytes + c->reserve <= out->free_bytes;
     return OFP_OK;
 }
 
 ofp_status ofp_prepare(ofp_ctx* c, int n, const uint32_t* moduli, int k) {
     if (!c || !moduli) return OFP_E_ARG;
-    ofp_status s = ensure_state(c, n, moduli, k);
+    ofp_status s = ensure_state(c, c->st, n, moduli, k);
     if (s) return s;
-    s = init_state(c);
+    s = init_state(c, c->st);
     if (s) return s;
     OFP_HIP(c, hipStreamSynchronize(c->stream), "prepare sync");
     return OFP_OK;
@@ -340,52 +432,48 @@ ofp_status ofp_prepare(ofp_ctx* c, int n, const uint32_t* moduli, int k) {
 
 ofp_status ofp_row_sweep(ofp_ctx* c, int rows, ofp_stats* st) {
     if (!c || rows < 0) return OFP_E_ARG;
-    if (!c->main || c->state_layout != c->layout) return fail(c, OFP_E_ARG, "no prepared state");
+    Sweep& s = c->st;
+    if (!s.main || s.layout != c->layout) return fail(c, OFP_E_ARG, "no prepared state");
     int launches = 0;
     auto t0 = std::chrono::steady_clock::now();
-    OFP_HIP(c, hipEventRecord(c->ev0, c->stream), "event record");
-    ofp_status s = run_rows(c, rows, &launches);
-    if (s) return s;
-    OFP_HIP(c, hipEventRecord(c->ev1, c->stream), "event record");
+    OFP_HIP(c, hipEventRecord(c->ev0, s.stream), "event record");
+    ofp_status r = run_rows(c, s, rows, &launches);
+    if (r) return r;
+    OFP_HIP(c, hipEventRecord(c->ev1, s.stream), "event record");
     OFP_HIP(c, hipEventSynchronize(c->ev1), "row sweep sync");
     float ms = 0;
     OFP_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1), "event time");
     if (st) {
         st->wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         st->kernel_s = ms * 1e-3;
-        st->main_states = c->nmain;
-        st->blocked_states = c->nblk;
+        st->main_states = s.nmain;
+        st->blocked_states = s.nblk;
         st->rows = rows;
-        st->cells_per_row = c->W - 1;
+        st->cells_per_row = s.W - 1;
         st->launches = launches;
-        st->transitions = (uint64_t)rows * (c->W - 1) * (c->nmain + c->nblk) * (uint64_t)c->K;
+        st->transitions = (uint64_t)rows * (s.W - 1) * (s.nmain + s.nblk) * (uint64_t)s.K;
     }
     return OFP_OK;
 }
 
 ofp_status ofp_read_main(ofp_ctx* c, uint64_t index, uint32_t* out_k) {
-    if (!c || !out_k || !c->main) return OFP_E_ARG;
-    if (index >= c->nmain) return fail(c, OFP_E_ARG, "index out of range");
-    OFP_HIP(c, hipStreamSynchronize(c->stream), "sync");
-    for (int j = 0; j < c->K; ++j)
-        OFP_HIP(c, hipMemcpy(out_k + j, c->main + state_elem(c->state_layout, index, c->Kp, j), 4,
-                             hipMemcpyDeviceToHost),
-                "read residue");
-    return OFP_OK;
+    if (!c || !out_k) return OFP_E_ARG;
+    if (c->st.layout != c->layout) return fail(c, OFP_E_ARG, "no prepared state");
+    return read_main(c, c->st, index, out_k);
 }
 
 ofp_status ofp_solve(ofp_ctx* c, int n, const uint32_t* moduli, int k, uint32_t* residues_out,
                      ofp_stats* st) {
     if (!c || !moduli || !residues_out) return OFP_E_ARG;
     auto t0 = std::chrono::steady_clock::now();
-    ofp_status s = ensure_state(c, n, moduli, k);
+    ofp_status s = ensure_state(c, c->st, n, moduli, k);
     if (s) return s;
-    s = init_state(c);
+    s = init_state(c, c->st);
     if (s) return s;
     ofp_stats rs{};
-    s = ofp_row_sweep(c, c->W, &rs);
+    s = ofp_row_sweep(c, c->st.W, &rs);
     if (s) return s;
-    s = ofp_read_main(c, 0, residues_out);  // rank_full(R) = 0 (fullmate_dropN_batch.cu:508,514)
+    s = read_main(c, c->st, 0, residues_out);  // rank_full(R) = 0 (fullmate_dropN_batch.cu:508,514)
     if (s) return s;
     if (st) {
         *st = rs;
@@ -400,38 +488,110 @@ ofp_status ofp_apply_cells(ofp_ctx* c, int n, const uint32_t* moduli, int k, uin
     if (n < 1 || n > OFP_MAX_N) return fail(c, OFP_E_RANGE, "n must be in 1..27");
     const int W = n + 1;
     if (p_hi > W - 1 || p_lo < 1 || p_hi < p_lo) return fail(c, OFP_E_ARG, "need W-1 >= p_hi >= p_lo >= 1");
-    ofp_status s = ensure_state(c, n, moduli, k);
+    Sweep& sw = c->st;
+    ofp_status s = ensure_state(c, sw, n, moduli, k);
     if (s) return s;
-    // host state-major -> device tiles
-    const int Kp = c->Kp, lay = c->layout;
-    std::vector<uint32_t> tm(padded_states(c->nmain) * (size_t)Kp, 0),
-        tb(padded_states(c->nblk ? c->nblk : 1) * (size_t)Kp, 0);
-    for (uint64_t i = 0; i < c->nmain; ++i)
+    // host state-major (index*k + j) <-> device layout
+    const int Kp = sw.Kp, lay = sw.layout;
+    std::vector<uint32_t> tm(padded_states(sw.nmain) * (size_t)Kp, 0),
+        tb(padded_states(sw.nblk ? sw.nblk : 1) * (size_t)Kp, 0);
+    for (uint64_t i = 0; i < sw.nmain; ++i)
         for (int j = 0; j < k; ++j) tm[state_elem(lay, i, Kp, j)] = main_io[i * k + j];
-    for (uint64_t i = 0; i < c->nblk; ++i)
+    for (uint64_t i = 0; i < sw.nblk; ++i)
         for (int j = 0; j < k; ++j) tb[state_elem(lay, i, Kp, j)] = blocked_io[i * k + j];
-    OFP_HIP(c, hipMemcpy(c->main, tm.data(), tm.size() * 4, hipMemcpyHostToDevice), "upload main");
-    OFP_HIP(c, hipMemcpy(c->blk, tb.data(), tb.size() * 4, hipMemcpyHostToDevice), "upload blocked");
-    s = enqueue_cells(c, p_hi, p_lo, blocked_valid != 0, nullptr);
+    OFP_HIP(c, hipMemcpy(sw.main, tm.data(), tm.size() * 4, hipMemcpyHostToDevice), "upload main");
+    OFP_HIP(c, hipMemcpy(sw.blk, tb.data(), tb.size() * 4, hipMemcpyHostToDevice), "upload blocked");
+    s = enqueue_cells(c, sw, p_hi, p_lo, blocked_valid != 0, nullptr);
     if (s) return s;
-    OFP_HIP(c, hipStreamSynchronize(c->stream), "apply sync");
-    OFP_HIP(c, hipMemcpy(tm.data(), c->main, tm.size() * 4, hipMemcpyDeviceToHost), "download main");
-    OFP_HIP(c, hipMemcpy(tb.data(), c->blk, tb.size() * 4, hipMemcpyDeviceToHost), "download blocked");
-    for (uint64_t i = 0; i < c->nmain; ++i)
+    OFP_HIP(c, hipStreamSynchronize(sw.stream), "apply sync");
+    OFP_HIP(c, hipMemcpy(tm.data(), sw.main, tm.size() * 4, hipMemcpyDeviceToHost), "download main");
+    OFP_HIP(c, hipMemcpy(tb.data(), sw.blk, tb.size() * 4, hipMemcpyDeviceToHost), "download blocked");
+    for (uint64_t i = 0; i < sw.nmain; ++i)
         for (int j = 0; j < k; ++j) main_io[i * k + j] = tm[state_elem(lay, i, Kp, j)];
     // after cell 1 nothing is blocked; the device buffer is simply not written
-    for (uint64_t i = 0; i < c->nblk; ++i)
+    for (uint64_t i = 0; i < sw.nblk; ++i)
         for (int j = 0; j < k; ++j) blocked_io[i * k + j] = (p_lo == 1) ? 0u : tb[state_elem(lay, i, Kp, j)];
     return OFP_OK;
+}
+
+// ---- jobs: independent device-resident solves that overlap on the GPU ----
+
+ofp_status ofp_job_create(ofp_ctx* c, int n, const uint32_t* moduli, int k, ofp_job** out) {
+    if (!c || !moduli || !out) return OFP_E_ARG;
+    *out = nullptr;
+    ofp_job* j = new ofp_job();
+    j->ctx = c;
+    auto cleanup = [&](ofp_status st) {
+        free_sweep(j->sw);
+        if (j->sw.stream) (void)hipStreamDestroy(j->sw.stream);
+        if (j->done) (void)hipEventDestroy(j->done);
+        delete j;
+        return st;
+    };
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return cleanup(hip_fail(c, e, "hipSetDevice"));
+    e = hipStreamCreateWithFlags(&j->sw.stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return cleanup(hip_fail(c, e, "job stream"));
+    e = hipEventCreateWithFlags(&j->done, hipEventDisableTiming);
+    if (e != hipSuccess) return cleanup(hip_fail(c, e, "job event"));
+    j->sw.async_alloc = true;
+    ofp_status st = ensure_state(c, j->sw, n, moduli, k);
+    if (st) return cleanup(st);
+    st = init_state(c, j->sw);
+    if (st) return cleanup(st);
+    e = hipEventRecord(j->done, j->sw.stream);
+    if (e != hipSuccess) return cleanup(hip_fail(c, e, "job event record"));
+    ++c->jobs;
+    *out = j;
+    return OFP_OK;
+}
+
+ofp_status ofp_job_enqueue(ofp_job* j, int rows) {
+    if (!j || rows < 0) return OFP_E_ARG;
+    ofp_ctx* c = j->ctx;
+    if (j->rows_enqueued + rows > j->sw.W) return fail(c, OFP_E_ARG, "job has only W rows");
+    ofp_status st = run_rows(c, j->sw, rows, nullptr);
+    if (st) return st;
+    j->rows_enqueued += rows;
+    OFP_HIP(c, hipEventRecord(j->done, j->sw.stream), "job event record");
+    return OFP_OK;
+}
+
+ofp_status ofp_job_info_get(ofp_job* j, ofp_job_info* info) {
+    if (!j || !info) return OFP_E_ARG;
+    hipError_t q = hipEventQuery(j->done);
+    if (q != hipSuccess && q != hipErrorNotReady) return hip_fail(j->ctx, q, "job query");
+    info->n = j->sw.n;
+    info->k = j->sw.K;
+    info->rows_total = j->sw.W;
+    info->rows_enqueued = j->rows_enqueued;
+    info->idle = (q == hipSuccess) ? 1 : 0;
+    info->state_bytes = j->sw.main_cap + j->sw.blk_cap;
+    info->transitions_per_row = (uint64_t)(j->sw.W - 1) * (j->sw.nmain + j->sw.nblk) * (uint64_t)j->sw.K;
+    return OFP_OK;
+}
+
+ofp_status ofp_job_result(ofp_job* j, uint32_t* residues_out) {
+    if (!j || !residues_out) return OFP_E_ARG;
+    if (j->rows_enqueued != j->sw.W) return fail(j->ctx, OFP_E_ARG, "job has not run all W rows");
+    return read_main(j->ctx, j->sw, 0, residues_out);
+}
+
+void ofp_job_destroy(ofp_job* j) {
+    if (!j) return;
+    (void)hipSetDevice(j->ctx->device);
+    if (j->sw.stream) (void)hipStreamSynchronize(j->sw.stream);
+    free_sweep(j->sw);
+    if (j->sw.stream) (void)hipStreamDestroy(j->sw.stream);
+    if (j->done) (void)hipEventDestroy(j->done);
+    --j->ctx->jobs;
+    delete j;
 }
 
 void ofp_destroy(ofp_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
-    drop_graph(c);
-    if (c->main) (void)hipFree(c->main);
-    if (c->blk) (void)hipFree(c->blk);
-    if (c->mods_dev) (void)hipFree(c->mods_dev);
+    free_sweep(c->st);
     if (c->dp32) (void)hipFree(c->dp32);
     if (c->dp64) (void)hipFree(c->dp64);
     if (c->ev0) (void)hipEventDestroy(c->ev0);

@@ -25,8 +25,9 @@ OFP_MAX_N = 27
 # Exported names, in the order include/ofp.h declares them.
 EXPORTS = (
     "ofp_create", "ofp_plan", "ofp_solve", "ofp_apply_cells", "ofp_prepare",
-    "ofp_row_sweep", "ofp_read_main", "ofp_set_launch", "ofp_set_layout", "ofp_state_counts",
+    "ofp_row_sweep", "ofp_read_main", "ofp_set_launch", "ofp_set_low_cells", "ofp_set_layout", "ofp_state_counts",
     "ofp_last_error", "ofp_version", "ofp_destroy",
+    "ofp_job_create", "ofp_job_enqueue", "ofp_job_info_get", "ofp_job_result", "ofp_job_destroy",
 )
 
 
@@ -74,6 +75,18 @@ class Stats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class JobInfo(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int),
+        ("k", ctypes.c_int),
+        ("rows_total", ctypes.c_int),
+        ("rows_enqueued", ctypes.c_int),
+        ("idle", ctypes.c_int),
+        ("state_bytes", ctypes.c_uint64),
+        ("transitions_per_row", ctypes.c_uint64),
+    ]
+
+
 _lib = None
 
 
@@ -101,10 +114,16 @@ def load() -> ctypes.CDLL:
         "ofp_read_main": (ctypes.c_int32, [vp, ctypes.c_uint64, u32p]),
         "ofp_set_launch": (ctypes.c_int32, [vp, ctypes.c_int, ctypes.c_int]),
         "ofp_set_layout": (ctypes.c_int32, [vp, ctypes.c_int]),
+        "ofp_set_low_cells": (ctypes.c_int32, [vp, ctypes.c_int, ctypes.c_int]),
         "ofp_state_counts": (ctypes.c_int32, [ctypes.c_int, u64p, u64p]),
         "ofp_last_error": (ctypes.c_char_p, [vp]),
         "ofp_version": (ctypes.c_char_p, []),
         "ofp_destroy": (None, [vp]),
+        "ofp_job_create": (ctypes.c_int32, [vp, ctypes.c_int, u32p, ctypes.c_int, ctypes.POINTER(vp)]),
+        "ofp_job_enqueue": (ctypes.c_int32, [vp, ctypes.c_int]),
+        "ofp_job_info_get": (ctypes.c_int32, [vp, ctypes.POINTER(JobInfo)]),
+        "ofp_job_result": (ctypes.c_int32, [vp, u32p]),
+        "ofp_job_destroy": (None, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -224,9 +243,48 @@ class Context:
     def set_launch(self, threads: int, blocks_per_cu: int) -> None:
         self._check(self._lib.ofp_set_launch(self._h, threads, blocks_per_cu), "ofp_set_launch")
 
+    def set_low_cells(self, low_cells: int, low_kc: int) -> None:
+        """Cells p <= low_cells advance at most low_kc moduli per launch."""
+        self._check(self._lib.ofp_set_low_cells(self._h, low_cells, low_kc), "ofp_set_low_cells")
+
     def set_layout(self, layout: int) -> None:
         """0 = 64-state tiles, 1 = state-major; invalidates the prepared state."""
         self._check(self._lib.ofp_set_layout(self._h, layout), "ofp_set_layout")
+
+
+class Job:
+    """A device-resident solve running asynchronously on its own stream."""
+
+    def __init__(self, ctx: Context, n: int, moduli: Sequence[int]):
+        self.ctx, self.n, self.moduli = ctx, n, list(moduli)
+        h = ctypes.c_void_p()
+        ctx._check(ctx._lib.ofp_job_create(ctx._h, n, _u32_array(self.moduli), len(self.moduli),
+                                           ctypes.byref(h)), "ofp_job_create")
+        self._h = h
+
+    def enqueue(self, rows: int) -> None:
+        self.ctx._check(self.ctx._lib.ofp_job_enqueue(self._h, rows), "ofp_job_enqueue")
+
+    def info(self) -> JobInfo:
+        inf = JobInfo()
+        self.ctx._check(self.ctx._lib.ofp_job_info_get(self._h, ctypes.byref(inf)), "ofp_job_info_get")
+        return inf
+
+    def result(self) -> list[int]:
+        out = (ctypes.c_uint32 * len(self.moduli))()
+        self.ctx._check(self.ctx._lib.ofp_job_result(self._h, out), "ofp_job_result")
+        return list(out)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self.ctx._lib.ofp_job_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def version() -> str:

@@ -1,0 +1,61 @@
+"""Serving loop on the GPU: overlapping jobs, merged moduli, memory admission.
+Every request's residues must be bit-identical to a solo solve of the same
+(n, moduli) and to the golden counts."""
+from __future__ import annotations
+
+import pytest
+
+from conftest import load_golden
+
+pytestmark = pytest.mark.gpu
+
+
+def _golden(n, p):
+    ex = load_golden("exact_counts.json")["counts"]
+    if str(n) in ex:
+        return int(ex[str(n)]) % p
+    try:
+        rows = load_golden("ggcount_residues.json")["residues"]
+    except FileNotFoundError:
+        return None
+    for r in rows:
+        if r["n"] == n and r["modulus"] == p:
+            return r["residue"]
+    return None
+
+
+@pytest.mark.parametrize("merge,max_jobs,budget_gib", [(True, 4, 64), (False, 6, 64), (True, 8, 0.02)])
+def test_served_equals_solo(gpu_ctx, merge, max_jobs, budget_gib):
+    from ofp import serving
+    reqs = serving.poisson_stream(16, rate=200.0, seed=7 + max_jobs, n_lo=8, n_hi=14, k_lo=1, k_hi=5)
+    budget = int(budget_gib * 2**30)
+    srv = serving.Server(gpu_ctx, budget, max_jobs=max_jobs, merge=merge)
+    print(f"case merge={merge} max_jobs={max_jobs} budget={budget}", flush=True)
+    rep = srv.run(reqs)
+    assert rep["requests"] == 16
+    assert srv.peak_bytes <= budget
+    if merge:
+        assert rep["jobs"] <= 16
+    for r in reqs:
+        solo, _ = gpu_ctx.solve(r.n, r.moduli)
+        assert r.residues == solo, (r.rid, r.n, r.moduli)
+        for p, res in zip(r.moduli, r.residues):
+            g = _golden(r.n, p)
+            if g is not None:
+                assert res == g, (r.n, p)
+    assert rep["latency_p99_s"] >= rep["latency_p50_s"] > 0
+
+
+def test_jobs_overlap_and_results(gpu_ctx):
+    """Several jobs enqueued back to back on their own streams all finish correctly."""
+    import ofp
+    primes = [4294967291, 4294966997, 2147483647]
+    jobs = [ofp.Job(gpu_ctx, n, primes[: 1 + n % 3]) for n in (10, 11, 12, 13, 9, 14)]
+    for j in jobs:
+        j.enqueue(j.n + 1)
+    for j in jobs:
+        solo, _ = gpu_ctx.solve(j.n, j.moduli)
+        assert j.result() == solo
+        inf = j.info()
+        assert inf.idle == 1 and inf.rows_enqueued == inf.rows_total == j.n + 1
+        j.close()
