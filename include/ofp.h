@@ -111,6 +111,12 @@ ofp_status ofp_read_main(ofp_ctx* ctx, uint64_t index, uint32_t* out_k);
  * GRIDFP_FRONTIER_BLOCKS (row6crt20.cu:1182,1192,1306-1307). */
 ofp_status ofp_set_launch(ofp_ctx* ctx, int threads, int blocks_per_cu);
 
+/* Cell-kernel variant for the state-major layout: 0 = an x=N thread loads its
+ * own orbit values before scanning for closure predecessors, 1 = after (fewer
+ * registers live during the scans).  Set by the autotuner; no reference
+ * counterpart. */
+ofp_status ofp_set_variant(ofp_ctx* ctx, int variant);
+
 /* Moduli per launch on the low cells: cells p <= low_cells advance at most
  * low_kc moduli per launch (more launches, fewer registers per thread); the
  * other cells advance up to 16.  Default low_cells = 0.  Set by the autotuner;

@@ -46,6 +46,7 @@ struct ofp_ctx {
     int blocks_per_cu = 32;
     int use_graphs = 1;
     int layout = 1;  // 0: 64-state tiles, 1: state-major with even stride
+    int variant = 0;         // kernel variant (cell_kernel V parameter), set by the autotuner
     int low_cells = 0;       // cells p <= low_cells launch at most low_kc moduli at a time
     int low_kc = kMaxKC;
     size_t reserve = 0;
@@ -213,6 +214,7 @@ ofp_status enqueue_cells(ofp_ctx* c, Sweep& s, int p_hi, int p_lo, bool first_re
             a.nblk = s.nblk;
             a.first = 0;
             a.layout = s.layout;
+            a.variant = c->variant;
             a.W = s.W;
             a.p = p;
             a.read_blk = (p == p_hi) ? (first_reads_blk ? 1 : 0) : 1;
@@ -239,8 +241,8 @@ ofp_status run_rows(ofp_ctx* c, Sweep& s, int rows, int* launches) {
         return OFP_OK;
     }
     char key[256];
-    snprintf(key, sizeof key, "%p/%p/%d/%d/%d/%d/%d/%d/%d", (void*)s.main, (void*)s.blk, s.n, s.K,
-             c->threads, c->blocks_per_cu, s.layout, c->low_cells, c->low_kc);
+    snprintf(key, sizeof key, "%p/%p/%d/%d/%d/%d/%d/%d/%d/%d", (void*)s.main, (void*)s.blk, s.n, s.K,
+             c->threads, c->blocks_per_cu, s.layout, c->low_cells, c->low_kc, c->variant);
     std::string k = key;
     for (uint32_t m : s.mods) k += "/" + std::to_string(m);
     if (k != s.row_key || !s.row_exec) {
@@ -377,6 +379,14 @@ ofp_status ofp_set_launch(ofp_ctx* c, int threads, int blocks_per_cu) {
     if (threads != c->threads || blocks_per_cu != c->blocks_per_cu) drop_graph(c->st);
     c->threads = threads;
     c->blocks_per_cu = blocks_per_cu;
+    return OFP_OK;
+}
+
+ofp_status ofp_set_variant(ofp_ctx* c, int variant) {
+    if (!c) return OFP_E_ARG;
+    if (variant != 0 && variant != 1) return fail(c, OFP_E_ARG, "variant must be 0 or 1");
+    if (variant != c->variant) drop_graph(c->st);
+    c->variant = variant;
     return OFP_OK;
 }
 

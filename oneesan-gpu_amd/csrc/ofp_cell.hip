@@ -43,7 +43,7 @@ using namespace cell;
 // workgroup g runs on XCD g%8 (round-robin dispatch), so the chunks are split
 // into 8 contiguous slabs and XCD x's workgroups stride through slab x.  Each
 // XCD's L2 then sees one contiguous region of the state.
-template <int KC, typename Idx, int L>
+template <int KC, typename Idx, int L, int V>
 __global__ __launch_bounds__(256) void cell_kernel(CellLaunch a) {
     __shared__ Quad<Idx> tab[kDpRows][kDpCols];
     const Quad<Idx>* g = (const Quad<Idx>*)a.dp;
@@ -56,7 +56,7 @@ __global__ __launch_bounds__(256) void cell_kernel(CellLaunch a) {
     const uint64_t c0 = (chunks * xcd) >> 3, c1 = (chunks * (xcd + 1)) >> 3;
     for (uint64_t c = c0 + slot; c < c1; c += nx) {
         const uint64_t b = c * blockDim.x + threadIdx.x;
-        if (b < a.nblk) cell_one<KC, Idx, L>(a, T, a.first + b);
+        if (b < a.nblk) cell_one<KC, Idx, L, V>(a, T, a.first + b);
     }
 }
 
@@ -68,10 +68,14 @@ __global__ void set_one_kernel(uint32_t* main, uint64_t index, int K, int k, int
 
 template <typename Idx, int KC>
 hipError_t launch_kc(const CellLaunch& a, int threads, int blocks, hipStream_t s) {
-    if (a.layout == kAoS)
-        hipLaunchKernelGGL((cell_kernel<KC, Idx, kAoS>), dim3(blocks), dim3(threads), 0, s, a);
-    else
-        hipLaunchKernelGGL((cell_kernel<KC, Idx, kAoSoA>), dim3(blocks), dim3(threads), 0, s, a);
+    if (a.layout == kAoS) {
+        if (a.variant == 1)
+            hipLaunchKernelGGL((cell_kernel<KC, Idx, kAoS, 1>), dim3(blocks), dim3(threads), 0, s, a);
+        else
+            hipLaunchKernelGGL((cell_kernel<KC, Idx, kAoS, 0>), dim3(blocks), dim3(threads), 0, s, a);
+    } else {
+        hipLaunchKernelGGL((cell_kernel<KC, Idx, kAoSoA, 0>), dim3(blocks), dim3(threads), 0, s, a);
+    }
     return hipGetLastError();
 }
 

@@ -131,7 +131,10 @@ struct Gather {
     }
 };
 
-template <int KC, typename Idx, int L>
+// V selects when an x=N thread loads its own orbit values (NN, LR, blocked-in):
+// 0 = before the closure scans (more loads in flight, more registers live during
+// the scans), 1 = after them (fewer live registers, so more waves per SIMD).
+template <int KC, typename Idx, int L, int V = 0>
 OFP_HD void cell_one(const CellLaunch& a, const Tab<Idx>& T, uint64_t bidx) {
     const int W = a.W, p = a.p, K = a.K, off = a.off;
     uint32_t* __restrict__ mainp = a.main;
@@ -163,9 +166,11 @@ OFP_HD void cell_one(const CellLaunch& a, const Tab<Idx>& T, uint64_t bidx) {
         const uint64_t iNN = (uint64_t)(mainbase + sr);
         const uint64_t iLR = (uint64_t)(mainbase + T.cL(p, hP) + T.at(p - 1, hP + 1) + sr);
         Vals<KC> nn, lr, bc;
-        load_vals<KC, L>(mainp, iNN, K, off, nn);
-        load_vals<KC, L>(mainp, iLR, K, off, lr);
-        if (a.read_blk) load_vals<KC, L>(blkp, bidx, K, off, bc);
+        if constexpr (V == 0) {
+            load_vals<KC, L>(mainp, iNN, K, off, nn);
+            load_vals<KC, L>(mainp, iLR, K, off, lr);
+            if (a.read_blk) load_vals<KC, L>(blkp, bidx, K, off, bc);
+        }
         Gather<KC, L> g;
         g.init();
         if (p >= 2) {
@@ -216,6 +221,11 @@ OFP_HD void cell_one(const CellLaunch& a, const Tab<Idx>& T, uint64_t bidx) {
             }
         }
         g.finish(mod);
+        if constexpr (V == 1) {
+            load_vals<KC, L>(mainp, iNN, K, off, nn);
+            load_vals<KC, L>(mainp, iLR, K, off, lr);
+            if (a.read_blk) load_vals<KC, L>(blkp, bidx, K, off, bc);
+        }
         Vals<KC> onn, olr;
 #pragma unroll
         for (int j = 0; j < KC; ++j) {

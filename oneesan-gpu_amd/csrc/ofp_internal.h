@@ -46,6 +46,7 @@ struct CellLaunch {
     int read_blk;         // blocked input of this cell is valid
     int K, off, kc;       // interleave stride, first modulus, moduli in this launch
     int layout;           // 0: 64-state tiles (AoSoA), 1: state-major (AoS)
+    int variant;          // state-major only: 0 = orbit loads before closure scans, 1 = after
     uint32_t mod[kMaxKC];
 };
 

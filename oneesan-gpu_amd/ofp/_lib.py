@@ -25,7 +25,7 @@ OFP_MAX_N = 27
 # Exported names, in the order include/ofp.h declares them.
 EXPORTS = (
     "ofp_create", "ofp_plan", "ofp_solve", "ofp_apply_cells", "ofp_prepare",
-    "ofp_row_sweep", "ofp_read_main", "ofp_set_launch", "ofp_set_low_cells", "ofp_set_layout", "ofp_state_counts",
+    "ofp_row_sweep", "ofp_read_main", "ofp_set_launch", "ofp_set_variant", "ofp_set_low_cells", "ofp_set_layout", "ofp_state_counts",
     "ofp_last_error", "ofp_version", "ofp_destroy",
     "ofp_job_create", "ofp_job_enqueue", "ofp_job_info_get", "ofp_job_result", "ofp_job_destroy",
 )
@@ -115,6 +115,7 @@ def load() -> ctypes.CDLL:
         "ofp_set_launch": (ctypes.c_int32, [vp, ctypes.c_int, ctypes.c_int]),
         "ofp_set_layout": (ctypes.c_int32, [vp, ctypes.c_int]),
         "ofp_set_low_cells": (ctypes.c_int32, [vp, ctypes.c_int, ctypes.c_int]),
+        "ofp_set_variant": (ctypes.c_int32, [vp, ctypes.c_int]),
         "ofp_state_counts": (ctypes.c_int32, [ctypes.c_int, u64p, u64p]),
         "ofp_last_error": (ctypes.c_char_p, [vp]),
         "ofp_version": (ctypes.c_char_p, []),
@@ -242,6 +243,10 @@ class Context:
 
     def set_launch(self, threads: int, blocks_per_cu: int) -> None:
         self._check(self._lib.ofp_set_launch(self._h, threads, blocks_per_cu), "ofp_set_launch")
+
+    def set_variant(self, variant: int) -> None:
+        """Cell-kernel variant (state-major layout): 0 = early orbit loads, 1 = late."""
+        self._check(self._lib.ofp_set_variant(self._h, variant), "ofp_set_variant")
 
     def set_low_cells(self, low_cells: int, low_kc: int) -> None:
         """Cells p <= low_cells advance at most low_kc moduli per launch."""

@@ -28,7 +28,8 @@ THREADS = (128, 256)
 BLOCKS_PER_CU = (4, 8, 16, 32)
 LAYOUTS = (0, 1)
 # second stage: cells p <= low_cells advance at most low_kc moduli per launch
-LOW_CELLS = ((0, 16), (6, 8), (6, 4), (8, 8), (4, 8), (6, 2))
+LOW_CELLS = ((0, 16), (6, 8), (4, 8))
+VARIANTS = (0, 1)
 
 
 def _key(n: int, k: int) -> str:
@@ -69,6 +70,7 @@ def apply(ctx: "_lib.Context", n: int, k: int) -> dict | None:
     if ent:
         ctx.set_layout(int(ent.get("layout", 1)))
         ctx.set_launch(int(ent["threads"]), int(ent["blocks_per_cu"]))
+        ctx.set_variant(int(ent.get("variant", 0)))
         ctx.set_low_cells(int(ent.get("low_cells", 0)), int(ent.get("low_kc", 16)))
     return ent
 
@@ -100,22 +102,26 @@ def tune(ctx: "_lib.Context", n: int, moduli: list[int], rows: int = 1,
                     best = ent
     ctx.set_layout(best["layout"])
     ctx.set_launch(best["threads"], best["blocks_per_cu"])
-    best = dict(best, low_cells=0, low_kc=16)
-    if len(moduli) > 1:
+    best = dict(best, low_cells=0, low_kc=16, variant=0)
+    if best["layout"] == 1:
         ctx.prepare(n, moduli)
-        for lc, lk in LOW_CELLS:
-            ctx.set_low_cells(lc, lk)
-            ctx.row_sweep(1)
-            t0 = time.perf_counter()
-            st = ctx.row_sweep(rows)
-            row_ms = (time.perf_counter() - t0) / rows * 1e3
-            ent = dict(best, low_cells=lc, low_kc=lk, row_ms=row_ms,
-                       kernel_ms_per_row=st["kernel_s"] / rows * 1e3)
-            table.append(ent)
-            if log:
-                log(f"autotune n={n} k={len(moduli)} low_cells={lc} low_kc={lk}: {row_ms:.2f} ms/row")
-            if row_ms < best["row_ms"]:
-                best = ent
+        for var in VARIANTS:
+            for lc, lk in (LOW_CELLS if len(moduli) > 1 else ((0, 16),)):
+                ctx.set_variant(var)
+                ctx.set_low_cells(lc, lk)
+                ctx.row_sweep(1)
+                t0 = time.perf_counter()
+                st = ctx.row_sweep(rows)
+                row_ms = (time.perf_counter() - t0) / rows * 1e3
+                ent = dict(best, variant=var, low_cells=lc, low_kc=lk, row_ms=row_ms,
+                           kernel_ms_per_row=st["kernel_s"] / rows * 1e3)
+                table.append(ent)
+                if log:
+                    log(f"autotune n={n} k={len(moduli)} variant={var} low_cells={lc} low_kc={lk}: "
+                        f"{row_ms:.2f} ms/row")
+                if row_ms < best["row_ms"]:
+                    best = ent
+    ctx.set_variant(best["variant"])
     ctx.set_low_cells(best["low_cells"], best["low_kc"])
     result = {"best": best, "table": table}
     if path is not None:
@@ -123,6 +129,7 @@ def tune(ctx: "_lib.Context", n: int, moduli: list[int], rows: int = 1,
         data[_key(n, len(moduli))] = {"layout": best["layout"], "threads": best["threads"],
                                       "blocks_per_cu": best["blocks_per_cu"],
                                       "low_cells": best["low_cells"], "low_kc": best["low_kc"],
+                                      "variant": best["variant"],
                                       "row_ms": best["row_ms"], "tuned_at": time.time()}
         save_cache(data, path)
     return result

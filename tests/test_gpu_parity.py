@@ -46,10 +46,13 @@ def test_every_cell_random_vectors(gpu_ctx, oracle, n):
                 assert np.array_equal(g_blk[:, j], eb), (n, p, mod, "blocked")
 
 
-@pytest.fixture(scope="module", params=[0, 1], ids=["tiles64", "statemajor"])
+@pytest.fixture(scope="module", params=[(0, 0), (1, 0), (1, 1)],
+                ids=["tiles64", "statemajor-early", "statemajor-late"])
 def layout_ctx(request):
     import ofp
-    ctx = ofp.Context(layout=request.param)
+    layout, variant = request.param
+    ctx = ofp.Context(layout=layout)
+    ctx.set_variant(variant)
     yield ctx
     ctx.close()
 
@@ -119,3 +122,15 @@ def test_n12_config_residue(gpu_ctx, oracle):
     got, _ = gpu_ctx.solve(12, [4294967291])
     assert got[0] == oracle.solve(12, 4294967291)
     assert got[0] == int(load_golden("exact_counts.json")["counts"]["12"]) % 4294967291
+
+
+def test_ggcount_residues(gpu_ctx):
+    """n=10..19 against the reference's third_party ggcount (independent frontier DP)."""
+    rows = load_golden("ggcount_residues.json")["residues"]
+    by_n: dict[int, list[tuple[int, int]]] = {}
+    for r in rows:
+        by_n.setdefault(r["n"], []).append((r["modulus"], r["residue"]))
+    for n, pairs in sorted(by_n.items()):
+        print(f"case ggcount n={n}", flush=True)
+        got, _ = gpu_ctx.solve(n, [m for m, _ in pairs])
+        assert got == [r for _, r in pairs], n
