@@ -50,13 +50,14 @@ __global__ __launch_bounds__(256) void cell_kernel(CellLaunch a) {
     for (int t = threadIdx.x; t < kDpRows * kDpCols; t += blockDim.x) (&tab[0][0])[t] = g[t];
     __syncthreads();
     Tab<Idx> T{tab};
+    const Direct<KC, L> mem{a.main, a.blk, a.K, a.off};
     const uint64_t chunks = (a.nblk + blockDim.x - 1) / blockDim.x;
     const uint32_t nwg = gridDim.x;  // multiple of 8
     const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3, nx = nwg >> 3;
     const uint64_t c0 = (chunks * xcd) >> 3, c1 = (chunks * (xcd + 1)) >> 3;
     for (uint64_t c = c0 + slot; c < c1; c += nx) {
         const uint64_t b = c * blockDim.x + threadIdx.x;
-        if (b < a.nblk) cell_one<KC, Idx, L, V>(a, T, a.first + b);
+        if (b < a.nblk) cell_one<KC, Idx, L, V>(a, T, a.first + b, mem);
     }
 }
 

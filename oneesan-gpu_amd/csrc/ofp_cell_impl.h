@@ -109,6 +109,18 @@ OFP_HD void add_into(Vals<KC>& acc, const Vals<KC>& x, const uint32_t* mod) {
 // Closure predecessors are found one at a time by a scan; their residues are
 // accumulated one candidate late, so the loads of candidate c+1 are issued
 // before the values of candidate c are added (no memory latency per candidate).
+// Memory policy of a launch: Direct addresses the whole device-resident state.
+template <int KC, int L>
+struct Direct {
+    uint32_t* __restrict__ main;
+    uint32_t* __restrict__ blk;
+    int K, off;
+    OFP_HD void lm(uint64_t i, Vals<KC>& v) const { load_vals<KC, L>(main, i, K, off, v); }
+    OFP_HD void sm(uint64_t i, const Vals<KC>& v) const { store_vals<KC, L>(main, i, K, off, v); }
+    OFP_HD void lb(uint64_t b, Vals<KC>& v) const { load_vals<KC, L>(blk, b, K, off, v); }
+    OFP_HD void sb(uint64_t b, const Vals<KC>& v) const { store_vals<KC, L>(blk, b, K, off, v); }
+};
+
 template <int KC, int L>
 struct Gather {
     Vals<KC> acc, pend;
@@ -118,9 +130,10 @@ struct Gather {
         for (int j = 0; j < KC; ++j) acc.v[j] = 0;
         have = false;
     }
-    OFP_HD void add(const uint32_t* __restrict__ base, uint64_t i, int K, int off, const uint32_t* mod) {
+    template <class Mem>
+    OFP_HD void add(const Mem& m, uint64_t i, const uint32_t* mod) {
         Vals<KC> cur;
-        load_vals<KC, L>(base, i, K, off, cur);
+        m.lm(i, cur);
         if (have) add_into<KC>(acc, pend, mod);
         pend = cur;
         have = true;
@@ -134,11 +147,9 @@ struct Gather {
 // V selects when an x=N thread loads its own orbit values (NN, LR, blocked-in):
 // 0 = before the closure scans (more loads in flight, more registers live during
 // the scans), 1 = after them (fewer live registers, so more waves per SIMD).
-template <int KC, typename Idx, int L, int V = 0>
-OFP_HD void cell_one(const CellLaunch& a, const Tab<Idx>& T, uint64_t bidx) {
-    const int W = a.W, p = a.p, K = a.K, off = a.off;
-    uint32_t* __restrict__ mainp = a.main;
-    uint32_t* __restrict__ blkp = a.blk;
+template <int KC, typename Idx, int L, int V, class Mem>
+OFP_HD void cell_one(const CellLaunch& a, const Tab<Idx>& T, uint64_t bidx, const Mem& m) {
+    const int W = a.W, p = a.p;
     uint32_t mod[KC];
 #pragma unroll
     for (int j = 0; j < KC; ++j) mod[j] = a.mod[j];
@@ -167,9 +178,9 @@ OFP_HD void cell_one(const CellLaunch& a, const Tab<Idx>& T, uint64_t bidx) {
         const uint64_t iLR = (uint64_t)(mainbase + T.cL(p, hP) + T.at(p - 1, hP + 1) + sr);
         Vals<KC> nn, lr, bc;
         if constexpr (V == 0) {
-            load_vals<KC, L>(mainp, iNN, K, off, nn);
-            load_vals<KC, L>(mainp, iLR, K, off, lr);
-            if (a.read_blk) load_vals<KC, L>(blkp, bidx, K, off, bc);
+            m.lm(iNN, nn);
+            m.lm(iLR, lr);
+            if (a.read_blk) m.lb(bidx, bc);
         }
         Gather<KC, L> g;
         g.init();
@@ -177,7 +188,7 @@ OFP_HD void cell_one(const CellLaunch& a, const Tab<Idx>& T, uint64_t bidx) {
             // RL predecessor: [P,R,L,S]
             if (hP > 0) {
                 const uint64_t iRL = (uint64_t)(mainbase + T.at(p, hP) + T.cL(p - 1, hP - 1) + sr);
-                g.add(mainp, iRL, K, off, mod);
+                g.add(m, iRL, mod);
             }
             // LL predecessors: [P,L,L,S*], S* = S with a top-level L at q turned into R,
             // entered at height hP+2 (reverse_block_predecessors, gridfp_reverse.hpp:62-68)
@@ -190,7 +201,7 @@ OFP_HD void cell_one(const CellLaunch& a, const Tab<Idx>& T, uint64_t bidx) {
                     const int v = T.step(pos, hh, rr, c0);
                     const Quad<Idx> e2 = T.q[pos][hh + 2];
                     if (v == 2 && depth == 1)
-                        g.add(mainp, (uint64_t)(baseLL + s2 + e2.zn + (sr - s0 - c0)), K, off, mod);
+                        g.add(m, (uint64_t)(baseLL + s2 + e2.zn + (sr - s0 - c0)), mod);
                     s0 += c0;
                     s2 += (v == 0) ? Idx(0) : (v == 1 ? e2.zn : Idx(e2.zn + e2.zr));
                     if (v == 2) ++depth;
@@ -212,7 +223,7 @@ OFP_HD void cell_one(const CellLaunch& a, const Tab<Idx>& T, uint64_t bidx) {
                 const Quad<Idx> e2 = T.q[mp][H + 2];
                 const Idx c0 = (v == 0) ? Idx(0) : (v == 1 ? e0.zn : Idx(e0.zn + e0.zr));
                 if (v == 1 && depth == 1)
-                    g.add(mainp, (uint64_t)(mainbase - (u0 + c0) + e0.zn + e0.zr + u2 + tail), K, off, mod);
+                    g.add(m, (uint64_t)(mainbase - (u0 + c0) + e0.zn + e0.zr + u2 + tail), mod);
                 u0 += c0;
                 u2 += (v == 0) ? Idx(0) : (v == 1 ? e2.zn : Idx(e2.zn + e2.zr));
                 if (v == 2) { if (--depth == 0) break; }
@@ -222,9 +233,9 @@ OFP_HD void cell_one(const CellLaunch& a, const Tab<Idx>& T, uint64_t bidx) {
         }
         g.finish(mod);
         if constexpr (V == 1) {
-            load_vals<KC, L>(mainp, iNN, K, off, nn);
-            load_vals<KC, L>(mainp, iLR, K, off, lr);
-            if (a.read_blk) load_vals<KC, L>(blkp, bidx, K, off, bc);
+            m.lm(iNN, nn);
+            m.lm(iLR, lr);
+            if (a.read_blk) m.lb(bidx, bc);
         }
         Vals<KC> onn, olr;
 #pragma unroll
@@ -234,47 +245,47 @@ OFP_HD void cell_one(const CellLaunch& a, const Tab<Idx>& T, uint64_t bidx) {
             if (p == 1) t = addm(t, g.acc.v[j], mod[j]);
             onn.v[j] = t;
         }
-        store_vals<KC, L>(mainp, iNN, K, off, onn);
-        store_vals<KC, L>(mainp, iLR, K, off, olr);
-        if (p >= 2) store_vals<KC, L>(blkp, bidx, K, off, g.acc);
+        m.sm(iNN, onn);
+        m.sm(iLR, olr);
+        if (p >= 2) m.sb(bidx, g.acc);
     } else if (x == 1) {
         // orbit RN -> NR (and at p=1 also NR -> RN)
         const uint64_t iNR = (uint64_t)(mainbase + T.at(p - 1, hP) + sr);
         const uint64_t iRN = (uint64_t)(mainbase + T.at(p, hP) + sr);
         Vals<KC> nr, rn, bc, o;
-        load_vals<KC, L>(mainp, iNR, K, off, nr);
-        load_vals<KC, L>(mainp, iRN, K, off, rn);
-        if (a.read_blk) load_vals<KC, L>(blkp, bidx, K, off, bc);
+        m.lm(iNR, nr);
+        m.lm(iRN, rn);
+        if (a.read_blk) m.lb(bidx, bc);
 #pragma unroll
         for (int j = 0; j < KC; ++j) {
             uint32_t t = addm(nr.v[j], rn.v[j], mod[j]);
             if (a.read_blk) t = addm(t, bc.v[j], mod[j]);
             o.v[j] = t;
         }
-        store_vals<KC, L>(mainp, iNR, K, off, o);
+        m.sm(iNR, o);
         if (p >= 2) {
-            store_vals<KC, L>(blkp, bidx, K, off, nr);
+            m.sb(bidx, nr);
         } else {
 #pragma unroll
             for (int j = 0; j < KC; ++j) o.v[j] = addm(rn.v[j], nr.v[j], mod[j]);
-            store_vals<KC, L>(mainp, iRN, K, off, o);
+            m.sm(iRN, o);
         }
     } else {
         // orbit LN -> NL (cannot occur at p=1)
         const uint64_t iNL = (uint64_t)(mainbase + T.cL(p - 1, hP) + sr);
         const uint64_t iLN = (uint64_t)(mainbase + T.cL(p, hP) + sr);
         Vals<KC> nl, ln, bc, o;
-        load_vals<KC, L>(mainp, iNL, K, off, nl);
-        load_vals<KC, L>(mainp, iLN, K, off, ln);
-        if (a.read_blk) load_vals<KC, L>(blkp, bidx, K, off, bc);
+        m.lm(iNL, nl);
+        m.lm(iLN, ln);
+        if (a.read_blk) m.lb(bidx, bc);
 #pragma unroll
         for (int j = 0; j < KC; ++j) {
             uint32_t t = addm(nl.v[j], ln.v[j], mod[j]);
             if (a.read_blk) t = addm(t, bc.v[j], mod[j]);
             o.v[j] = t;
         }
-        store_vals<KC, L>(mainp, iNL, K, off, o);
-        if (p >= 2) store_vals<KC, L>(blkp, bidx, K, off, nl);
+        m.sm(iNL, o);
+        if (p >= 2) m.sb(bidx, nl);
     }
 }
 
