@@ -132,6 +132,7 @@ def main() -> int:
     ap.add_argument("--plain-rows", type=int, default=2, help="rows timed for the k=1 plain form")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-shard", action="store_true", help="skip the n=27 one-GPU-share measurement")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -208,6 +209,15 @@ def main() -> int:
         log(f"bench: plain k=1 {plain['ms_per_row']:.1f} ms/row -> {plain['residues_per_hour']:.0f} residues/h")
     ctx.close()
 
+    shard = None
+    if not args.no_shard:
+        from ofp import shard as shard_mod
+        with ofp.Context(device=0) as sctx:
+            autotune.apply(sctx, 22, 4)
+            shard = shard_mod.measure(sctx)
+        log(f"bench: n=27 share window {shard['window_bytes'] / 2**30:.1f} GiB, "
+            f"{shard['avg_launch_ms']:.1f} ms/cell, {shard['roofline']['frac']:.3f} of HBM peak")
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(n, k, args.cpu_seconds)
@@ -242,6 +252,7 @@ def main() -> int:
                      "kernel": "cell_kernel", "bytes_per_launch": bytes_per_launch,
                      "avg_launch_ms": avg_launch_s * 1e3},
         "cpu_baseline": cpu,
+        "shard_n27": shard,
     }
     print(json.dumps(out), flush=True)
     return 0

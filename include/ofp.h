@@ -152,6 +152,41 @@ ofp_status ofp_job_info_get(ofp_job* job, ofp_job_info* info);
 ofp_status ofp_job_result(ofp_job* job, uint32_t* residues_out);
 void ofp_job_destroy(ofp_job* job);
 
+/* ---- shard sweep: one GPU's share of the low cells of a row ----
+ * The share is slice `slice` of `slices` equal ranges of the blocked index
+ * space, widened to whole prefix classes (a prefix class = all words with the
+ * same content at the positions above the low cells).  For cells
+ * p = low_cells..1 every word such a share writes is in its own class, so the
+ * share's main words form one contiguous window and its blocked words one
+ * contiguous range; main words of other shards (read only by closures that
+ * flip a position above the low cells) read as a seeded synthetic residue.
+ * The window is initialised with the same synthetic residues ("input residues
+ * seeded uniform in [0,p)"), the cells are swept in place on the device, and
+ * the kernel time is returned.  Reference analogue: one GPU's owner-local
+ * LOW window in the 8-GPU bucket layout (docs/b300-bucket-snake.md:7-20). */
+typedef struct ofp_shard_spec {
+    int n;
+    int k;                     /* 1, 2, 4 or 8 moduli, interleaved */
+    const uint32_t* moduli;
+    int slices, slice;         /* this shard = slice of slices */
+    int low_cells;             /* cells low_cells..1 (0 = (n+1)/2) */
+    uint64_t seed;
+} ofp_shard_spec;
+typedef struct ofp_shard_info {
+    uint64_t first_blocked, blocked_count;   /* resident blocked words */
+    uint64_t first_main, main_count;         /* resident main window */
+    uint64_t state_bytes;                    /* device bytes of the window */
+    int low_cells, launches;
+    double kernel_s;                         /* all cells, HIP events */
+    uint64_t stray_stores;                   /* stores outside the window (0 by construction) */
+} ofp_shard_info;
+/* Window of a shard (no device work; ctx may be NULL). */
+ofp_status ofp_shard_plan(ofp_ctx* ctx, const ofp_shard_spec* spec, ofp_shard_info* info);
+/* main_out (main_count*k) / blocked_out (blocked_count*k): optional host copies
+ * of the window after the sweep, state-major. */
+ofp_status ofp_shard_sweep(ofp_ctx* ctx, const ofp_shard_spec* spec, ofp_shard_info* info,
+                           uint32_t* main_out, uint32_t* blocked_out);
+
 /* Counts only, no device needed: main / blocked word counts for n. */
 ofp_status ofp_state_counts(int n, uint64_t* main_states, uint64_t* blocked_states);
 

@@ -285,6 +285,51 @@ ofp_status init_state(ofp_ctx* c, Sweep& s) {
     return OFP_OK;
 }
 
+// Host rank / unrank over the height table (lexicographic, N<R<L, entered at height 1).
+uint64_t rank_word(uint64_t w, int width) {
+    uint64_t r = 0;
+    int h = 1;
+    for (int pos = width - 1; pos >= 0; --pos) {
+        int v = (int)((w >> (2 * pos)) & 3u);
+        if (v >= 1) r += dp_at(pos, h);
+        if (v == 2 && h > 0) r += dp_at(pos, h - 1);
+        h += (v == 2) - (v == 1);
+    }
+    return r;
+}
+uint64_t unrank_word(uint64_t r, int width, int stop_pos, int* h_at_stop) {
+    uint64_t w = 0;
+    int h = 1;
+    for (int pos = width - 1; pos >= 0; --pos) {
+        if (pos == stop_pos - 1 && h_at_stop) *h_at_stop = h;
+        uint64_t z = dp_at(pos, h);
+        if (r < z) continue;
+        r -= z;
+        uint64_t z2 = h > 0 ? dp_at(pos, h - 1) : 0;
+        if (r < z2) { w |= 1ull << (2 * pos); --h; continue; }
+        r -= z2;
+        w |= 2ull << (2 * pos);
+        ++h;
+    }
+    if (stop_pos == 0 && h_at_stop) *h_at_stop = h;
+    return w;
+}
+// Smallest legal suffix over `len` positions entered at height h (N's, then h R's).
+uint64_t min_suffix(int len, int h) {
+    uint64_t w = 0;
+    for (int pos = 0; pos < h && pos < len; ++pos) w |= 1ull << (2 * pos);
+    return w;
+}
+// First blocked / main index of the prefix class (blocked positions >= lc) that
+// contains blocked word r.
+void class_start(int W, int lc, uint64_t r, uint64_t* b_first, uint64_t* m_first) {
+    int h = 1;
+    uint64_t w = unrank_word(r, W - 1, lc, &h);
+    uint64_t prefix = w & ~((1ull << (2 * lc)) - 1ull);
+    *b_first = rank_word(prefix | min_suffix(lc, h), W - 1);
+    *m_first = rank_word((prefix << 2) | min_suffix(lc + 1, h), W);
+}
+
 ofp_status read_main(ofp_ctx* c, Sweep& s, uint64_t index, uint32_t* out_k) {
     if (!s.main) return fail(c, OFP_E_ARG, "no prepared state");
     if (index >= s.nmain) return fail(c, OFP_E_ARG, "index out of range");
@@ -521,6 +566,128 @@ ofp_status ofp_apply_cells(ofp_ctx* c, int n, const uint32_t* moduli, int k, uin
     // after cell 1 nothing is blocked; the device buffer is simply not written
     for (uint64_t i = 0; i < sw.nblk; ++i)
         for (int j = 0; j < k; ++j) blocked_io[i * k + j] = (p_lo == 1) ? 0u : tb[state_elem(lay, i, Kp, j)];
+    return OFP_OK;
+}
+
+// ---- shard sweeps: one GPU's share of the low cells of a row ----
+
+ofp_status ofp_shard_plan(ofp_ctx* c, const ofp_shard_spec* sp, ofp_shard_info* info) {
+    // host-only computation: c may be NULL (then no error message is kept)
+    if (!sp || !info) return OFP_E_ARG;
+    uint64_t nm, nb;
+    if (!counts(sp->n, &nm, &nb)) return fail(c, OFP_E_RANGE, "n must be in 1..27");
+    const int W = sp->n + 1;
+    const int lc = sp->low_cells ? sp->low_cells : W / 2;
+    if (lc < 1 || lc > W - 2) return fail(c, OFP_E_ARG, "low_cells must be in 1..W-2");
+    if (sp->slices < 1 || sp->slice < 0 || sp->slice >= sp->slices) return fail(c, OFP_E_ARG, "bad slice");
+    if (sp->k < 1 || sp->k > 8 || (sp->k & (sp->k - 1))) return fail(c, OFP_E_ARG, "shard k must be 1, 2, 4 or 8");
+    const uint64_t r0 = (uint64_t)((unsigned __int128)nb * (unsigned)sp->slice / (unsigned)sp->slices);
+    const uint64_t r1 = (uint64_t)((unsigned __int128)nb * (unsigned)(sp->slice + 1) / (unsigned)sp->slices);
+    uint64_t b0, m0, b1 = nb, m1 = nm;
+    class_start(W, lc, r0, &b0, &m0);
+    if (r1 < nb) class_start(W, lc, r1, &b1, &m1);
+    info->first_blocked = b0;
+    info->blocked_count = b1 - b0;
+    info->first_main = m0;
+    info->main_count = m1 - m0;
+    info->low_cells = lc;
+    const int Kp = interleave_stride(1, sp->k);
+    info->state_bytes = (info->main_count + info->blocked_count) * (uint64_t)Kp * 4;
+    info->kernel_s = 0;
+    info->launches = 0;
+    info->stray_stores = 0;
+    return OFP_OK;
+}
+
+ofp_status ofp_shard_sweep(ofp_ctx* c, const ofp_shard_spec* sp, ofp_shard_info* info, uint32_t* main_out,
+                           uint32_t* blk_out) {
+    if (!c) return OFP_E_ARG;
+    ofp_status st = ofp_shard_plan(c, sp, info);
+    if (st) return st;
+    for (int j = 0; j < sp->k; ++j)
+        if (sp->moduli[j] < 2) return fail(c, OFP_E_ARG, "modulus must be in [2, 4294967295]");
+    const int W = sp->n + 1, k = sp->k, Kp = interleave_stride(1, k);
+    const uint64_t M = info->main_count, B = info->blocked_count;
+    OFP_HIP(c, hipSetDevice(c->device), "hipSetDevice");
+    size_t fr = 0, tot = 0;
+    OFP_HIP(c, hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
+    if (info->state_bytes + c->reserve > fr) return fail(c, OFP_E_MEMORY, "shard window does not fit HBM");
+    uint32_t *dm = nullptr, *db = nullptr, *dmods = nullptr;
+    unsigned long long* dstray = nullptr;
+    auto release = [&]() {
+        if (dm) (void)hipFree(dm);
+        if (db) (void)hipFree(db);
+        if (dmods) (void)hipFree(dmods);
+        if (dstray) (void)hipFree(dstray);
+    };
+#define OFP_HIP_R(expr, what)                                        \
+    do {                                                             \
+        hipError_t e_ = (expr);                                      \
+        if (e_ != hipSuccess) { release(); return hip_fail(c, e_, what); } \
+    } while (0)
+    OFP_HIP_R(hipMalloc(&dm, std::max<uint64_t>(M, 1) * Kp * 4), "shard main window");
+    OFP_HIP_R(hipMalloc(&db, std::max<uint64_t>(B, 1) * Kp * 4), "shard blocked window");
+    OFP_HIP_R(hipMalloc(&dmods, 16 * 4), "shard moduli");
+    OFP_HIP_R(hipMalloc(&dstray, sizeof(unsigned long long)), "shard counter");
+    OFP_HIP_R(hipMemcpy(dmods, sp->moduli, k * 4, hipMemcpyHostToDevice), "copy moduli");
+    OFP_HIP_R(hipMemset(dstray, 0, sizeof(unsigned long long)), "zero counter");
+    hipError_t e = launch_fill_remote(dm, info->first_main, M, db, info->first_blocked, B, Kp, k, dmods,
+                                      sp->seed, c->stream);
+    if (e != hipSuccess) { release(); return hip_fail(c, e, "fill shard window"); }
+    const bool wide = W > 23;
+    const int blocks = grid_blocks(c, B);
+    OFP_HIP_R(hipEventRecord(c->ev0, c->stream), "event");
+    int launches = 0;
+    for (int p = info->low_cells; p >= 1; --p) {
+        for (int off = 0; off < k; off += 8) {
+            CellLaunch a{};
+            a.main = dm;
+            a.blk = db;
+            a.dp = wide ? c->dp64 : c->dp32;
+            a.nblk = B;
+            a.first = info->first_blocked;
+            a.layout = 1;
+            a.variant = c->variant;
+            a.W = W;
+            a.p = p;
+            a.read_blk = (p == W - 1) ? 0 : 1;
+            a.K = Kp;
+            a.off = off;
+            a.kc = std::min(8, k - off);
+            for (int j = 0; j < a.kc; ++j) a.mod[j] = sp->moduli[off + j];
+            a.m0 = info->first_main;
+            a.M = M;
+            a.seed = sp->seed;
+            a.stray = dstray;
+            e = launch_cell_shard(a, wide, c->threads, blocks, c->stream);
+            if (e != hipSuccess) { release(); return hip_fail(c, e, "shard cell launch"); }
+            ++launches;
+        }
+    }
+    OFP_HIP_R(hipEventRecord(c->ev1, c->stream), "event");
+    OFP_HIP_R(hipEventSynchronize(c->ev1), "shard sync");
+    float ms = 0;
+    OFP_HIP_R(hipEventElapsedTime(&ms, c->ev0, c->ev1), "event time");
+    info->kernel_s = ms * 1e-3;
+    info->launches = launches;
+    unsigned long long stray = 0;
+    OFP_HIP_R(hipMemcpy(&stray, dstray, sizeof stray, hipMemcpyDeviceToHost), "read counter");
+    info->stray_stores = stray;
+    if (main_out || blk_out) {
+        std::vector<uint32_t> t(std::max(M, B) * (size_t)Kp);
+        if (main_out && M) {
+            OFP_HIP_R(hipMemcpy(t.data(), dm, M * Kp * 4, hipMemcpyDeviceToHost), "download main");
+            for (uint64_t i = 0; i < M; ++i)
+                for (int j = 0; j < k; ++j) main_out[i * k + j] = t[i * Kp + j];
+        }
+        if (blk_out && B) {
+            OFP_HIP_R(hipMemcpy(t.data(), db, B * Kp * 4, hipMemcpyDeviceToHost), "download blocked");
+            for (uint64_t i = 0; i < B; ++i)
+                for (int j = 0; j < k; ++j) blk_out[i * k + j] = t[i * Kp + j];
+        }
+    }
+#undef OFP_HIP_R
+    release();
     return OFP_OK;
 }
 

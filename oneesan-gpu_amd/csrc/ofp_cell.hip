@@ -61,6 +61,47 @@ __global__ __launch_bounds__(256) void cell_kernel(CellLaunch a) {
     }
 }
 
+// Shard variant: same per-thread body, windowed memory policy (state-major).
+template <int KC, typename Idx, int V>
+__global__ __launch_bounds__(256) void cell_shard_kernel(CellLaunch a) {
+    __shared__ Quad<Idx> tab[kDpRows][kDpCols];
+    const Quad<Idx>* g = (const Quad<Idx>*)a.dp;
+    for (int t = threadIdx.x; t < kDpRows * kDpCols; t += blockDim.x) (&tab[0][0])[t] = g[t];
+    __syncthreads();
+    Tab<Idx> T{tab};
+    Windowed<KC> mem;
+    mem.main = a.main;
+    mem.blk = a.blk;
+    mem.m0 = a.m0;
+    mem.M = a.M;
+    mem.b0 = a.first;
+    mem.seed = a.seed;
+    mem.K = a.K;
+    mem.off = a.off;
+#pragma unroll
+    for (int j = 0; j < KC; ++j) mem.mod[j] = a.mod[j];
+    mem.stray = a.stray;
+    const uint64_t chunks = (a.nblk + blockDim.x - 1) / blockDim.x;
+    const uint32_t nwg = gridDim.x;
+    const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3, nx = nwg >> 3;
+    const uint64_t c0 = (chunks * xcd) >> 3, c1 = (chunks * (xcd + 1)) >> 3;
+    for (uint64_t c = c0 + slot; c < c1; c += nx) {
+        const uint64_t b = c * blockDim.x + threadIdx.x;
+        if (b < a.nblk) cell_one<KC, Idx, kAoS, V>(a, T, a.first + b, mem);
+    }
+}
+
+__global__ void fill_remote_kernel(uint32_t* main, uint64_t m0, uint64_t M, uint32_t* blk, uint64_t b0,
+                                   uint64_t B, int K, int k, const uint32_t* mods, uint64_t seed) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < (M + B) * (uint64_t)k; t += stride) {
+        const uint64_t s = t / (uint64_t)k;
+        const int j = (int)(t - s * (uint64_t)k);
+        if (s < M) main[s * K + j] = remote_residue(seed, m0 + s, j, mods[j]);
+        else blk[(s - M) * K + j] = remote_residue(seed, kBlockedDomain + b0 + (s - M), j, mods[j]);
+    }
+}
+
 __global__ void set_one_kernel(uint32_t* main, uint64_t index, int K, int k, int layout,
                                const uint32_t* mods) {
     int j = threadIdx.x;
@@ -78,6 +119,25 @@ hipError_t launch_kc(const CellLaunch& a, int threads, int blocks, hipStream_t s
         hipLaunchKernelGGL((cell_kernel<KC, Idx, kAoSoA, 0>), dim3(blocks), dim3(threads), 0, s, a);
     }
     return hipGetLastError();
+}
+
+template <typename Idx, int KC>
+hipError_t launch_shard_kc(const CellLaunch& a, int threads, int blocks, hipStream_t s) {
+    if (a.variant == 1)
+        hipLaunchKernelGGL((cell_shard_kernel<KC, Idx, 1>), dim3(blocks), dim3(threads), 0, s, a);
+    else
+        hipLaunchKernelGGL((cell_shard_kernel<KC, Idx, 0>), dim3(blocks), dim3(threads), 0, s, a);
+    return hipGetLastError();
+}
+
+template <typename Idx>
+hipError_t launch_shard_idx(const CellLaunch& a, int threads, int blocks, hipStream_t s) {
+    switch (a.kc) {
+#define OFP_KC(n) case n: return launch_shard_kc<Idx, n>(a, threads, blocks, s);
+        OFP_KC(1) OFP_KC(2) OFP_KC(4) OFP_KC(8)
+#undef OFP_KC
+    default: return hipErrorInvalidValue;
+    }
 }
 
 template <typename Idx>
@@ -103,6 +163,26 @@ hipError_t launch_cell(const CellLaunch& a, bool wide_index, int threads, int bl
         return hipErrorInvalidValue;
     return wide_index ? launch_idx<uint64_t>(a, threads, blocks, s)
                       : launch_idx<uint32_t>(a, threads, blocks, s);
+}
+
+hipError_t launch_cell_shard(const CellLaunch& a, bool wide_index, int threads, int blocks,
+                             hipStream_t s) {
+    if (threads <= 0 || threads > 256 || (threads & 63) || blocks <= 0 || (blocks & 7))
+        return hipErrorInvalidConfiguration;
+    if (a.W < 2 || a.W > kMaxW || a.p < 1 || a.p > a.W - 1 || a.layout != kAoS || a.off < 0 ||
+        a.off + a.kc > a.K || (a.K > 1 && ((a.K & 1) || (a.off & 1))) || !a.stray)
+        return hipErrorInvalidValue;
+    return wide_index ? launch_shard_idx<uint64_t>(a, threads, blocks, s)
+                      : launch_shard_idx<uint32_t>(a, threads, blocks, s);
+}
+
+hipError_t launch_fill_remote(uint32_t* main, uint64_t m0, uint64_t M, uint32_t* blk, uint64_t b0,
+                              uint64_t B, int K, int k, const uint32_t* mods_dev, uint64_t seed,
+                              hipStream_t s) {
+    if (k < 1 || K < k) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(fill_remote_kernel, dim3(4096), dim3(256), 0, s, main, m0, M, blk, b0, B, K, k,
+                       mods_dev, seed);
+    return hipGetLastError();
 }
 
 hipError_t launch_set_one(uint32_t* main, uint64_t index, int K, int k, int layout,

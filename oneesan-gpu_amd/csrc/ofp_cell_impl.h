@@ -121,6 +121,56 @@ struct Direct {
     OFP_HD void sb(uint64_t b, const Vals<KC>& v) const { store_vals<KC, L>(blk, b, K, off, v); }
 };
 
+// Synthetic residues of states that live on other shards: a seeded hash of the
+// global index (blocked words in their own domain), reduced mod p.  Used for
+// shard measurements; the CPU oracle evaluates the same function.
+OFP_HD uint64_t mix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+OFP_HD uint32_t remote_residue(uint64_t seed, uint64_t index, int j, uint32_t mod) {
+    return (uint32_t)(mix64(seed ^ (index * 0x100000001B3ull + (uint64_t)j)) >> 32) % mod;
+}
+constexpr uint64_t kBlockedDomain = 1ull << 62;
+
+// Memory policy of a shard launch (state-major layout): main words in the
+// resident window [m0, m0+M) and blocked words in [b0, b0+B) are in HBM; main
+// words outside the window belong to other shards and read as remote_residue().
+// A shard made of whole prefix classes owns every word a cell below its window
+// boundary writes, so stores outside the window do not occur; they are counted
+// (an invariant the tests check) and dropped.
+template <int KC>
+struct Windowed {
+    uint32_t* __restrict__ main;
+    uint32_t* __restrict__ blk;
+    uint64_t m0, M, b0;
+    uint64_t seed;
+    int K, off;
+    uint32_t mod[KC];
+    unsigned long long* stray;  // count of out-of-window stores (must stay 0)
+    OFP_HD void lm(uint64_t i, Vals<KC>& v) const {
+        if (i - m0 < M) load_vals<KC, 1>(main, i - m0, K, off, v);
+        else {
+#pragma unroll
+            for (int j = 0; j < KC; ++j) v.v[j] = remote_residue(seed, i, off + j, mod[j]);
+        }
+    }
+    OFP_HD void sm(uint64_t i, const Vals<KC>& v) const {
+        if (i - m0 < M) store_vals<KC, 1>(main, i - m0, K, off, v);
+        else {
+#ifdef __HIP_DEVICE_COMPILE__
+            atomicAdd(stray, 1ull);
+#else
+            ++*stray;
+#endif
+        }
+    }
+    OFP_HD void lb(uint64_t b, Vals<KC>& v) const { load_vals<KC, 1>(blk, b - b0, K, off, v); }
+    OFP_HD void sb(uint64_t b, const Vals<KC>& v) const { store_vals<KC, 1>(blk, b - b0, K, off, v); }
+};
+
 template <int KC, int L>
 struct Gather {
     Vals<KC> acc, pend;

@@ -48,11 +48,25 @@ struct CellLaunch {
     int layout;           // 0: 64-state tiles (AoSoA), 1: state-major (AoS)
     int variant;          // state-major only: 0 = orbit loads before closure scans, 1 = after
     uint32_t mod[kMaxKC];
+    // shard launches only (launch_cell_shard): resident main window and remote seed
+    uint64_t m0, M;
+    uint64_t seed;
+    unsigned long long* stray;
 };
 
 // Launch one cell step (all blocked words, moduli off..off+kc-1).
 hipError_t launch_cell(const CellLaunch& a, bool wide_index, int threads, int blocks,
                        hipStream_t s);
+// Launch one cell over the blocked words [first, first+nblk) of a shard whose
+// main window is [m0, m0+M) (state-major layout; main/blk point at the window
+// arrays).  Main words outside the window read as remote_residue(seed, ...).
+hipError_t launch_cell_shard(const CellLaunch& a, bool wide_index, int threads, int blocks,
+                             hipStream_t s);
+// Fill a shard window with remote_residue values: main words [m0, m0+M) and
+// blocked words [b0, b0+B) (blocked indices in their own hash domain).
+hipError_t launch_fill_remote(uint32_t* main, uint64_t m0, uint64_t M, uint32_t* blk, uint64_t b0,
+                              uint64_t B, int K, int k, const uint32_t* mods_dev, uint64_t seed,
+                              hipStream_t s);
 // Set main[(index, j)] = 1 mod moduli[j] for j < k (interleave stride K).
 hipError_t launch_set_one(uint32_t* main, uint64_t index, int K, int k, int layout,
                           const uint32_t* mods_dev, hipStream_t s);

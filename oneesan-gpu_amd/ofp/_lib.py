@@ -28,6 +28,7 @@ EXPORTS = (
     "ofp_row_sweep", "ofp_read_main", "ofp_set_launch", "ofp_set_variant", "ofp_set_low_cells", "ofp_set_layout", "ofp_state_counts",
     "ofp_last_error", "ofp_version", "ofp_destroy",
     "ofp_job_create", "ofp_job_enqueue", "ofp_job_info_get", "ofp_job_result", "ofp_job_destroy",
+    "ofp_shard_plan", "ofp_shard_sweep",
 )
 
 
@@ -87,6 +88,35 @@ class JobInfo(ctypes.Structure):
     ]
 
 
+class ShardSpec(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int),
+        ("k", ctypes.c_int),
+        ("moduli", ctypes.POINTER(ctypes.c_uint32)),
+        ("slices", ctypes.c_int),
+        ("slice", ctypes.c_int),
+        ("low_cells", ctypes.c_int),
+        ("seed", ctypes.c_uint64),
+    ]
+
+
+class ShardInfo(ctypes.Structure):
+    _fields_ = [
+        ("first_blocked", ctypes.c_uint64),
+        ("blocked_count", ctypes.c_uint64),
+        ("first_main", ctypes.c_uint64),
+        ("main_count", ctypes.c_uint64),
+        ("state_bytes", ctypes.c_uint64),
+        ("low_cells", ctypes.c_int),
+        ("launches", ctypes.c_int),
+        ("kernel_s", ctypes.c_double),
+        ("stray_stores", ctypes.c_uint64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -125,6 +155,8 @@ def load() -> ctypes.CDLL:
         "ofp_job_info_get": (ctypes.c_int32, [vp, ctypes.POINTER(JobInfo)]),
         "ofp_job_result": (ctypes.c_int32, [vp, u32p]),
         "ofp_job_destroy": (None, [vp]),
+        "ofp_shard_plan": (ctypes.c_int32, [vp, ctypes.POINTER(ShardSpec), ctypes.POINTER(ShardInfo)]),
+        "ofp_shard_sweep": (ctypes.c_int32, [vp, ctypes.POINTER(ShardSpec), ctypes.POINTER(ShardInfo), u32p, u32p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -244,6 +276,38 @@ class Context:
     def set_launch(self, threads: int, blocks_per_cu: int) -> None:
         self._check(self._lib.ofp_set_launch(self._h, threads, blocks_per_cu), "ofp_set_launch")
 
+    def _shard_spec(self, n, moduli, slices, slice_, low_cells, seed):
+        mods = _u32_array(moduli)
+        spec = ShardSpec(n, len(moduli), ctypes.cast(mods, ctypes.POINTER(ctypes.c_uint32)), slices, slice_,
+                         low_cells, seed)
+        return spec, mods
+
+    def shard_plan(self, n: int, moduli: Sequence[int], slices: int, slice_: int, low_cells: int = 0,
+                   seed: int = 0) -> dict:
+        spec, _keep = self._shard_spec(n, moduli, slices, slice_, low_cells, seed)
+        info = ShardInfo()
+        self._check(self._lib.ofp_shard_plan(self._h, ctypes.byref(spec), ctypes.byref(info)), "ofp_shard_plan")
+        return info.as_dict()
+
+    def shard_sweep(self, n: int, moduli: Sequence[int], slices: int, slice_: int, low_cells: int = 0,
+                    seed: int = 0, copy_out: bool = False):
+        """Sweep cells low_cells..1 over one shard; returns (info, main, blocked) (arrays if copy_out)."""
+        import numpy as np
+        spec, _keep = self._shard_spec(n, moduli, slices, slice_, low_cells, seed)
+        info = ShardInfo()
+        self._check(self._lib.ofp_shard_plan(self._h, ctypes.byref(spec), ctypes.byref(info)), "ofp_shard_plan")
+        k = len(moduli)
+        u32p = ctypes.POINTER(ctypes.c_uint32)
+        mo = bo = None
+        if copy_out:
+            mo = np.zeros((info.main_count, k), np.uint32)
+            bo = np.zeros((info.blocked_count, k), np.uint32)
+        st = self._lib.ofp_shard_sweep(self._h, ctypes.byref(spec), ctypes.byref(info),
+                                       mo.ctypes.data_as(u32p) if copy_out else None,
+                                       bo.ctypes.data_as(u32p) if copy_out else None)
+        self._check(st, "ofp_shard_sweep")
+        return info.as_dict(), mo, bo
+
     def set_variant(self, variant: int) -> None:
         """Cell-kernel variant (state-major layout): 0 = early orbit loads, 1 = late."""
         self._check(self._lib.ofp_set_variant(self._h, variant), "ofp_set_variant")
@@ -290,6 +354,18 @@ class Job:
             self.close()
         except Exception:
             pass
+
+
+def shard_plan(n: int, k: int, slices: int, slice_: int, low_cells: int = 0) -> dict:
+    """Window of one shard (host computation, no GPU needed)."""
+    lib = load()
+    mods = _u32_array([4294967291] * k)
+    spec = ShardSpec(n, k, ctypes.cast(mods, ctypes.POINTER(ctypes.c_uint32)), slices, slice_, low_cells, 0)
+    info = ShardInfo()
+    st = lib.ofp_shard_plan(None, ctypes.byref(spec), ctypes.byref(info))
+    if st != OFP_OK:
+        raise OfpError(st, f"shard_plan(n={n}, k={k}, slices={slices}, slice={slice_})")
+    return info.as_dict()
 
 
 def version() -> str:

@@ -49,6 +49,9 @@ def load(auto_build: bool = True) -> ctypes.CDLL:
                                           ctypes.c_int, ctypes.c_int, ctypes.c_int]),
         "or_solve": (ctypes.c_uint32, [ctypes.c_int, ctypes.c_uint32, ctypes.c_int, ip]),
         "or_num_threads": (ctypes.c_int, []),
+        "or_remote_residue": (ctypes.c_uint32, [u64, u64, ctypes.c_int, ctypes.c_uint32]),
+        "or_shard_cells": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint32, ctypes.c_int, u64, u64, u64, u64, u64,
+                                          ctypes.c_int, ctypes.c_int, u32p, u32p]),
         "xdp_count_exact": (ctypes.c_int, [ctypes.c_int, ctypes.c_char_p, ctypes.c_int]),
         "xdp_count_mod": (ctypes.c_uint32, [ctypes.c_int, ctypes.c_uint32, ip]),
     }
@@ -95,3 +98,18 @@ def apply_cells(n: int, mod: int, main, blk, p_hi: int, p_lo: int, blocked_valid
                             p_hi, p_lo, 1 if blocked_valid else 0, 1 if pull else 0)
     if rc:
         raise MemoryError("oracle apply_cells failed")
+
+
+def shard_cells(n: int, mod: int, j: int, seed: int, b0: int, B: int, m0: int, M: int,
+                p_hi: int, p_lo: int):
+    """Windowed shard semantics (see or_shard_cells); returns (main_window, blocked_window)."""
+    import numpy as np
+    lib = load()
+    mw = np.zeros(max(M, 1), np.uint32)
+    bw = np.zeros(max(B, 1), np.uint32)
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    rc = lib.or_shard_cells(n, mod, j, seed, b0, B, m0, M, p_hi, p_lo, mw.ctypes.data_as(u32p),
+                            bw.ctypes.data_as(u32p))
+    if rc:
+        raise ValueError(f"or_shard_cells failed ({rc})")
+    return mw[:M], bw[:B]

@@ -366,3 +366,81 @@ int or_num_threads(void) {
     return 1;
 #endif
 }
+
+/* ---- shard semantics (windowed), for checking one GPU's share ----
+ * Synthetic residues of words that live on other shards: a seeded hash of the
+ * global index (blocked words in their own domain, bit 62), reduced mod p --
+ * the same definition the product's shard sweep uses.  */
+static uint64_t or_mix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+uint32_t or_remote_residue(uint64_t seed, uint64_t index, int j, uint32_t mod) {
+    return (uint32_t)(or_mix64(seed ^ (index * 0x100000001B3ull + (uint64_t)j)) >> 32) % mod;
+}
+
+/* Cells p_hi..p_lo on a shard: main window [m0, m0+M), blocked words
+ * [b0, b0+B).  Window values start as the synthetic residues of their global
+ * indices (modulus index j); reads outside the main window return the
+ * synthetic residue.  Each cell is the pull form of the reference recurrence
+ * over the words of the window (main targets gather identity + included-edge
+ * predecessors + blocked re-expansion; blocked targets gather
+ * reverse_block_predecessors).  The blocked input of cell W-1 is zero; cell 1
+ * leaves the blocked words as they were.  Returns -1 if a blocked read falls
+ * outside [b0, b0+B) (a shard must be whole prefix classes). */
+int or_shard_cells(int n, uint32_t mod, int j, uint64_t seed, uint64_t b0, uint64_t B, uint64_t m0,
+                   uint64_t M, int p_hi, int p_lo, uint32_t* main_win, uint32_t* blk_win) {
+    build_dp();
+    const int W = n + 1;
+    const uint64_t dom = 1ull << 62;
+    for (uint64_t i = 0; i < M; ++i) main_win[i] = or_remote_residue(seed, m0 + i, j, mod);
+    for (uint64_t b = 0; b < B; ++b) blk_win[b] = or_remote_residue(seed, dom + b0 + b, j, mod);
+    uint32_t* nm = (uint32_t*)malloc((M ? M : 1) * sizeof(uint32_t));
+    uint32_t* nb = (uint32_t*)malloc((B ? B : 1) * sizeof(uint32_t));
+    if (!nm || !nb) { free(nm); free(nb); return -2; }
+    int bad = 0;
+    for (int p = p_hi; p >= p_lo; --p) {
+        const int read_blk = (p != W - 1);
+#define OLD(ix) (((uint64_t)(ix) - m0 < M) ? main_win[(uint64_t)(ix) - m0] : or_remote_residue(seed, (ix), j, mod))
+        for (uint64_t i = 0; i < M; ++i) {
+            word_t t = or_unrank(m0 + i, W);
+            uint32_t acc = main_win[i];
+            word_t pred[OR_MAXW * 2 + 4];
+            int np = 0;
+            if (p == 1) np = or_reverse_boundary_preds(t, W, pred, OR_MAXW * 2 + 4);
+            else {
+                switch (mpair(t, p)) {
+                case PAIR(VL, VR): pred[np++] = msetpair(t, p, PAIR(VN, VN)); break;
+                case PAIR(VN, VR): pred[np++] = msetpair(t, p, PAIR(VR, VN)); break;
+                case PAIR(VN, VL): pred[np++] = msetpair(t, p, PAIR(VL, VN)); break;
+                default: break;
+                }
+            }
+            for (int q = 0; q < np; ++q) acc = addm(acc, OLD(or_rank(pred[q], W)), mod);
+            if (read_blk && mget(t, p) == VN) {
+                uint64_t bi = or_rank(mshrink(t, p), W - 1);
+                if (bi - b0 >= B) bad = 1;
+                else acc = addm(acc, blk_win[bi - b0], mod);
+            }
+            nm[i] = acc;
+        }
+        if (p >= 2) {
+            for (uint64_t b = 0; b < B; ++b) {
+                word_t bw = or_unrank(b0 + b, W - 1);
+                word_t pred[OR_MAXW * 2 + 4];
+                int np = or_reverse_block_preds(bw, W, p, pred, OR_MAXW * 2 + 4);
+                uint32_t acc = 0;
+                for (int q = 0; q < np; ++q) acc = addm(acc, OLD(or_rank(pred[q], W)), mod);
+                nb[b] = acc;
+            }
+            memcpy(blk_win, nb, B * sizeof(uint32_t));
+        }
+#undef OLD
+        memcpy(main_win, nm, M * sizeof(uint32_t));
+    }
+    free(nm);
+    free(nb);
+    return bad ? -1 : 0;
+}
