@@ -404,7 +404,9 @@ int or_shard_cells(int n, uint32_t mod, int j, uint64_t seed, uint64_t b0, uint6
     for (int p = p_hi; p >= p_lo; --p) {
         const int read_blk = (p != W - 1);
 #define OLD(ix) (((uint64_t)(ix) - m0 < M) ? main_win[(uint64_t)(ix) - m0] : or_remote_residue(seed, (ix), j, mod))
-        for (uint64_t i = 0; i < M; ++i) {
+#pragma omp parallel for schedule(dynamic, 1024) reduction(| : bad)
+        for (int64_t ii = 0; ii < (int64_t)M; ++ii) {
+            const uint64_t i = (uint64_t)ii;
             word_t t = or_unrank(m0 + i, W);
             uint32_t acc = main_win[i];
             word_t pred[OR_MAXW * 2 + 4];
@@ -427,7 +429,9 @@ int or_shard_cells(int n, uint32_t mod, int j, uint64_t seed, uint64_t b0, uint6
             nm[i] = acc;
         }
         if (p >= 2) {
-            for (uint64_t b = 0; b < B; ++b) {
+#pragma omp parallel for schedule(dynamic, 1024)
+            for (int64_t bb = 0; bb < (int64_t)B; ++bb) {
+                const uint64_t b = (uint64_t)bb;
                 word_t bw = or_unrank(b0 + b, W - 1);
                 word_t pred[OR_MAXW * 2 + 4];
                 int np = or_reverse_block_preds(bw, W, p, pred, OR_MAXW * 2 + 4);
