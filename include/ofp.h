@@ -113,8 +113,10 @@ ofp_status ofp_set_launch(ofp_ctx* ctx, int threads, int blocks_per_cu);
 
 /* Cell-kernel variant for the state-major layout: 0 = an x=N thread loads its
  * own orbit values before scanning for closure predecessors, 1 = after (fewer
- * registers live during the scans).  Set by the autotuner; no reference
- * counterpart. */
+ * registers live during the scans), 2 = one branch-free orbit path for every
+ * kind of blocked word (all lanes' loads issued before the scans), 3 = as 2
+ * with the orbit results stored before the scans (fewer live registers).  Set
+ * by the autotuner; no reference counterpart. */
 ofp_status ofp_set_variant(ofp_ctx* ctx, int variant);
 
 /* Moduli per launch on the low cells: cells p <= low_cells advance at most

@@ -429,7 +429,7 @@ ofp_status ofp_set_launch(ofp_ctx* c, int threads, int blocks_per_cu) {
 
 ofp_status ofp_set_variant(ofp_ctx* c, int variant) {
     if (!c) return OFP_E_ARG;
-    if (variant != 0 && variant != 1) return fail(c, OFP_E_ARG, "variant must be 0 or 1");
+    if (variant < 0 || variant > 3) return fail(c, OFP_E_ARG, "variant must be 0..3");
     if (variant != c->variant) drop_graph(c->st);
     c->variant = variant;
     return OFP_OK;

@@ -113,6 +113,10 @@ hipError_t launch_kc(const CellLaunch& a, int threads, int blocks, hipStream_t s
     if (a.layout == kAoS) {
         if (a.variant == 1)
             hipLaunchKernelGGL((cell_kernel<KC, Idx, kAoS, 1>), dim3(blocks), dim3(threads), 0, s, a);
+        else if (a.variant == 2)
+            hipLaunchKernelGGL((cell_kernel<KC, Idx, kAoS, 2>), dim3(blocks), dim3(threads), 0, s, a);
+        else if (a.variant == 3)
+            hipLaunchKernelGGL((cell_kernel<KC, Idx, kAoS, 3>), dim3(blocks), dim3(threads), 0, s, a);
         else
             hipLaunchKernelGGL((cell_kernel<KC, Idx, kAoS, 0>), dim3(blocks), dim3(threads), 0, s, a);
     } else {
@@ -125,6 +129,10 @@ template <typename Idx, int KC>
 hipError_t launch_shard_kc(const CellLaunch& a, int threads, int blocks, hipStream_t s) {
     if (a.variant == 1)
         hipLaunchKernelGGL((cell_shard_kernel<KC, Idx, 1>), dim3(blocks), dim3(threads), 0, s, a);
+    else if (a.variant == 2)
+        hipLaunchKernelGGL((cell_shard_kernel<KC, Idx, 2>), dim3(blocks), dim3(threads), 0, s, a);
+    else if (a.variant == 3)
+        hipLaunchKernelGGL((cell_shard_kernel<KC, Idx, 3>), dim3(blocks), dim3(threads), 0, s, a);
     else
         hipLaunchKernelGGL((cell_shard_kernel<KC, Idx, 0>), dim3(blocks), dim3(threads), 0, s, a);
     return hipGetLastError();

@@ -194,9 +194,66 @@ struct Gather {
     }
 };
 
+// Closure predecessors of the blocked word [P, N, S] (x = N): RL, LL and RR
+// sources, accumulated into g (the reverse of gridfp_transition.hpp:113-146's
+// closure cases; order and multiplicity as gridfp_reverse.hpp:39-98).
+template <int KC, typename Idx, int L, class Mem>
+OFP_HD void gather_closures(const Tab<Idx>& T, int W, int p, int hP, Idx mainbase, Idx sr, uint64_t word,
+                            const Mem& m, Gather<KC, L>& g, const uint32_t* mod) {
+    if (p >= 2) {
+        // RL predecessor: [P,R,L,S]
+        if (hP > 0) {
+            const uint64_t iRL = (uint64_t)(mainbase + T.at(p, hP) + T.cL(p - 1, hP - 1) + sr);
+            g.add(m, iRL, mod);
+        }
+        // LL predecessors: [P,L,L,S*], S* = S with a top-level L at q turned into R,
+        // entered at height hP+2 (reverse_block_predecessors, gridfp_reverse.hpp:62-68)
+        if (p >= 3) {
+            const Idx baseLL = mainbase + T.cL(p, hP) + T.cL(p - 1, hP + 1);
+            Idx rr = sr, s0 = 0, s2 = 0;
+            int hh = hP, depth = 1;
+            for (int pos = p - 2; pos >= 0; --pos) {
+                Idx c0;
+                const int v = T.step(pos, hh, rr, c0);
+                const Quad<Idx> e2 = T.q[pos][hh + 2];
+                if (v == 2 && depth == 1)
+                    g.add(m, (uint64_t)(baseLL + s2 + e2.zn + (sr - s0 - c0)), mod);
+                s0 += c0;
+                s2 += (v == 0) ? Idx(0) : (v == 1 ? e2.zn : Idx(e2.zn + e2.zr));
+                if (v == 2) ++depth;
+                else if (v == 1) { if (--depth == 0) break; }
+                hh += (v == 2) - (v == 1);
+            }
+        }
+    }
+    // RR predecessors: [P*,R,R,S], P* = P with a top-level R at q turned into L
+    // (gridfp_reverse.hpp:70-82); at p=1 they are the closure sources of NN.
+    {
+        const Idx tail = T.at(p, hP + 2) + T.at(p - 1, hP + 1) + sr;
+        Idx u0 = 0, u2 = 0;
+        int depth = 1, Ha = hP;
+        for (int mp = p + 1; mp <= W - 1; ++mp) {
+            const int v = (int)((word >> (2 * (mp - 1))) & 3u);
+            const int H = Ha + (v == 1) - (v == 2);
+            const Quad<Idx> e0 = T.q[mp][H];
+            const Quad<Idx> e2 = T.q[mp][H + 2];
+            const Idx c0 = (v == 0) ? Idx(0) : (v == 1 ? e0.zn : Idx(e0.zn + e0.zr));
+            if (v == 1 && depth == 1)
+                g.add(m, (uint64_t)(mainbase - (u0 + c0) + e0.zn + e0.zr + u2 + tail), mod);
+            u0 += c0;
+            u2 += (v == 0) ? Idx(0) : (v == 1 ? e2.zn : Idx(e2.zn + e2.zr));
+            if (v == 2) { if (--depth == 0) break; }
+            else if (v == 1) ++depth;
+            Ha = H;
+        }
+    }
+}
+
 // V selects when an x=N thread loads its own orbit values (NN, LR, blocked-in):
 // 0 = before the closure scans (more loads in flight, more registers live during
-// the scans), 1 = after them (fewer live registers, so more waves per SIMD).
+// the scans), 1 = after them (fewer live registers, so more waves per SIMD),
+// 2 = one branch-free orbit path for all three kinds of blocked word, loads first,
+// 3 = as 2 but the orbit results are stored before the closure scans.
 template <int KC, typename Idx, int L, int V, class Mem>
 OFP_HD void cell_one(const CellLaunch& a, const Tab<Idx>& T, uint64_t bidx, const Mem& m) {
     const int W = a.W, p = a.p;
@@ -222,6 +279,86 @@ OFP_HD void cell_one(const CellLaunch& a, const Tab<Idx>& T, uint64_t bidx, cons
     const int x = T.step(p - 1, hP, r, cx);
     const Idx sr = r;  // rank of S among suffixes entered at the height after x
 
+    if constexpr (V >= 2) {
+        // One code path for the three orbits: every lane issues its orbit loads
+        // (A, B and blocked-in) before the x = N lanes scan for closure sources,
+        // so at small p, where a wave holds all three kinds of blocked word, the
+        // loads of R and L lanes overlap the scans instead of following them.
+        //   x = N: A = NN, B = LR;  A' = NN + blk (+ closures at p = 1), B' = LR + NN
+        //   x = R: A = NR, B = RN;  A' = NR + RN + blk, at p = 1 B' = RN + NR
+        //   x = L: A = NL, B = LN;  A' = NL + LN + blk
+        // blocked-out (p >= 2) = closure sum for x = N, old A otherwise.
+        const Idx cLp = T.cL(p, hP);
+        Idx offA, offB;
+        if (x == 0) {
+            offA = 0;
+            offB = cLp + T.at(p - 1, hP + 1);
+        } else if (x == 1) {
+            offA = T.at(p - 1, hP);
+            offB = T.at(p, hP);
+        } else {
+            offA = T.cL(p - 1, hP);
+            offB = cLp;
+        }
+        const uint64_t iA = (uint64_t)(mainbase + offA + sr);
+        const uint64_t iB = (uint64_t)(mainbase + offB + sr);
+        Vals<KC> va, vb, bc;
+        m.lm(iA, va);
+        m.lm(iB, vb);
+        if (a.read_blk) m.lb(bidx, bc);
+        if constexpr (V == 3) {
+            // Early retire: the orbit words and the R/L lanes' blocked-out are
+            // stored before the closure scans, so only the closure sum (and, at
+            // p = 1, NN's partial sum) stays live across them.
+            Vals<KC> oa, ob;
+#pragma unroll
+            for (int j = 0; j < KC; ++j) {
+                uint32_t t = va.v[j];
+                if (a.read_blk) t = addm(t, bc.v[j], mod[j]);
+                if (x != 0) t = addm(t, vb.v[j], mod[j]);
+                oa.v[j] = t;
+                ob.v[j] = addm(vb.v[j], va.v[j], mod[j]);
+            }
+            if (x == 0 || p == 1) m.sm(iB, ob);  // x = L does not occur at p = 1
+            if (x != 0 || p != 1) m.sm(iA, oa);
+            if (x != 0 && p >= 2) m.sb(bidx, va);
+            if (x == 0) {
+                Gather<KC, L> g;
+                g.init();
+                gather_closures<KC, Idx, L>(T, W, p, hP, mainbase, sr, word, m, g, mod);
+                g.finish(mod);
+                if (p == 1) {
+                    add_into<KC>(oa, g.acc, mod);
+                    m.sm(iA, oa);
+                } else {
+                    m.sb(bidx, g.acc);
+                }
+            }
+            return;
+        }
+        Gather<KC, L> g;
+        g.init();
+        if (x == 0) gather_closures<KC, Idx, L>(T, W, p, hP, mainbase, sr, word, m, g, mod);
+        g.finish(mod);
+        Vals<KC> oa, ob;
+#pragma unroll
+        for (int j = 0; j < KC; ++j) {
+            uint32_t t = va.v[j];
+            if (a.read_blk) t = addm(t, bc.v[j], mod[j]);
+            if (x != 0) t = addm(t, vb.v[j], mod[j]);
+            else if (p == 1) t = addm(t, g.acc.v[j], mod[j]);
+            oa.v[j] = t;
+            ob.v[j] = addm(vb.v[j], va.v[j], mod[j]);
+        }
+        m.sm(iA, oa);
+        if (x == 0 || p == 1) m.sm(iB, ob);  // x = L does not occur at p = 1
+        if (p >= 2) {
+#pragma unroll
+            for (int j = 0; j < KC; ++j) ob.v[j] = (x == 0) ? g.acc.v[j] : va.v[j];
+            m.sb(bidx, ob);
+        }
+        return;
+    }
     if (x == 0) {
         // orbit NN -> LR
         const uint64_t iNN = (uint64_t)(mainbase + sr);
@@ -234,53 +371,7 @@ OFP_HD void cell_one(const CellLaunch& a, const Tab<Idx>& T, uint64_t bidx, cons
         }
         Gather<KC, L> g;
         g.init();
-        if (p >= 2) {
-            // RL predecessor: [P,R,L,S]
-            if (hP > 0) {
-                const uint64_t iRL = (uint64_t)(mainbase + T.at(p, hP) + T.cL(p - 1, hP - 1) + sr);
-                g.add(m, iRL, mod);
-            }
-            // LL predecessors: [P,L,L,S*], S* = S with a top-level L at q turned into R,
-            // entered at height hP+2 (reverse_block_predecessors, gridfp_reverse.hpp:62-68)
-            if (p >= 3) {
-                const Idx baseLL = mainbase + T.cL(p, hP) + T.cL(p - 1, hP + 1);
-                Idx rr = sr, s0 = 0, s2 = 0;
-                int hh = hP, depth = 1;
-                for (int pos = p - 2; pos >= 0; --pos) {
-                    Idx c0;
-                    const int v = T.step(pos, hh, rr, c0);
-                    const Quad<Idx> e2 = T.q[pos][hh + 2];
-                    if (v == 2 && depth == 1)
-                        g.add(m, (uint64_t)(baseLL + s2 + e2.zn + (sr - s0 - c0)), mod);
-                    s0 += c0;
-                    s2 += (v == 0) ? Idx(0) : (v == 1 ? e2.zn : Idx(e2.zn + e2.zr));
-                    if (v == 2) ++depth;
-                    else if (v == 1) { if (--depth == 0) break; }
-                    hh += (v == 2) - (v == 1);
-                }
-            }
-        }
-        // RR predecessors: [P*,R,R,S], P* = P with a top-level R at q turned into L
-        // (gridfp_reverse.hpp:70-82); at p=1 they are the closure sources of NN.
-        {
-            const Idx tail = T.at(p, hP + 2) + T.at(p - 1, hP + 1) + sr;
-            Idx u0 = 0, u2 = 0;
-            int depth = 1, Ha = hP;
-            for (int mp = p + 1; mp <= W - 1; ++mp) {
-                const int v = (int)((word >> (2 * (mp - 1))) & 3u);
-                const int H = Ha + (v == 1) - (v == 2);
-                const Quad<Idx> e0 = T.q[mp][H];
-                const Quad<Idx> e2 = T.q[mp][H + 2];
-                const Idx c0 = (v == 0) ? Idx(0) : (v == 1 ? e0.zn : Idx(e0.zn + e0.zr));
-                if (v == 1 && depth == 1)
-                    g.add(m, (uint64_t)(mainbase - (u0 + c0) + e0.zn + e0.zr + u2 + tail), mod);
-                u0 += c0;
-                u2 += (v == 0) ? Idx(0) : (v == 1 ? e2.zn : Idx(e2.zn + e2.zr));
-                if (v == 2) { if (--depth == 0) break; }
-                else if (v == 1) ++depth;
-                Ha = H;
-            }
-        }
+        gather_closures<KC, Idx, L>(T, W, p, hP, mainbase, sr, word, m, g, mod);
         g.finish(mod);
         if constexpr (V == 1) {
             m.lm(iNN, nn);

@@ -29,7 +29,7 @@ BLOCKS_PER_CU = (4, 8, 16, 32)
 LAYOUTS = (0, 1)
 # second stage: cells p <= low_cells advance at most low_kc moduli per launch
 LOW_CELLS = ((0, 16), (6, 8), (4, 8))
-VARIANTS = (0, 1)
+VARIANTS = (0, 1, 2, 3)
 
 
 def _key(n: int, k: int) -> str:

@@ -46,8 +46,9 @@ def test_every_cell_random_vectors(gpu_ctx, oracle, n):
                 assert np.array_equal(g_blk[:, j], eb), (n, p, mod, "blocked")
 
 
-@pytest.fixture(scope="module", params=[(0, 0), (1, 0), (1, 1)],
-                ids=["tiles64", "statemajor-early", "statemajor-late"])
+@pytest.fixture(scope="module", params=[(0, 0), (1, 0), (1, 1), (1, 2), (1, 3)],
+                ids=["tiles64", "statemajor-early", "statemajor-late", "statemajor-uniform",
+                     "statemajor-early-retire"])
 def layout_ctx(request):
     import ofp
     layout, variant = request.param

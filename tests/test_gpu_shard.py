@@ -8,9 +8,19 @@ pytestmark = pytest.mark.gpu
 MODS = [4294967291, 4294966997, 2147483647, 1000003]
 
 
+@pytest.mark.parametrize("variant", [0, 3])
 @pytest.mark.parametrize("n,slices,picks", [(12, 1, [0]), (12, 5, [0, 2, 4]), (16, 16, [3, 15]),
                                             (27, 1 << 30, [7, (1 << 29) + 12345])])
-def test_shard_sweep_matches_oracle(gpu_ctx, oracle, n, slices, picks):
+def test_shard_sweep_matches_oracle(gpu_ctx, oracle, n, slices, picks, variant):
+    import ofp
+    gpu_ctx.set_variant(variant)
+    try:
+        _check_slices(gpu_ctx, oracle, n, slices, picks)
+    finally:
+        gpu_ctx.set_variant(0)
+
+
+def _check_slices(gpu_ctx, oracle, n, slices, picks):
     import ofp
     for s in picks:
         # a slice narrower than a prefix class is empty by construction: take the
