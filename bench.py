@@ -11,12 +11,18 @@ against the reference's published value for the first prime.
 Workload: n=22, k = number of primes the checkerboard-strip bound needs (14),
 the largest primes below 2^32 -- an exact-count (CRT) run.  Inputs: the
 solve's own initial state (synthetic in the sense that nothing is read from
-disk).  Also measured in the same process:
-  * plain: the same sweep with k=1 (one residue at a time) on a few rows,
-    extrapolated to a full solve -> residues/hour of the plain form;
+disk).  The first rows and the last two rows of a solve sweep only the words
+of small height (height caps; the answer is unchanged), so rows differ in
+cost: residues/hour comes from the time of the whole solve (initialisation,
+warmup row and timed rows), and `value` counts every row at its full
+algorithmic work W-1 cells x S states x k moduli.  Also measured in the same
+process:
+  * plain: the same solve with k=1 (one residue at a time), timed whole
+    -> residues/hour of the plain form;
   * roofline of the cell kernel: declared algorithmic bytes per launch
     (8 B per state per modulus: read + write of one uint32 residue) divided by
-    the average launch duration from HIP events on the library's stream;
+    the average duration of the full-state launches (rows without capped
+    cells), from HIP events on the library's stream;
   * cpu_baseline: the CPU oracle's pull-form step timed on the host cores on
     a bounded sample of the same n=22 cell step.
 
@@ -129,7 +135,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1, help="untimed row sweeps before timing")
     ap.add_argument("--n", type=int, default=22)
     ap.add_argument("--k", type=int, default=0, help="moduli (0 = as many as the CRT bound needs)")
-    ap.add_argument("--plain-rows", type=int, default=2, help="rows timed for the k=1 plain form")
+    ap.add_argument("--no-plain", action="store_true", help="skip the k=1 plain-form solve")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-shard", action="store_true", help="skip the n=27 one-GPU-share measurement")
@@ -162,10 +168,12 @@ def main() -> int:
     log(f"bench: launch config {tuned}")
     plan = ctx.plan(n, k)
     log(f"bench: state {plan.state_bytes / 2**30:.1f} GiB, free {plan.free_bytes / 2**30:.1f} GiB")
+    tw0 = time.perf_counter()
     ctx.prepare(n, mods)
     if args.warmup:
         ctx.row_sweep(args.warmup)
         log(f"bench: warmup {args.warmup} rows done")
+    warm_s = time.perf_counter() - tw0
     # The library synchronises its stream before returning from row_sweep, so the
     # timed region is bracketed by device synchronisation on both sides.
     t0 = time.perf_counter()
@@ -184,29 +192,35 @@ def main() -> int:
     transitions = st["transitions"]
     value = transitions / el
     ms_per_step = el / args.steps * 1e3
-    solve_s = el / args.steps * W
+    whole = args.warmup + args.steps == W
+    # a whole solve: initialisation + warmup + timed rows; otherwise extrapolated
+    solve_s = (warm_s + el) if whole else el / args.steps * W
     residues_per_hour = 3600.0 * k / solve_s
-    launches = st["launches"]
-    avg_launch_s = st["kernel_s"] / launches
+    if st["full_launches"]:
+        avg_launch_s = st["full_kernel_s"] / st["full_launches"]
+        launch_basis = f"{st['full_launches']} full-state launches (rows without capped cells)"
+    else:
+        avg_launch_s = st["kernel_s"] / st["launches"]
+        launch_basis = f"all {st['launches']} launches (no row of the timed region ran uncapped)"
     bytes_per_launch = 8.0 * S * min(k, 16)  # one launch carries up to 16 moduli
     achieved = bytes_per_launch / avg_launch_s / 1e9
 
-    # plain one-residue-at-a-time form (k=1), a few rows, extrapolated to a solve
+    # plain one-residue-at-a-time form (k=1): one whole solve
     plain = None
-    if args.plain_rows > 0:
+    if not args.no_plain:
         autotune.apply(ctx, n, 1)
-        ctx.prepare(n, [mods[0]])
-        ctx.row_sweep(1)
         tp0 = time.perf_counter()
-        sp = ctx.row_sweep(args.plain_rows)
+        ctx.prepare(n, [mods[0]])
+        sp = ctx.row_sweep(W)
         tp = time.perf_counter() - tp0
-        plain_solve_s = tp / args.plain_rows * W
-        plain = {"rows_timed": args.plain_rows, "ms_per_row": tp / args.plain_rows * 1e3,
-                 "solve_s_extrapolated": plain_solve_s,
-                 "residues_per_hour": 3600.0 / plain_solve_s,
+        pres = ctx.read_main(0, 1)
+        plain = {"solve_s": tp, "residues_per_hour": 3600.0 / tp,
                  "transitions_per_s": sp["transitions"] / tp,
-                 "avg_launch_ms": sp["kernel_s"] / sp["launches"] * 1e3}
-        log(f"bench: plain k=1 {plain['ms_per_row']:.1f} ms/row -> {plain['residues_per_hour']:.0f} residues/h")
+                 "full_row_ms": (sp["full_kernel_s"] / sp["full_launches"] * (W - 1) * 1e3
+                                 if sp["full_launches"] else None),
+                 "residue": pres[0], "residue_matches_batched": residues is not None and pres[0] == residues[0]}
+        log(f"bench: plain k=1 solve {tp:.2f} s -> {plain['residues_per_hour']:.0f} residues/h "
+            f"(residue {pres[0]})")
     ctx.close()
 
     shard = None
@@ -250,7 +264,8 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "cell_kernel", "bytes_per_launch": bytes_per_launch,
-                     "avg_launch_ms": avg_launch_s * 1e3},
+                     "avg_launch_ms": avg_launch_s * 1e3, "launch_basis": launch_basis,
+                     "capped_kernel_s": st["capped_kernel_s"], "capped_launches": st["capped_launches"]},
         "cpu_baseline": cpu,
         "shard_n27": shard,
     }

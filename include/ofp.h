@@ -66,6 +66,10 @@ typedef struct ofp_stats {
     uint64_t transitions;     /* frontier transitions performed */
     uint64_t main_states, blocked_states;
     int rows, cells_per_row, launches;
+    /* kernel time split: runs of rows whose cells all sweep the full state
+     * (full_*) and rows with height-capped cells (capped_*) */
+    double full_kernel_s, capped_kernel_s;
+    int full_launches, capped_launches;
 } ofp_stats;
 
 /* Create a context on cfg->device (NULL = defaults).  Replaces the
@@ -118,6 +122,21 @@ ofp_status ofp_set_launch(ofp_ctx* ctx, int threads, int blocks_per_cu);
  * with the orbit results stored before the scans (fewer live registers).  Set
  * by the autotuner; no reference counterpart. */
 ofp_status ofp_set_variant(ofp_ctx* ctx, int variant);
+
+/* Height caps (default on).  In a sweep from the initial state (ofp_solve,
+ * ofp_prepare + ofp_row_sweep, jobs) the cells of the first rows only touch
+ * words of small height (every other word is still zero), and in the last two
+ * rows only such words can still reach the answer word; capped cells enumerate
+ * just those blocked words.  The answer is the same; other main words read
+ * after the last two rows are not final.  Reference analogue: the row-6 prefix
+ * initialiser and the mirrored last row (row6crt20.cu:1434-1688,1787-1814,
+ * 935-952), which likewise skip the first and last rows' full sweeps. */
+ofp_status ofp_set_height_caps(ofp_ctx* ctx, int on);
+
+/* The height cap cell p of row `row` of an n-solve runs with (0: the whole
+ * state), and the number of blocked words it sweeps.  Host-only, no GPU.
+ * Returns -1 for n out of range. */
+int ofp_height_cap(int n, int row, int p, uint64_t* capped_blocked);
 
 /* Moduli per launch on the low cells: cells p <= low_cells advance at most
  * low_kc moduli per launch (more launches, fewer registers per thread); the

@@ -37,6 +37,8 @@ struct Sweep {
     hipGraph_t row_graph = nullptr;
     std::string row_key;
     bool async_alloc = false;  // buffers come from the stream-ordered allocator (jobs)
+    int row = -1;              // rows swept since the initial state; -1: state of unknown origin
+    std::vector<hipEvent_t> seg_ev;  // timing events of row segments (ofp_row_sweep)
 };
 
 struct ofp_ctx {
@@ -54,6 +56,9 @@ struct ofp_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     void* dp32 = nullptr;  // packed uint32 height table (n <= 22)
     void* dp64 = nullptr;  // packed uint64 height table
+    void* dpc32 = nullptr; // packed capped tables, caps 1..kMaxCap (uint32, uint64)
+    void* dpc64 = nullptr;
+    int height_caps = 1;   // skip the blocked words a cell provably leaves unchanged or unread
     std::string err;
     Sweep st;              // the context's own state (solve / prepare / row_sweep)
     int jobs = 0;          // live jobs (serving)
@@ -130,6 +135,8 @@ void sweep_free(Sweep& s, void* p) {
 
 void free_sweep(Sweep& s) {
     drop_graph(s);
+    for (hipEvent_t e : s.seg_ev) (void)hipEventDestroy(e);
+    s.seg_ev.clear();
     sweep_free(s, s.main);
     sweep_free(s, s.blk);
     sweep_free(s, s.mods_dev);
@@ -195,11 +202,62 @@ ofp_status ensure_state(ofp_ctx* c, Sweep& s, int n, const uint32_t* moduli, int
     return OFP_OK;
 }
 
-// Enqueue cells p_hi..p_lo of s on s.stream.
-ofp_status enqueue_cells(ofp_ctx* c, Sweep& s, int p_hi, int p_lo, bool first_reads_blk, int* launches) {
+// Height caps.  H(w) is the largest height of a word w (the entry height 1
+// included).  A thread of cell p reads and writes only words with H >= H(b) of
+// its blocked word b (its orbit words, its closure sources and b itself), so:
+//  * forward: if every word with H > c holds zero before the cell, the threads
+//    with H(b) > c would only add zeros to zeros;
+//  * backward: if no word with H > c after the cell can reach the answer word,
+//    the outputs of threads with H(b) > c are never read by anything that does;
+// either way they can be skipped.  Bounds, for a sweep from the initial state
+// (rows r = 0..W-1, cells p = W-1..1):
+//  * forward, before cell p of row r: H <= 1 in row 0; in row r >= 1, H <= r at
+//    p >= W - r and H <= r + 1 below (a frontier word nests one more arc level
+//    only after a further grid row; the kink enters the new level at cell W-r);
+//  * backward, after cell p: in the last row only words with H <= 1 still reach
+//    the answer (an all-N word with a single R at position 0); in row W-2,
+//    after cell p (p >= 2) only those with H <= 2*ceil((p-1)/2)+1, after cell 1
+//    only H <= 1.
+// Both bounds were checked against the oracle's cell-by-cell sweep at n = 6..14
+// (all zeros forward; backward, randomising every word above the bound leaves
+// the answer unchanged), and the solves they speed up are checked end to end.
+// Returns 0 when the cell runs uncapped.
+int cell_height_cap(int W, int r, int p) {
+    if (r < 0 || r >= W || p < 1 || p > W - 1) return 0;
+    int cap = (r == 0) ? 1 : (p >= W - r ? r : r + 1);
+    if (r == W - 1) cap = 1;
+    else if (r == W - 2) cap = std::min(cap, p == 1 ? 1 : 2 * (p / 2) + 1);
+    return cap <= kMaxCap ? cap : 0;
+}
+
+// Blocked words (width W-1, entered at height 1) whose heights stay <= cap.
+uint64_t capped_blocked_count(int W, int cap) {
+    static thread_local uint64_t d[kDpRows][kDpCols];
+    build_capped_table(cap, d);
+    return d[W - 1][1];
+}
+
+// The cap a cell runs with: the bound above, when the capped kernel has few
+// enough words to pay for its longer decode (<= 85% of the blocked words).
+int effective_cap(int W, int r, int p, uint64_t nblk, uint64_t* count) {
+    const int cap = cell_height_cap(W, r, p);
+    if (!cap) return 0;
+    const uint64_t nc = capped_blocked_count(W, cap);
+    if (nc * 100 > nblk * 85) return 0;
+    if (count) *count = nc;
+    return cap;
+}
+
+// Enqueue cells p_hi..p_lo of s on s.stream; row >= 0 names the row of a sweep
+// from the initial state (height caps apply), -1 an arbitrary state.
+ofp_status enqueue_cells(ofp_ctx* c, Sweep& s, int p_hi, int p_lo, bool first_reads_blk, int* launches,
+                         int row = -1) {
     const bool wide = s.W > 23;
     const int blocks = grid_blocks(c, s.nblk);
+    const bool caps = c->height_caps && row >= 0 && s.layout == 1;
     for (int p = p_hi; p >= p_lo; --p) {
+        uint64_t ncap = 0;
+        const int cap = caps ? effective_cap(s.W, row, p, s.nblk, &ncap) : 0;
         // moduli per launch: all (up to 16) at once, or fewer on the low cells,
         // whose scattered closure reads are latency-bound (fewer registers per
         // thread -> more waves in flight).  Chunks keep an even offset.
@@ -222,7 +280,16 @@ ofp_status enqueue_cells(ofp_ctx* c, Sweep& s, int p_hi, int p_lo, bool first_re
             a.off = off;
             a.kc = std::min(step, s.K - off);
             for (int j = 0; j < a.kc; ++j) a.mod[j] = s.mods[off + j];
-            hipError_t e = launch_cell(a, wide, c->threads, blocks, s.stream);
+            hipError_t e;
+            if (cap) {
+                if (ncap == 0) continue;
+                a.nblk = ncap;
+                const size_t one = (wide ? 2 * sizeof(uint64_t) : 2 * sizeof(uint32_t)) * kDpRows * kDpCols;
+                a.dpc = (const char*)(wide ? c->dpc64 : c->dpc32) + one * (cap - 1);
+                e = launch_cell_capped(a, wide, c->threads, grid_blocks(c, ncap), s.stream);
+            } else {
+                e = launch_cell(a, wide, c->threads, blocks, s.stream);
+            }
             if (e != hipSuccess) return hip_fail(c, e, "cell kernel launch");
             if (launches) ++*launches;
         }
@@ -230,9 +297,9 @@ ofp_status enqueue_cells(ofp_ctx* c, Sweep& s, int p_hi, int p_lo, bool first_re
     return OFP_OK;
 }
 
-// `rows` full rows (cells W-1..1; the blocked input of cell W-1 is zero), via a
-// hipGraph captured once per (state, launch shape) and replayed.
-ofp_status run_rows(ofp_ctx* c, Sweep& s, int rows, int* launches) {
+// `rows` uncapped rows (cells W-1..1; the blocked input of cell W-1 is zero),
+// via a hipGraph captured once per (state, launch shape) and replayed.
+ofp_status full_rows(ofp_ctx* c, Sweep& s, int rows, int* launches) {
     if (!c->use_graphs) {
         for (int r = 0; r < rows; ++r) {
             ofp_status st = enqueue_cells(c, s, s.W - 1, 1, false, launches);
@@ -274,11 +341,88 @@ ofp_status run_rows(ofp_ctx* c, Sweep& s, int rows, int* launches) {
     return OFP_OK;
 }
 
-// Zero main and place the start word R at position W-1 (rank H_DP[W-1][1]) with
-// residue 1 (fullmate_dropN_batch.cu:508-511).  The blocked array needs no
-// initialisation: cell W-1 does not read it and writes every entry.
+// True when some cell of row r (of a sweep from the initial state) runs capped.
+bool row_capped(const ofp_ctx* c, const Sweep& s, int r) {
+    if (!c->height_caps || r < 0 || s.layout != 1) return false;
+    for (int p = s.W - 1; p >= 1; --p)
+        if (effective_cap(s.W, r, p, s.nblk, nullptr)) return true;
+    return false;
+}
+
+// Kernel time split between rows that run only uncapped cells (the full-state
+// kernel, replayed from the row graph) and rows with capped cells.
+struct RowTimes {
+    double full_s = 0, capped_s = 0;
+    int full_launches = 0, capped_launches = 0;
+};
+
+// `rows` rows from the sweep's current row: rows with capped cells are launched
+// directly, runs of uncapped rows from the row graph.  With `times`, each run
+// is bracketed by events and the call waits for the last one.
+ofp_status run_rows(ofp_ctx* c, Sweep& s, int rows, int* launches, RowTimes* times = nullptr) {
+    struct Seg {
+        bool capped;
+        int launches;
+    };
+    std::vector<Seg> segs;
+    auto event = [&](size_t i) -> hipEvent_t {
+        while (s.seg_ev.size() <= i) {
+            hipEvent_t e = nullptr;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            s.seg_ev.push_back(e);
+        }
+        return s.seg_ev[i];
+    };
+    int done = 0;
+    while (done < rows) {
+        const int r = s.row >= 0 ? s.row + done : -1;
+        const bool capped = row_capped(c, s, r);
+        int n = 1;
+        if (!capped)
+            while (done + n < rows && !row_capped(c, s, r < 0 ? -1 : r + n)) ++n;
+        if (times) {
+            hipEvent_t e = event(2 * segs.size());
+            if (!e) return fail(c, OFP_E_RUNTIME, "event create");
+            OFP_HIP(c, hipEventRecord(e, s.stream), "event record");
+        }
+        int l = 0;
+        ofp_status st = capped ? enqueue_cells(c, s, s.W - 1, 1, false, &l, r) : full_rows(c, s, n, &l);
+        if (st) return st;
+        if (times) {
+            hipEvent_t e = event(2 * segs.size() + 1);
+            if (!e) return fail(c, OFP_E_RUNTIME, "event create");
+            OFP_HIP(c, hipEventRecord(e, s.stream), "event record");
+        }
+        segs.push_back({capped, l});
+        if (launches) *launches += l;
+        done += n;
+    }
+    if (s.row >= 0) s.row += rows;
+    if (times && !segs.empty()) {
+        OFP_HIP(c, hipEventSynchronize(s.seg_ev[2 * segs.size() - 1]), "row sweep sync");
+        for (size_t i = 0; i < segs.size(); ++i) {
+            float ms = 0;
+            OFP_HIP(c, hipEventElapsedTime(&ms, s.seg_ev[2 * i], s.seg_ev[2 * i + 1]), "event time");
+            if (segs[i].capped) {
+                times->capped_s += ms * 1e-3;
+                times->capped_launches += segs[i].launches;
+            } else {
+                times->full_s += ms * 1e-3;
+                times->full_launches += segs[i].launches;
+            }
+        }
+    }
+    return OFP_OK;
+}
+
+// Zero main and blocked and place the start word R at position W-1 (rank
+// H_DP[W-1][1]) with residue 1 (fullmate_dropN_batch.cu:508-511).
 ofp_status init_state(ofp_ctx* c, Sweep& s) {
     OFP_HIP(c, hipMemsetAsync(s.main, 0, padded_states(s.nmain) * (size_t)s.Kp * 4, s.stream), "zero main state");
+    // capped cells leave the blocked words above their cap untouched: those must read as zero
+    OFP_HIP(c, hipMemsetAsync(s.blk, 0, padded_states(s.nblk ? s.nblk : 1) * (size_t)s.Kp * 4, s.stream),
+            "zero blocked state");
+    s.row = 0;
     uint64_t start = dp_at(s.W - 1, 1);
     hipError_t e = launch_set_one(s.main, start, s.Kp, s.K, s.layout, s.mods_dev, s.stream);
     if (e != hipSuccess) return hip_fail(c, e, "init kernel");
@@ -382,6 +526,8 @@ ofp_status ofp_create(const ofp_config* cfg, ofp_ctx** out) {
         if (e == hipSuccess) return false;
         if (c->dp32) (void)hipFree(c->dp32);
         if (c->dp64) (void)hipFree(c->dp64);
+        if (c->dpc32) (void)hipFree(c->dpc32);
+        if (c->dpc64) (void)hipFree(c->dpc64);
         if (c->ev0) (void)hipEventDestroy(c->ev0);
         if (c->ev1) (void)hipEventDestroy(c->ev1);
         if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -411,6 +557,17 @@ ofp_status ofp_create(const ofp_config* cfg, ofp_ctx** out) {
     if (bad(hipMemcpy(c->dp32, q32.data(), b32, hipMemcpyHostToDevice)) ||
         bad(hipMemcpy(c->dp64, q64.data(), b64, hipMemcpyHostToDevice)))
         return OFP_E_RUNTIME;
+    std::vector<unsigned char> c32(512 * 1024), c64(512 * 1024);
+    size_t bc32 = pack_capped_tables(false, c32.data(), c32.size());
+    size_t bc64 = pack_capped_tables(true, c64.data(), c64.size());
+    if (!bc32 || !bc64) {
+        (void)bad(hipErrorInvalidValue);
+        return OFP_E_RUNTIME;
+    }
+    if (bad(hipMalloc(&c->dpc32, bc32)) || bad(hipMalloc(&c->dpc64, bc64))) return OFP_E_RUNTIME;
+    if (bad(hipMemcpy(c->dpc32, c32.data(), bc32, hipMemcpyHostToDevice)) ||
+        bad(hipMemcpy(c->dpc64, c64.data(), bc64, hipMemcpyHostToDevice)))
+        return OFP_E_RUNTIME;
     *out = c;
     return OFP_OK;
 }
@@ -433,6 +590,23 @@ ofp_status ofp_set_variant(ofp_ctx* c, int variant) {
     if (variant != c->variant) drop_graph(c->st);
     c->variant = variant;
     return OFP_OK;
+}
+
+ofp_status ofp_set_height_caps(ofp_ctx* c, int on) {
+    if (!c) return OFP_E_ARG;
+    c->height_caps = on ? 1 : 0;
+    return OFP_OK;
+}
+
+int ofp_height_cap(int n, int row, int p, uint64_t* capped_blocked) {
+    if (n < 1 || n > OFP_MAX_N) return -1;
+    const int W = n + 1;
+    uint64_t nm, nb;
+    if (!counts(n, &nm, &nb)) return -1;
+    uint64_t nc = nb;
+    const int cap = effective_cap(W, row, p, nb, &nc);
+    if (capped_blocked) *capped_blocked = cap ? nc : nb;
+    return cap;
 }
 
 ofp_status ofp_set_low_cells(ofp_ctx* c, int low_cells, int low_kc) {
@@ -492,7 +666,8 @@ ofp_status ofp_row_sweep(ofp_ctx* c, int rows, ofp_stats* st) {
     int launches = 0;
     auto t0 = std::chrono::steady_clock::now();
     OFP_HIP(c, hipEventRecord(c->ev0, s.stream), "event record");
-    ofp_status r = run_rows(c, s, rows, &launches);
+    RowTimes times;
+    ofp_status r = run_rows(c, s, rows, &launches, &times);
     if (r) return r;
     OFP_HIP(c, hipEventRecord(c->ev1, s.stream), "event record");
     OFP_HIP(c, hipEventSynchronize(c->ev1), "row sweep sync");
@@ -501,6 +676,10 @@ ofp_status ofp_row_sweep(ofp_ctx* c, int rows, ofp_stats* st) {
     if (st) {
         st->wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         st->kernel_s = ms * 1e-3;
+        st->full_kernel_s = times.full_s;
+        st->full_launches = times.full_launches;
+        st->capped_kernel_s = times.capped_s;
+        st->capped_launches = times.capped_launches;
         st->main_states = s.nmain;
         st->blocked_states = s.nblk;
         st->rows = rows;
@@ -556,6 +735,7 @@ ofp_status ofp_apply_cells(ofp_ctx* c, int n, const uint32_t* moduli, int k, uin
         for (int j = 0; j < k; ++j) tb[state_elem(lay, i, Kp, j)] = blocked_io[i * k + j];
     OFP_HIP(c, hipMemcpy(sw.main, tm.data(), tm.size() * 4, hipMemcpyHostToDevice), "upload main");
     OFP_HIP(c, hipMemcpy(sw.blk, tb.data(), tb.size() * 4, hipMemcpyHostToDevice), "upload blocked");
+    sw.row = -1;  // arbitrary state: no height caps
     s = enqueue_cells(c, sw, p_hi, p_lo, blocked_valid != 0, nullptr);
     if (s) return s;
     OFP_HIP(c, hipStreamSynchronize(sw.stream), "apply sync");
@@ -771,6 +951,8 @@ void ofp_destroy(ofp_ctx* c) {
     free_sweep(c->st);
     if (c->dp32) (void)hipFree(c->dp32);
     if (c->dp64) (void)hipFree(c->dp64);
+    if (c->dpc32) (void)hipFree(c->dpc32);
+    if (c->dpc64) (void)hipFree(c->dpc64);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -91,6 +91,33 @@ __global__ __launch_bounds__(256) void cell_shard_kernel(CellLaunch a) {
     }
 }
 
+// Capped variant: the threads enumerate, in rank order, only the blocked words
+// whose heights stay <= the launch's cap (their count is a.nblk), decoded with
+// the capped completion table; every other blocked word is left untouched.
+// The uniform orbit body (V = 2) runs on each decoded word.
+template <int KC, typename Idx>
+__global__ __launch_bounds__(256) void cell_capped_kernel(CellLaunch a) {
+    __shared__ Quad<Idx> tab[kDpRows][kDpCols];
+    __shared__ Pair<Idx> ctab[kDpRows][kDpCols];
+    const Quad<Idx>* g = (const Quad<Idx>*)a.dp;
+    const Pair<Idx>* gc = (const Pair<Idx>*)a.dpc;
+    for (int t = threadIdx.x; t < kDpRows * kDpCols; t += blockDim.x) {
+        (&tab[0][0])[t] = g[t];
+        (&ctab[0][0])[t] = gc[t];
+    }
+    __syncthreads();
+    Tab<Idx> T{tab};
+    const Direct<KC, kAoS> mem{a.main, a.blk, a.K, a.off};
+    const uint64_t chunks = (a.nblk + blockDim.x - 1) / blockDim.x;
+    const uint32_t nwg = gridDim.x;
+    const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3, nx = nwg >> 3;
+    const uint64_t c0 = (chunks * xcd) >> 3, c1 = (chunks * (xcd + 1)) >> 3;
+    for (uint64_t c = c0 + slot; c < c1; c += nx) {
+        const uint64_t t = c * blockDim.x + threadIdx.x;
+        if (t < a.nblk) cell_body<KC, Idx, kAoS, 2>(a, T, decode_capped<Idx>(T, ctab, a.W, a.p, t), mem);
+    }
+}
+
 __global__ void fill_remote_kernel(uint32_t* main, uint64_t m0, uint64_t M, uint32_t* blk, uint64_t b0,
                                    uint64_t B, int K, int k, const uint32_t* mods, uint64_t seed) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -149,6 +176,18 @@ hipError_t launch_shard_idx(const CellLaunch& a, int threads, int blocks, hipStr
 }
 
 template <typename Idx>
+hipError_t launch_capped_idx(const CellLaunch& a, int threads, int blocks, hipStream_t s) {
+    switch (a.kc) {
+#define OFP_KC(n) case n: hipLaunchKernelGGL((cell_capped_kernel<n, Idx>), dim3(blocks), dim3(threads), 0, s, a); break;
+        OFP_KC(1) OFP_KC(2) OFP_KC(3) OFP_KC(4) OFP_KC(5) OFP_KC(6) OFP_KC(7) OFP_KC(8)
+        OFP_KC(9) OFP_KC(10) OFP_KC(11) OFP_KC(12) OFP_KC(13) OFP_KC(14) OFP_KC(15) OFP_KC(16)
+#undef OFP_KC
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <typename Idx>
 hipError_t launch_idx(const CellLaunch& a, int threads, int blocks, hipStream_t s) {
     switch (a.kc) {
 #define OFP_KC(n) case n: return launch_kc<Idx, n>(a, threads, blocks, s);
@@ -171,6 +210,18 @@ hipError_t launch_cell(const CellLaunch& a, bool wide_index, int threads, int bl
         return hipErrorInvalidValue;
     return wide_index ? launch_idx<uint64_t>(a, threads, blocks, s)
                       : launch_idx<uint32_t>(a, threads, blocks, s);
+}
+
+hipError_t launch_cell_capped(const CellLaunch& a, bool wide_index, int threads, int blocks,
+                              hipStream_t s) {
+    if (threads <= 0 || threads > 256 || (threads & 63) || blocks <= 0 || (blocks & 7))
+        return hipErrorInvalidConfiguration;
+    if (a.W < 2 || a.W > kMaxW || a.p < 1 || a.p > a.W - 1 || a.kc < 1 || a.kc > kMaxKC || a.off < 0 ||
+        a.off + a.kc > a.K || a.layout != kAoS || (a.K > 1 && ((a.K & 1) || (a.off & 1))) || !a.dpc ||
+        a.first != 0)
+        return hipErrorInvalidValue;
+    return wide_index ? launch_capped_idx<uint64_t>(a, threads, blocks, s)
+                      : launch_capped_idx<uint32_t>(a, threads, blocks, s);
 }
 
 hipError_t launch_cell_shard(const CellLaunch& a, bool wide_index, int threads, int blocks,
@@ -212,6 +263,32 @@ size_t pack_height_table(bool wide, void* out, size_t cap) {
     if (cap < sizeof(Quad<uint32_t>) * kDpRows * kDpCols) return 0;
     pack_quads<uint32_t>(d, (Quad<uint32_t>(*)[kDpCols])out);
     return sizeof(Quad<uint32_t>) * kDpRows * kDpCols;
+}
+
+void build_capped_table(int cap, uint64_t (*d)[kDpCols]) {
+    for (int w = 0; w < kDpRows; ++w)
+        for (int h = 0; h < kDpCols; ++h) d[w][h] = 0;
+    d[0][0] = 1;
+    for (int w = 1; w < kDpRows; ++w)
+        for (int h = 0; h <= cap && h < kDpCols; ++h) {
+            uint64_t x = d[w - 1][h];
+            if (h > 0) x += d[w - 1][h - 1];
+            if (h + 1 <= cap && h + 1 < kDpCols) x += d[w - 1][h + 1];
+            d[w][h] = x;
+        }
+}
+
+size_t pack_capped_tables(bool wide, void* out, size_t cap) {
+    const size_t one = (wide ? sizeof(Pair<uint64_t>) : sizeof(Pair<uint32_t>)) * kDpRows * kDpCols;
+    if (cap < one * kMaxCap) return 0;
+    uint64_t d[kDpRows][kDpCols];
+    for (int c = 1; c <= kMaxCap; ++c) {
+        build_capped_table(c, d);
+        char* dst = (char*)out + one * (c - 1);
+        if (wide) pack_pairs<uint64_t>(d, (Pair<uint64_t>(*)[kDpCols])dst);
+        else pack_pairs<uint32_t>(d, (Pair<uint32_t>(*)[kDpCols])dst);
+    }
+    return one * kMaxCap;
 }
 
 void build_height_table(uint64_t (*d)[kDpCols]) {

@@ -254,17 +254,26 @@ OFP_HD void gather_closures(const Tab<Idx>& T, int W, int p, int hP, Idx mainbas
 // the scans), 1 = after them (fewer live registers, so more waves per SIMD),
 // 2 = one branch-free orbit path for all three kinds of blocked word, loads first,
 // 3 = as 2 but the orbit results are stored before the closure scans.
-template <int KC, typename Idx, int L, int V, class Mem>
-OFP_HD void cell_one(const CellLaunch& a, const Tab<Idx>& T, uint64_t bidx, const Mem& m) {
-    const int W = a.W, p = a.p;
-    uint32_t mod[KC];
-#pragma unroll
-    for (int j = 0; j < KC; ++j) mod[j] = a.mod[j];
+// A blocked word b = [P, x, S] of cell p, decoded: its index, the height hP
+// entering x, x itself, the main-numbering rank contribution of P (mainbase),
+// the rank of S among suffixes entered at the height after x (sr) and the word
+// (2 bits per blocked position; the closure scans read P from it).
+template <typename Idx>
+struct Decoded {
+    uint64_t bidx;
+    Idx mainbase, sr;
+    uint64_t word;
+    int hP, x;
+};
 
-    // ---- decode blocked word b over positions W-2..p-1 ----
+// Decode blocked word bidx over positions W-2..p-1 (S is left as its rank).
+template <typename Idx>
+OFP_HD Decoded<Idx> decode_blocked(const Tab<Idx>& T, int W, int p, uint64_t bidx) {
+    Decoded<Idx> d;
+    d.bidx = bidx;
     Idx r = (Idx)bidx;
     int h = 1;
-    Idx mainbase = 0;  // sum of main-numbering contributions of P
+    Idx mainbase = 0;
     uint64_t word = 0;
     for (int pos = W - 2; pos >= p; --pos) {
         const Quad<Idx> e = T.q[pos][h];
@@ -274,10 +283,69 @@ OFP_HD void cell_one(const CellLaunch& a, const Tab<Idx>& T, uint64_t bidx, cons
         word |= (uint64_t)v << (2 * pos);
         h += (v == 2) - (v == 1);
     }
-    const int hP = h;
+    d.hP = h;
     Idx cx;
-    const int x = T.step(p - 1, hP, r, cx);
-    const Idx sr = r;  // rank of S among suffixes entered at the height after x
+    d.x = T.step(p - 1, h, r, cx);
+    d.sr = r;
+    d.mainbase = mainbase;
+    d.word = word;
+    return d;
+}
+
+// Capped completions for the words whose heights stay <= a cap c: at position
+// pos entered at height h, n = Dc[pos][h] (choose N), r = Dc[pos][h-1] (choose R).
+template <typename Idx>
+struct Pair {
+    Idx n, r;
+};
+
+// Decode the t-th (in rank order) blocked word whose heights stay <= the cap
+// of C.  Symbols are chosen with the capped counts; every rank (the blocked
+// index, P's main-numbering contribution, S's rank) is summed with the full
+// table T, so the result is the same as decode_blocked(T, W, p, bidx) for the
+// word's own index bidx.
+template <typename Idx>
+OFP_HD Decoded<Idx> decode_capped(const Tab<Idx>& T, const Pair<Idx> (*C)[kDpCols], int W, int p, uint64_t t) {
+    Decoded<Idx> d;
+    Idx r = (Idx)t, bidx = 0, mainbase = 0, sr = 0;
+    uint64_t word = 0;
+    int h = 1, x = 0, hP = 1;
+    for (int pos = W - 2; pos >= 0; --pos) {
+        const Pair<Idx> pc = C[pos][h];
+        const Quad<Idx> e = T.q[pos][h];
+        int v = 0;
+        if (r >= pc.n) {
+            r -= pc.n;
+            if (r < pc.r) v = 1;
+            else { r -= pc.r; v = 2; }
+        }
+        const Idx cb = (v == 0) ? Idx(0) : (v == 1 ? e.zn : Idx(e.zn + e.zr));
+        bidx += cb;
+        if (pos >= p) mainbase += (v == 0) ? Idx(0) : (v == 1 ? e.mn : Idx(e.mn + e.mr));
+        else if (pos == p - 1) { x = v; hP = h; }
+        else sr += cb;
+        word |= (uint64_t)v << (2 * pos);
+        h += (v == 2) - (v == 1);
+    }
+    d.bidx = (uint64_t)bidx;
+    d.mainbase = mainbase;
+    d.sr = sr;
+    d.word = word;
+    d.hP = hP;
+    d.x = x;
+    return d;
+}
+
+template <int KC, typename Idx, int L, int V, class Mem>
+OFP_HD void cell_body(const CellLaunch& a, const Tab<Idx>& T, const Decoded<Idx>& d, const Mem& m) {
+    const int W = a.W, p = a.p;
+    uint32_t mod[KC];
+#pragma unroll
+    for (int j = 0; j < KC; ++j) mod[j] = a.mod[j];
+    const uint64_t bidx = d.bidx;
+    const int hP = d.hP, x = d.x;
+    const Idx mainbase = d.mainbase, sr = d.sr;
+    const uint64_t word = d.word;
 
     if constexpr (V >= 2) {
         // One code path for the three orbits: every lane issues its orbit loads
@@ -428,6 +496,21 @@ OFP_HD void cell_one(const CellLaunch& a, const Tab<Idx>& T, uint64_t bidx, cons
         m.sm(iNL, o);
         if (p >= 2) m.sb(bidx, nl);
     }
+}
+
+template <int KC, typename Idx, int L, int V, class Mem>
+OFP_HD void cell_one(const CellLaunch& a, const Tab<Idx>& T, uint64_t bidx, const Mem& m) {
+    cell_body<KC, Idx, L, V>(a, T, decode_blocked<Idx>(T, a.W, a.p, bidx), m);
+}
+
+// Packed capped table from a capped height table dc (kDpRows x kDpCols).
+template <typename Idx>
+inline void pack_pairs(const uint64_t (*dc)[kDpCols], Pair<Idx> (*q)[kDpCols]) {
+    for (int pos = 0; pos < kDpRows; ++pos)
+        for (int h = 0; h < kDpCols; ++h) {
+            q[pos][h].n = (Idx)dc[pos][h];
+            q[pos][h].r = h > 0 ? (Idx)dc[pos][h - 1] : Idx(0);
+        }
 }
 
 // Packed table from the plain height table d (kDpRows x kDpCols).

@@ -11,6 +11,7 @@ constexpr int kDpRows = kMaxW + 2; // positions 0..kMaxW+1
 constexpr int kDpCols = 32;        // heights 0..31 (h+2 <= W+2 <= 30)
 constexpr int kTile = 64;          // AoSoA tile: 64 states x K moduli
 constexpr int kMaxKC = 16;         // moduli handled by one launch
+constexpr int kMaxCap = 15;        // height caps 1..kMaxCap have packed tables (max height at W=28 is 15)
 
 // Packed height table (cell::Quad entries) for device upload; returns bytes.
 size_t pack_height_table(bool wide, void* out, size_t cap);
@@ -19,6 +20,13 @@ size_t pack_height_table(bool wide, void* out, size_t cap);
 // height h that never go negative and end at height 0.  Same recurrence as the
 // reference's H_DP (src/cuda/b300/oneesan_cuda_gridfp_b300_hbm32_fullmate_dropN_batch.cu:62).
 void build_height_table(uint64_t (*d)[kDpCols]);
+// Height table of words whose every height (entry height included) is <= cap:
+// d[w][h] = number of words of width w entered at height h <= cap that end at 0
+// without leaving [0, cap].
+void build_capped_table(int cap, uint64_t (*d)[kDpCols]);
+// Packed capped tables for caps 1..kMaxCap (cell::Pair<uint32 or uint64>,
+// kMaxCap x kDpRows x kDpCols) into out; returns the bytes written, 0 if cap is too small.
+size_t pack_capped_tables(bool wide, void* out, size_t cap);
 
 // Element (state i, modulus j) of an interleaved state array with K moduli:
 // tiles of 64 consecutive states; inside a tile, modulus-major.  A wave of 64
@@ -46,7 +54,8 @@ struct CellLaunch {
     int read_blk;         // blocked input of this cell is valid
     int K, off, kc;       // interleave stride, first modulus, moduli in this launch
     int layout;           // 0: 64-state tiles (AoSoA), 1: state-major (AoS)
-    int variant;          // state-major only: 0 = orbit loads before closure scans, 1 = after
+    int variant;          // state-major only: cell_kernel V parameter (0..3)
+    const void* dpc;      // capped launches only: packed capped table of this launch's cap
     uint32_t mod[kMaxKC];
     // shard launches only (launch_cell_shard): resident main window and remote seed
     uint64_t m0, M;
@@ -57,6 +66,10 @@ struct CellLaunch {
 // Launch one cell step (all blocked words, moduli off..off+kc-1).
 hipError_t launch_cell(const CellLaunch& a, bool wide_index, int threads, int blocks,
                        hipStream_t s);
+// Launch one cell over only the blocked words whose heights stay <= the cap of
+// a.dpc (nblk = their number); the other blocked words are skipped.
+hipError_t launch_cell_capped(const CellLaunch& a, bool wide_index, int threads, int blocks,
+                              hipStream_t s);
 // Launch one cell over the blocked words [first, first+nblk) of a shard whose
 // main window is [m0, m0+M) (state-major layout; main/blk point at the window
 // arrays).  Main words outside the window read as remote_residue(seed, ...).

@@ -28,7 +28,7 @@ EXPORTS = (
     "ofp_row_sweep", "ofp_read_main", "ofp_set_launch", "ofp_set_variant", "ofp_set_low_cells", "ofp_set_layout", "ofp_state_counts",
     "ofp_last_error", "ofp_version", "ofp_destroy",
     "ofp_job_create", "ofp_job_enqueue", "ofp_job_info_get", "ofp_job_result", "ofp_job_destroy",
-    "ofp_shard_plan", "ofp_shard_sweep",
+    "ofp_shard_plan", "ofp_shard_sweep", "ofp_set_height_caps", "ofp_height_cap",
 )
 
 
@@ -70,6 +70,10 @@ class Stats(ctypes.Structure):
         ("rows", ctypes.c_int),
         ("cells_per_row", ctypes.c_int),
         ("launches", ctypes.c_int),
+        ("full_kernel_s", ctypes.c_double),
+        ("capped_kernel_s", ctypes.c_double),
+        ("full_launches", ctypes.c_int),
+        ("capped_launches", ctypes.c_int),
     ]
 
     def as_dict(self) -> dict:
@@ -146,6 +150,8 @@ def load() -> ctypes.CDLL:
         "ofp_set_layout": (ctypes.c_int32, [vp, ctypes.c_int]),
         "ofp_set_low_cells": (ctypes.c_int32, [vp, ctypes.c_int, ctypes.c_int]),
         "ofp_set_variant": (ctypes.c_int32, [vp, ctypes.c_int]),
+        "ofp_set_height_caps": (ctypes.c_int32, [vp, ctypes.c_int]),
+        "ofp_height_cap": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, u64p]),
         "ofp_state_counts": (ctypes.c_int32, [ctypes.c_int, u64p, u64p]),
         "ofp_last_error": (ctypes.c_char_p, [vp]),
         "ofp_version": (ctypes.c_char_p, []),
@@ -180,6 +186,17 @@ def state_counts(n: int) -> tuple[int, int]:
     if st != OFP_OK:
         raise OfpError(st, f"state_counts(n={n})")
     return m.value, b.value
+
+
+def height_cap(n: int, row: int, p: int) -> tuple[int, int]:
+    """(cap, blocked words swept) of cell p in row `row` of an n-solve; cap 0 =
+    the whole state.  Host computation, no GPU needed."""
+    lib = load()
+    cnt = ctypes.c_uint64()
+    cap = lib.ofp_height_cap(n, row, p, ctypes.byref(cnt))
+    if cap < 0:
+        raise OfpError(-1, f"height_cap(n={n})")
+    return cap, cnt.value
 
 
 @dataclass
@@ -309,8 +326,14 @@ class Context:
         return info.as_dict(), mo, bo
 
     def set_variant(self, variant: int) -> None:
-        """Cell-kernel variant (state-major layout): 0 = early orbit loads, 1 = late."""
+        """Cell-kernel variant (state-major layout): 0 = early orbit loads, 1 = late,
+        2 = one branch-free orbit path, 3 = as 2 with early stores."""
         self._check(self._lib.ofp_set_variant(self._h, variant), "ofp_set_variant")
+
+    def set_height_caps(self, on: bool) -> None:
+        """Skip, in the first and last rows of a solve, the blocked words whose
+        height exceeds what the row can hold or still use (default on)."""
+        self._check(self._lib.ofp_set_height_caps(self._h, 1 if on else 0), "ofp_set_height_caps")
 
     def set_low_cells(self, low_cells: int, low_kc: int) -> None:
         """Cells p <= low_cells advance at most low_kc moduli per launch."""

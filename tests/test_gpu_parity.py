@@ -135,3 +135,27 @@ def test_ggcount_residues(gpu_ctx):
         print(f"case ggcount n={n}", flush=True)
         got, _ = gpu_ctx.solve(n, [m for m, _ in pairs])
         assert got == [r for _, r in pairs], n
+
+
+def test_height_caps_match_whole_state_sweep(gpu_ctx):
+    """Capped first and last rows give the residues of the uncapped sweep, also when
+    the rows are swept in several calls (the row count carries across calls)."""
+    mods = [4294967291, 4294966997, 2147483647]
+    try:
+        for n in (3, 5, 8, 11, 14, 17):
+            W = n + 1
+            print(f"case caps n={n}", flush=True)
+            gpu_ctx.set_height_caps(False)
+            whole, _ = gpu_ctx.solve(n, mods)
+            gpu_ctx.set_height_caps(True)
+            capped, st = gpu_ctx.solve(n, mods)
+            assert capped == whole, n
+            if n >= 8:
+                assert st["capped_launches"] > 0 and st["full_launches"] > 0
+            gpu_ctx.prepare(n, mods)
+            gpu_ctx.row_sweep(2)
+            gpu_ctx.row_sweep(W - 3)
+            gpu_ctx.row_sweep(1)
+            assert gpu_ctx.read_main(0, len(mods)) == whole, n
+    finally:
+        gpu_ctx.set_height_caps(True)
