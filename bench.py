@@ -204,6 +204,12 @@ def main() -> int:
         launch_basis = f"all {st['launches']} launches (no row of the timed region ran uncapped)"
     bytes_per_launch = 8.0 * S * min(k, 16)  # one launch carries up to 16 moduli
     achieved = bytes_per_launch / avg_launch_s / 1e9
+    # HBM bytes per full-state launch as measured with rocprofv3 counters for this
+    # exact launch configuration (recorded with the tuned entry), else unknown
+    traffic, traffic_source = None, None
+    if tuned and tuned.get("hbm_bytes_per_full_launch"):
+        traffic = float(tuned["hbm_bytes_per_full_launch"])
+        traffic_source = tuned.get("hbm_bytes_source")
 
     # plain one-residue-at-a-time form (k=1): one whole solve
     plain = None
@@ -262,7 +268,8 @@ def main() -> int:
         "plain_k1": plain,
         "batched_vs_plain_residues_per_hour": (residues_per_hour / plain["residues_per_hour"]) if plain else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_source,
                      "kernel": "cell_kernel", "bytes_per_launch": bytes_per_launch,
                      "avg_launch_ms": avg_launch_s * 1e3, "launch_basis": launch_basis,
                      "capped_kernel_s": st["capped_kernel_s"], "capped_launches": st["capped_launches"]},

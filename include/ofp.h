@@ -118,25 +118,27 @@ ofp_status ofp_set_launch(ofp_ctx* ctx, int threads, int blocks_per_cu);
 /* Cell-kernel variant for the state-major layout: 0 = an x=N thread loads its
  * own orbit values before scanning for closure predecessors, 1 = after (fewer
  * registers live during the scans), 2 = one branch-free orbit path for every
- * kind of blocked word (all lanes' loads issued before the scans), 3 = as 2
- * with the orbit results stored before the scans (fewer live registers).  Set
- * by the autotuner; no reference counterpart. */
+ * kind of blocked word (all lanes' loads issued before the scans; the
+ * default), 3 = as 2 with the orbit results stored before the scans (fewer live
+ * registers).  Set by the autotuner; no reference counterpart. */
 ofp_status ofp_set_variant(ofp_ctx* ctx, int variant);
 
-/* Height caps (default on).  In a sweep from the initial state (ofp_solve,
- * ofp_prepare + ofp_row_sweep, jobs) the cells of the first rows only touch
- * words of small height (every other word is still zero), and in the last two
- * rows only such words can still reach the answer word; capped cells enumerate
- * just those blocked words.  The answer is the same; other main words read
- * after the last two rows are not final.  Reference analogue: the row-6 prefix
- * initialiser and the mirrored last row (row6crt20.cu:1434-1688,1787-1814,
- * 935-952), which likewise skip the first and last rows' full sweeps. */
-ofp_status ofp_set_height_caps(ofp_ctx* ctx, int on);
+/* Height caps.  In a sweep from the initial state (ofp_solve, ofp_prepare +
+ * ofp_row_sweep, jobs) the cells of the first rows only touch words of small
+ * height (every other word is still zero), and in the last two rows only such
+ * words can still reach the answer word; a capped cell enumerates just those
+ * blocked words.  A cell is capped when it then sweeps at most max_percent % of
+ * the blocked words (default 85; 0 turns caps off).  The answer is the same;
+ * other main words read after the last two rows are not final.  Reference
+ * analogue: the row-6 prefix initialiser and the mirrored last row
+ * (row6crt20.cu:1434-1688,1787-1814,935-952), which likewise skip the first
+ * and last rows' full sweeps. */
+ofp_status ofp_set_height_caps(ofp_ctx* ctx, int max_percent);
 
-/* The height cap cell p of row `row` of an n-solve runs with (0: the whole
- * state), and the number of blocked words it sweeps.  Host-only, no GPU.
- * Returns -1 for n out of range. */
-int ofp_height_cap(int n, int row, int p, uint64_t* capped_blocked);
+/* The height cap cell p of row `row` of an n-solve runs with at max_percent
+ * (0: the whole state), and the number of blocked words it sweeps.
+ * Host-only, no GPU.  Returns -1 for n out of range. */
+int ofp_height_cap(int n, int row, int p, int max_percent, uint64_t* capped_blocked);
 
 /* Moduli per launch on the low cells: cells p <= low_cells advance at most
  * low_kc moduli per launch (more launches, fewer registers per thread); the

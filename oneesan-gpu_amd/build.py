@@ -3,6 +3,8 @@
 Produces, in-tree:
   oneesan-gpu_amd/lib/libofp.so    the C-ABI library (HIP kernels + host runtime)
   oneesan-gpu_amd/lib/ofp_solver   the CLI (reference argv / result-line contract)
+  oneesan-gpu_amd/lib/ofp_hbm_probe  streaming read / copy / read-modify-write kernels
+                                     (the HBM ceiling the cell kernels are compared with)
 
 hipcc cross-compiles for gfx950 without a GPU.  Every HIP translation unit is
 built with packed-FP32 code generation disabled.
@@ -78,6 +80,9 @@ def build(force: bool = True, jobs: int = 4) -> Path:
         cli = LIB / "ofp_solver"
         _run([HIPCC, *HIP_FLAGS, "-x", "c++", str(cli_src), f"-L{LIB}", "-lofp",
               f"-Wl,-rpath,$ORIGIN", "-o", str(cli)])
+    probe_src = CSRC / "hbm_probe.hip"
+    if probe_src.exists():
+        _run([HIPCC, *HIP_FLAGS, str(probe_src), "-o", str(LIB / "ofp_hbm_probe")])
     return so
 
 

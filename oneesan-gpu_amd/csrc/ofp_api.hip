@@ -48,7 +48,7 @@ struct ofp_ctx {
     int blocks_per_cu = 32;
     int use_graphs = 1;
     int layout = 1;  // 0: 64-state tiles, 1: state-major with even stride
-    int variant = 0;         // kernel variant (cell_kernel V parameter), set by the autotuner
+    int variant = 2;         // kernel variant (cell_kernel V parameter), set by the autotuner
     int low_cells = 0;       // cells p <= low_cells launch at most low_kc moduli at a time
     int low_kc = kMaxKC;
     size_t reserve = 0;
@@ -58,7 +58,7 @@ struct ofp_ctx {
     void* dp64 = nullptr;  // packed uint64 height table
     void* dpc32 = nullptr; // packed capped tables, caps 1..kMaxCap (uint32, uint64)
     void* dpc64 = nullptr;
-    int height_caps = 1;   // skip the blocked words a cell provably leaves unchanged or unread
+    int cap_percent = 85;  // height caps: cap a cell when it then sweeps <= this % of the blocked words
     std::string err;
     Sweep st;              // the context's own state (solve / prepare / row_sweep)
     int jobs = 0;          // live jobs (serving)
@@ -238,12 +238,12 @@ uint64_t capped_blocked_count(int W, int cap) {
 }
 
 // The cap a cell runs with: the bound above, when the capped kernel has few
-// enough words to pay for its longer decode (<= 85% of the blocked words).
-int effective_cap(int W, int r, int p, uint64_t nblk, uint64_t* count) {
+// enough words to pay for its longer decode (<= percent% of the blocked words).
+int effective_cap(int W, int r, int p, uint64_t nblk, uint64_t* count, int percent = 85) {
     const int cap = cell_height_cap(W, r, p);
-    if (!cap) return 0;
+    if (!cap || percent <= 0) return 0;
     const uint64_t nc = capped_blocked_count(W, cap);
-    if (nc * 100 > nblk * 85) return 0;
+    if (nc * 100 > nblk * (uint64_t)percent) return 0;
     if (count) *count = nc;
     return cap;
 }
@@ -254,10 +254,10 @@ ofp_status enqueue_cells(ofp_ctx* c, Sweep& s, int p_hi, int p_lo, bool first_re
                          int row = -1) {
     const bool wide = s.W > 23;
     const int blocks = grid_blocks(c, s.nblk);
-    const bool caps = c->height_caps && row >= 0 && s.layout == 1;
+    const bool caps = c->cap_percent > 0 && row >= 0 && s.layout == 1;
     for (int p = p_hi; p >= p_lo; --p) {
         uint64_t ncap = 0;
-        const int cap = caps ? effective_cap(s.W, row, p, s.nblk, &ncap) : 0;
+        const int cap = caps ? effective_cap(s.W, row, p, s.nblk, &ncap, c->cap_percent) : 0;
         // moduli per launch: all (up to 16) at once, or fewer on the low cells,
         // whose scattered closure reads are latency-bound (fewer registers per
         // thread -> more waves in flight).  Chunks keep an even offset.
@@ -343,9 +343,9 @@ ofp_status full_rows(ofp_ctx* c, Sweep& s, int rows, int* launches) {
 
 // True when some cell of row r (of a sweep from the initial state) runs capped.
 bool row_capped(const ofp_ctx* c, const Sweep& s, int r) {
-    if (!c->height_caps || r < 0 || s.layout != 1) return false;
+    if (c->cap_percent <= 0 || r < 0 || s.layout != 1) return false;
     for (int p = s.W - 1; p >= 1; --p)
-        if (effective_cap(s.W, r, p, s.nblk, nullptr)) return true;
+        if (effective_cap(s.W, r, p, s.nblk, nullptr, c->cap_percent)) return true;
     return false;
 }
 
@@ -592,19 +592,20 @@ ofp_status ofp_set_variant(ofp_ctx* c, int variant) {
     return OFP_OK;
 }
 
-ofp_status ofp_set_height_caps(ofp_ctx* c, int on) {
+ofp_status ofp_set_height_caps(ofp_ctx* c, int max_percent) {
     if (!c) return OFP_E_ARG;
-    c->height_caps = on ? 1 : 0;
+    if (max_percent < 0 || max_percent > 100) return fail(c, OFP_E_ARG, "max_percent must be 0..100");
+    c->cap_percent = max_percent;
     return OFP_OK;
 }
 
-int ofp_height_cap(int n, int row, int p, uint64_t* capped_blocked) {
+int ofp_height_cap(int n, int row, int p, int max_percent, uint64_t* capped_blocked) {
     if (n < 1 || n > OFP_MAX_N) return -1;
     const int W = n + 1;
     uint64_t nm, nb;
     if (!counts(n, &nm, &nb)) return -1;
     uint64_t nc = nb;
-    const int cap = effective_cap(W, row, p, nb, &nc);
+    const int cap = effective_cap(W, row, p, nb, &nc, max_percent);
     if (capped_blocked) *capped_blocked = cap ? nc : nb;
     return cap;
 }

@@ -151,7 +151,7 @@ def load() -> ctypes.CDLL:
         "ofp_set_low_cells": (ctypes.c_int32, [vp, ctypes.c_int, ctypes.c_int]),
         "ofp_set_variant": (ctypes.c_int32, [vp, ctypes.c_int]),
         "ofp_set_height_caps": (ctypes.c_int32, [vp, ctypes.c_int]),
-        "ofp_height_cap": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, u64p]),
+        "ofp_height_cap": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, u64p]),
         "ofp_state_counts": (ctypes.c_int32, [ctypes.c_int, u64p, u64p]),
         "ofp_last_error": (ctypes.c_char_p, [vp]),
         "ofp_version": (ctypes.c_char_p, []),
@@ -188,12 +188,15 @@ def state_counts(n: int) -> tuple[int, int]:
     return m.value, b.value
 
 
-def height_cap(n: int, row: int, p: int) -> tuple[int, int]:
+DEFAULT_CAP_PERCENT = 85
+
+
+def height_cap(n: int, row: int, p: int, max_percent: int = DEFAULT_CAP_PERCENT) -> tuple[int, int]:
     """(cap, blocked words swept) of cell p in row `row` of an n-solve; cap 0 =
     the whole state.  Host computation, no GPU needed."""
     lib = load()
     cnt = ctypes.c_uint64()
-    cap = lib.ofp_height_cap(n, row, p, ctypes.byref(cnt))
+    cap = lib.ofp_height_cap(n, row, p, max_percent, ctypes.byref(cnt))
     if cap < 0:
         raise OfpError(-1, f"height_cap(n={n})")
     return cap, cnt.value
@@ -330,10 +333,13 @@ class Context:
         2 = one branch-free orbit path, 3 = as 2 with early stores."""
         self._check(self._lib.ofp_set_variant(self._h, variant), "ofp_set_variant")
 
-    def set_height_caps(self, on: bool) -> None:
+    def set_height_caps(self, max_percent: int | bool) -> None:
         """Skip, in the first and last rows of a solve, the blocked words whose
-        height exceeds what the row can hold or still use (default on)."""
-        self._check(self._lib.ofp_set_height_caps(self._h, 1 if on else 0), "ofp_set_height_caps")
+        height exceeds what the row can hold or still use, in cells where that
+        leaves at most max_percent % of them (True = the default 85, False/0 = off)."""
+        if max_percent is True:
+            max_percent = DEFAULT_CAP_PERCENT
+        self._check(self._lib.ofp_set_height_caps(self._h, int(max_percent)), "ofp_set_height_caps")
 
     def set_low_cells(self, low_cells: int, low_kc: int) -> None:
         """Cells p <= low_cells advance at most low_kc moduli per launch."""

@@ -3,7 +3,9 @@
 For a given (n, k) it prepares the device state once, then times row sweeps
 (cells W-1..1) for each candidate (threads per workgroup, workgroups per CU)
 and keeps the fastest.  Timing runs advance the state, which does not change
-the cost of a row.  Results persist in a JSON cache keyed by device name, n
+the cost of a row as long as height caps are off (they are, for these stages).
+A last stage picks the height-cap threshold on the first rows of a solve, the
+ones caps shorten.  Results persist in a JSON cache keyed by device name, n
 and k, so a solver or the serving loop can look them up without re-tuning.
 
 Reference analogue: scripts/run/autotune.py:28 (the reference tunes its
@@ -30,6 +32,9 @@ LAYOUTS = (0, 1)
 # second stage: cells p <= low_cells advance at most low_kc moduli per launch
 LOW_CELLS = ((0, 16), (6, 8), (4, 8))
 VARIANTS = (0, 1, 2, 3)
+# third stage: cap a cell when it then sweeps at most this % of the blocked words
+CAP_PERCENTS = (75, 85, 100)
+CAPPED_ROWS = 5  # rows 1..5 of a solve hold every capped cell of the forward bound
 
 
 def _key(n: int, k: int) -> str:
@@ -70,8 +75,9 @@ def apply(ctx: "_lib.Context", n: int, k: int) -> dict | None:
     if ent:
         ctx.set_layout(int(ent.get("layout", 1)))
         ctx.set_launch(int(ent["threads"]), int(ent["blocks_per_cu"]))
-        ctx.set_variant(int(ent.get("variant", 0)))
+        ctx.set_variant(int(ent.get("variant", 2)))
         ctx.set_low_cells(int(ent.get("low_cells", 0)), int(ent.get("low_kc", 16)))
+        ctx.set_height_caps(int(ent.get("cap_percent", _lib.DEFAULT_CAP_PERCENT)))
     return ent
 
 
@@ -81,6 +87,9 @@ def tune(ctx: "_lib.Context", n: int, moduli: list[int], rows: int = 1,
     """Time every (layout, launch shape); set the best on ctx; return the table."""
     table = []
     best = None
+    ctx.set_height_caps(0)
+    ctx.set_variant(2)
+    ctx.set_low_cells(0, 16)
     for lay in layouts:
         ctx.set_layout(lay)
         ctx.prepare(n, moduli)
@@ -102,7 +111,7 @@ def tune(ctx: "_lib.Context", n: int, moduli: list[int], rows: int = 1,
                     best = ent
     ctx.set_layout(best["layout"])
     ctx.set_launch(best["threads"], best["blocks_per_cu"])
-    best = dict(best, low_cells=0, low_kc=16, variant=0)
+    best = dict(best, low_cells=0, low_kc=16, variant=2)
     if best["layout"] == 1:
         ctx.prepare(n, moduli)
         for var in VARIANTS:
@@ -123,6 +132,22 @@ def tune(ctx: "_lib.Context", n: int, moduli: list[int], rows: int = 1,
                     best = ent
     ctx.set_variant(best["variant"])
     ctx.set_low_cells(best["low_cells"], best["low_kc"])
+    best = dict(best, cap_percent=_lib.DEFAULT_CAP_PERCENT)
+    if best["layout"] == 1 and n + 1 > CAPPED_ROWS + 2:
+        timed = {}
+        for pct in CAP_PERCENTS:
+            ctx.set_height_caps(pct)
+            ctx.prepare(n, moduli)
+            ctx.row_sweep(1)
+            t0 = time.perf_counter()
+            ctx.row_sweep(CAPPED_ROWS)
+            timed[pct] = (time.perf_counter() - t0) / CAPPED_ROWS * 1e3
+            table.append(dict(best, cap_percent=pct, capped_rows_ms=timed[pct]))
+            if log:
+                log(f"autotune n={n} k={len(moduli)} cap_percent={pct}: rows 1..{CAPPED_ROWS} "
+                    f"{timed[pct]:.2f} ms/row")
+        best["cap_percent"] = min(timed, key=timed.get)
+    ctx.set_height_caps(best["cap_percent"])
     result = {"best": best, "table": table}
     if path is not None:
         data = load_cache(path)
@@ -130,6 +155,7 @@ def tune(ctx: "_lib.Context", n: int, moduli: list[int], rows: int = 1,
                                       "blocks_per_cu": best["blocks_per_cu"],
                                       "low_cells": best["low_cells"], "low_kc": best["low_kc"],
                                       "variant": best["variant"],
+                                      "cap_percent": best["cap_percent"],
                                       "row_ms": best["row_ms"], "tuned_at": time.time()}
         save_cache(data, path)
     return result

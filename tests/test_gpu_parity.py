@@ -147,6 +147,9 @@ def test_height_caps_match_whole_state_sweep(gpu_ctx):
             print(f"case caps n={n}", flush=True)
             gpu_ctx.set_height_caps(False)
             whole, _ = gpu_ctx.solve(n, mods)
+            gpu_ctx.set_height_caps(100)
+            every, _ = gpu_ctx.solve(n, mods)
+            assert every == whole, n
             gpu_ctx.set_height_caps(True)
             capped, st = gpu_ctx.solve(n, mods)
             assert capped == whole, n

@@ -76,7 +76,7 @@ def test_capped_sweep_equals_oracle(oracle, n):
         main[lib.or_rank(1 << (2 * (W - 1)), W)] = 1
         for r in range(W):
             for p in range(W - 1, 0, -1):
-                cap, swept = ofp.height_cap(n, r, p)
+                cap, swept = ofp.height_cap(n, r, p, 100)
                 if cap == 0:
                     oracle.apply_cells(n, mod, main, blk, p, p, p != W - 1)
                     continue
