@@ -32,9 +32,10 @@ LAYOUTS = (0, 1)
 # second stage: cells p <= low_cells advance at most low_kc moduli per launch
 LOW_CELLS = ((0, 16), (6, 8), (4, 8))
 VARIANTS = (0, 1, 2, 3)
-# third stage: cap a cell when it then sweeps at most this % of the blocked words
-CAP_PERCENTS = (75, 85, 100)
-CAPPED_ROWS = 5  # rows 1..5 of a solve hold every capped cell of the forward bound
+# third stage: cap a cell when it then sweeps at most this % of the blocked words;
+# timed on rows 1..CAPPED_ROWS of a solve (every row the forward bound caps at n <= 22)
+CAP_PERCENTS = (75, 85, 95)
+CAPPED_ROWS = 12
 
 
 def _key(n: int, k: int) -> str:
@@ -133,18 +134,19 @@ def tune(ctx: "_lib.Context", n: int, moduli: list[int], rows: int = 1,
     ctx.set_variant(best["variant"])
     ctx.set_low_cells(best["low_cells"], best["low_kc"])
     best = dict(best, cap_percent=_lib.DEFAULT_CAP_PERCENT)
-    if best["layout"] == 1 and n + 1 > CAPPED_ROWS + 2:
+    rows_capped = min(CAPPED_ROWS, n - 1)
+    if best["layout"] == 1 and rows_capped >= 1:
         timed = {}
         for pct in CAP_PERCENTS:
             ctx.set_height_caps(pct)
             ctx.prepare(n, moduli)
             ctx.row_sweep(1)
             t0 = time.perf_counter()
-            ctx.row_sweep(CAPPED_ROWS)
-            timed[pct] = (time.perf_counter() - t0) / CAPPED_ROWS * 1e3
+            ctx.row_sweep(rows_capped)
+            timed[pct] = (time.perf_counter() - t0) / rows_capped * 1e3
             table.append(dict(best, cap_percent=pct, capped_rows_ms=timed[pct]))
             if log:
-                log(f"autotune n={n} k={len(moduli)} cap_percent={pct}: rows 1..{CAPPED_ROWS} "
+                log(f"autotune n={n} k={len(moduli)} cap_percent={pct}: rows 1..{rows_capped} "
                     f"{timed[pct]:.2f} ms/row")
         best["cap_percent"] = min(timed, key=timed.get)
     ctx.set_height_caps(best["cap_percent"])
