@@ -123,6 +123,12 @@ ofp_status ofp_set_launch(ofp_ctx* ctx, int threads, int blocks_per_cu);
  * registers).  Set by the autotuner; no reference counterpart. */
 ofp_status ofp_set_variant(ofp_ctx* ctx, int variant);
 
+/* Work distribution over the 8 XCDs: the blocked words are cut into chunks of
+ * one workgroup and dealt to the XCDs in pieces of 2^log2_chunks chunks,
+ * round-robin (default 5); -1 gives each XCD one contiguous eighth.  Set by the
+ * autotuner; no reference counterpart. */
+ofp_status ofp_set_xcd_pieces(ofp_ctx* ctx, int log2_chunks);
+
 /* Height caps.  In a sweep from the initial state (ofp_solve, ofp_prepare +
  * ofp_row_sweep, jobs) the cells of the first rows only touch words of small
  * height (every other word is still zero), and in the last two rows only such

@@ -59,6 +59,7 @@ struct ofp_ctx {
     void* dpc32 = nullptr; // packed capped tables, caps 1..kMaxCap (uint32, uint64)
     void* dpc64 = nullptr;
     int cap_percent = 85;  // height caps: cap a cell when it then sweeps <= this % of the blocked words
+    int xcd_shift = 5;     // chunk pieces per XCD: 2^xcd_shift chunks, < 0 = one eighth each
     std::string err;
     Sweep st;              // the context's own state (solve / prepare / row_sweep)
     int jobs = 0;          // live jobs (serving)
@@ -273,6 +274,7 @@ ofp_status enqueue_cells(ofp_ctx* c, Sweep& s, int p_hi, int p_lo, bool first_re
             a.first = 0;
             a.layout = s.layout;
             a.variant = c->variant;
+            a.xcd_shift = c->xcd_shift;
             a.W = s.W;
             a.p = p;
             a.read_blk = (p == p_hi) ? (first_reads_blk ? 1 : 0) : 1;
@@ -308,8 +310,8 @@ ofp_status full_rows(ofp_ctx* c, Sweep& s, int rows, int* launches) {
         return OFP_OK;
     }
     char key[256];
-    snprintf(key, sizeof key, "%p/%p/%d/%d/%d/%d/%d/%d/%d/%d", (void*)s.main, (void*)s.blk, s.n, s.K,
-             c->threads, c->blocks_per_cu, s.layout, c->low_cells, c->low_kc, c->variant);
+    snprintf(key, sizeof key, "%p/%p/%d/%d/%d/%d/%d/%d/%d/%d/%d", (void*)s.main, (void*)s.blk, s.n, s.K,
+             c->threads, c->blocks_per_cu, s.layout, c->low_cells, c->low_kc, c->variant, c->xcd_shift);
     std::string k = key;
     for (uint32_t m : s.mods) k += "/" + std::to_string(m);
     if (k != s.row_key || !s.row_exec) {
@@ -592,6 +594,14 @@ ofp_status ofp_set_variant(ofp_ctx* c, int variant) {
     return OFP_OK;
 }
 
+ofp_status ofp_set_xcd_pieces(ofp_ctx* c, int log2_chunks) {
+    if (!c) return OFP_E_ARG;
+    if (log2_chunks < -1 || log2_chunks > 24) return fail(c, OFP_E_ARG, "log2_chunks must be -1..24");
+    if (log2_chunks != c->xcd_shift) drop_graph(c->st);
+    c->xcd_shift = log2_chunks;
+    return OFP_OK;
+}
+
 ofp_status ofp_set_height_caps(ofp_ctx* c, int max_percent) {
     if (!c) return OFP_E_ARG;
     if (max_percent < 0 || max_percent > 100) return fail(c, OFP_E_ARG, "max_percent must be 0..100");
@@ -829,6 +839,7 @@ ofp_status ofp_shard_sweep(ofp_ctx* c, const ofp_shard_spec* sp, ofp_shard_info*
             a.first = info->first_blocked;
             a.layout = 1;
             a.variant = c->variant;
+            a.xcd_shift = c->xcd_shift;
             a.W = W;
             a.p = p;
             a.read_blk = (p == W - 1) ? 0 : 1;

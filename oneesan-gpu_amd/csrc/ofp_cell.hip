@@ -40,9 +40,13 @@ namespace {
 using namespace cell;
 
 // Work distribution: the blocked range is cut into chunks of blockDim.x words;
-// workgroup g runs on XCD g%8 (round-robin dispatch), so the chunks are split
-// into 8 contiguous slabs and XCD x's workgroups stride through slab x.  Each
-// XCD's L2 then sees one contiguous region of the state.
+// workgroup g runs on XCD g%8 (round-robin dispatch).  The chunks are dealt to
+// the XCDs in pieces of 2^shift consecutive chunks, round-robin (piece i to XCD
+// i%8), and XCD x's workgroups stride through its pieces in order; shift < 0
+// gives each XCD one contiguous eighth.  Each XCD's L2 sees contiguous pieces,
+// and work whose cost depends on where in the rank order it lies (at the first
+// cells the x=N words, which scan for closures, are the first third of the
+// blocked range) is shared by all XCDs.
 template <int KC, typename Idx, int L, int V>
 __global__ __launch_bounds__(256) void cell_kernel(CellLaunch a) {
     __shared__ Quad<Idx> tab[kDpRows][kDpCols];
@@ -51,12 +55,9 @@ __global__ __launch_bounds__(256) void cell_kernel(CellLaunch a) {
     __syncthreads();
     Tab<Idx> T{tab};
     const Direct<KC, L> mem{a.main, a.blk, a.K, a.off};
-    const uint64_t chunks = (a.nblk + blockDim.x - 1) / blockDim.x;
-    const uint32_t nwg = gridDim.x;  // multiple of 8
-    const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3, nx = nwg >> 3;
-    const uint64_t c0 = (chunks * xcd) >> 3, c1 = (chunks * (xcd + 1)) >> 3;
-    for (uint64_t c = c0 + slot; c < c1; c += nx) {
-        const uint64_t b = c * blockDim.x + threadIdx.x;
+    const Sched sc(a.nblk, blockDim.x, gridDim.x, blockIdx.x, a.xcd_shift);
+    for (uint64_t t = sc.slot; t < sc.count; t += sc.nx) {
+        const uint64_t b = sc.chunk(t) * blockDim.x + threadIdx.x;
         if (b < a.nblk) cell_one<KC, Idx, L, V>(a, T, a.first + b, mem);
     }
 }
@@ -81,12 +82,9 @@ __global__ __launch_bounds__(256) void cell_shard_kernel(CellLaunch a) {
 #pragma unroll
     for (int j = 0; j < KC; ++j) mem.mod[j] = a.mod[j];
     mem.stray = a.stray;
-    const uint64_t chunks = (a.nblk + blockDim.x - 1) / blockDim.x;
-    const uint32_t nwg = gridDim.x;
-    const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3, nx = nwg >> 3;
-    const uint64_t c0 = (chunks * xcd) >> 3, c1 = (chunks * (xcd + 1)) >> 3;
-    for (uint64_t c = c0 + slot; c < c1; c += nx) {
-        const uint64_t b = c * blockDim.x + threadIdx.x;
+    const Sched sc(a.nblk, blockDim.x, gridDim.x, blockIdx.x, a.xcd_shift);
+    for (uint64_t t = sc.slot; t < sc.count; t += sc.nx) {
+        const uint64_t b = sc.chunk(t) * blockDim.x + threadIdx.x;
         if (b < a.nblk) cell_one<KC, Idx, kAoS, V>(a, T, a.first + b, mem);
     }
 }
@@ -108,12 +106,9 @@ __global__ __launch_bounds__(256) void cell_capped_kernel(CellLaunch a) {
     __syncthreads();
     Tab<Idx> T{tab};
     const Direct<KC, kAoS> mem{a.main, a.blk, a.K, a.off};
-    const uint64_t chunks = (a.nblk + blockDim.x - 1) / blockDim.x;
-    const uint32_t nwg = gridDim.x;
-    const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3, nx = nwg >> 3;
-    const uint64_t c0 = (chunks * xcd) >> 3, c1 = (chunks * (xcd + 1)) >> 3;
-    for (uint64_t c = c0 + slot; c < c1; c += nx) {
-        const uint64_t t = c * blockDim.x + threadIdx.x;
+    const Sched sc(a.nblk, blockDim.x, gridDim.x, blockIdx.x, a.xcd_shift);
+    for (uint64_t u = sc.slot; u < sc.count; u += sc.nx) {
+        const uint64_t t = sc.chunk(u) * blockDim.x + threadIdx.x;
         if (t < a.nblk) cell_body<KC, Idx, kAoS, 2>(a, T, decode_capped<Idx>(T, ctab, a.W, a.p, t), mem);
     }
 }
@@ -144,6 +139,7 @@ hipError_t launch_kc(const CellLaunch& a, int threads, int blocks, hipStream_t s
             hipLaunchKernelGGL((cell_kernel<KC, Idx, kAoS, 2>), dim3(blocks), dim3(threads), 0, s, a);
         else if (a.variant == 3)
             hipLaunchKernelGGL((cell_kernel<KC, Idx, kAoS, 3>), dim3(blocks), dim3(threads), 0, s, a);
+
         else
             hipLaunchKernelGGL((cell_kernel<KC, Idx, kAoS, 0>), dim3(blocks), dim3(threads), 0, s, a);
     } else {
@@ -160,6 +156,7 @@ hipError_t launch_shard_kc(const CellLaunch& a, int threads, int blocks, hipStre
         hipLaunchKernelGGL((cell_shard_kernel<KC, Idx, 2>), dim3(blocks), dim3(threads), 0, s, a);
     else if (a.variant == 3)
         hipLaunchKernelGGL((cell_shard_kernel<KC, Idx, 3>), dim3(blocks), dim3(threads), 0, s, a);
+
     else
         hipLaunchKernelGGL((cell_shard_kernel<KC, Idx, 0>), dim3(blocks), dim3(threads), 0, s, a);
     return hipGetLastError();

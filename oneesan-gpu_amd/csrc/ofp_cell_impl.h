@@ -52,6 +52,29 @@ struct Tab {
     }
 };
 
+// Chunk schedule of one workgroup over n words in chunks of bdim (see the
+// work-distribution note in ofp_cell.hip).
+struct Sched {
+    uint64_t count;  // chunks of this workgroup's XCD (multiple of the piece size)
+    uint64_t chunks;
+    uint32_t xcd, slot, nx;
+    int shift;
+    OFP_HD Sched(uint64_t n, uint32_t bdim, uint32_t nwg, uint32_t block, int sh)
+        : chunks((n + bdim - 1) / bdim), xcd(block & 7u), slot(block >> 3), nx(nwg >> 3), shift(sh) {
+        if (shift < 0) {
+            count = ((chunks * (xcd + 1)) >> 3) - ((chunks * xcd) >> 3);
+        } else {
+            const uint64_t pieces = (chunks + (1ull << shift) - 1) >> shift;
+            count = (pieces > xcd ? (pieces - xcd + 7) >> 3 : 0) << shift;
+        }
+    }
+    // the t-th chunk of this XCD; may be >= chunks in a last, partial piece
+    OFP_HD uint64_t chunk(uint64_t t) const {
+        if (shift < 0) return ((chunks * xcd) >> 3) + t;
+        return ((((t >> shift) << 3) + xcd) << shift) + (t & ((1ull << shift) - 1));
+    }
+};
+
 // State layouts.  kAoSoA: tiles of 64 states, modulus-major inside a tile
 // (a wave reading one modulus of 64 consecutive states reads 256 contiguous
 // bytes).  kAoS: state-major with an even stride, residues of one state
@@ -188,7 +211,8 @@ struct Gather {
         pend = cur;
         have = true;
     }
-    OFP_HD void finish(const uint32_t* mod) {
+    template <class Mem>
+    OFP_HD void finish(const Mem&, const uint32_t* mod) {
         if (have) add_into<KC>(acc, pend, mod);
         have = false;
     }
@@ -197,9 +221,9 @@ struct Gather {
 // Closure predecessors of the blocked word [P, N, S] (x = N): RL, LL and RR
 // sources, accumulated into g (the reverse of gridfp_transition.hpp:113-146's
 // closure cases; order and multiplicity as gridfp_reverse.hpp:39-98).
-template <int KC, typename Idx, int L, class Mem>
+template <int KC, typename Idx, int L, class Mem, class G>
 OFP_HD void gather_closures(const Tab<Idx>& T, int W, int p, int hP, Idx mainbase, Idx sr, uint64_t word,
-                            const Mem& m, Gather<KC, L>& g, const uint32_t* mod) {
+                            const Mem& m, G& g, const uint32_t* mod) {
     if (p >= 2) {
         // RL predecessor: [P,R,L,S]
         if (hP > 0) {
@@ -394,7 +418,7 @@ OFP_HD void cell_body(const CellLaunch& a, const Tab<Idx>& T, const Decoded<Idx>
                 Gather<KC, L> g;
                 g.init();
                 gather_closures<KC, Idx, L>(T, W, p, hP, mainbase, sr, word, m, g, mod);
-                g.finish(mod);
+                g.finish(m, mod);
                 if (p == 1) {
                     add_into<KC>(oa, g.acc, mod);
                     m.sm(iA, oa);
@@ -407,7 +431,7 @@ OFP_HD void cell_body(const CellLaunch& a, const Tab<Idx>& T, const Decoded<Idx>
         Gather<KC, L> g;
         g.init();
         if (x == 0) gather_closures<KC, Idx, L>(T, W, p, hP, mainbase, sr, word, m, g, mod);
-        g.finish(mod);
+        g.finish(m, mod);
         Vals<KC> oa, ob;
 #pragma unroll
         for (int j = 0; j < KC; ++j) {
@@ -440,7 +464,7 @@ OFP_HD void cell_body(const CellLaunch& a, const Tab<Idx>& T, const Decoded<Idx>
         Gather<KC, L> g;
         g.init();
         gather_closures<KC, Idx, L>(T, W, p, hP, mainbase, sr, word, m, g, mod);
-        g.finish(mod);
+        g.finish(m, mod);
         if constexpr (V == 1) {
             m.lm(iNN, nn);
             m.lm(iLR, lr);

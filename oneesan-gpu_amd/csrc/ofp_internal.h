@@ -56,6 +56,7 @@ struct CellLaunch {
     int layout;           // 0: 64-state tiles (AoSoA), 1: state-major (AoS)
     int variant;          // state-major only: cell_kernel V parameter (0..3)
     const void* dpc;      // capped launches only: packed capped table of this launch's cap
+    int xcd_shift;        // chunks are dealt to the 8 XCDs in pieces of 2^xcd_shift (< 0: eighths)
     uint32_t mod[kMaxKC];
     // shard launches only (launch_cell_shard): resident main window and remote seed
     uint64_t m0, M;

@@ -32,6 +32,8 @@ LAYOUTS = (0, 1)
 # second stage: cells p <= low_cells advance at most low_kc moduli per launch
 LOW_CELLS = ((0, 16), (6, 8), (4, 8))
 VARIANTS = (0, 1, 2, 3)
+# chunks are dealt to the 8 XCDs in pieces of 2**shift chunks (-1: one eighth each)
+XCD_SHIFTS = (-1, 0, 5, 8)
 # third stage: cap a cell when it then sweeps at most this % of the blocked words;
 # timed on rows 1..CAPPED_ROWS of a solve (every row the forward bound caps at n <= 22)
 CAP_PERCENTS = (75, 85, 95)
@@ -79,6 +81,7 @@ def apply(ctx: "_lib.Context", n: int, k: int) -> dict | None:
         ctx.set_variant(int(ent.get("variant", 2)))
         ctx.set_low_cells(int(ent.get("low_cells", 0)), int(ent.get("low_kc", 16)))
         ctx.set_height_caps(int(ent.get("cap_percent", _lib.DEFAULT_CAP_PERCENT)))
+        ctx.set_xcd_pieces(int(ent.get("xcd_shift", 5)))
     return ent
 
 
@@ -91,6 +94,7 @@ def tune(ctx: "_lib.Context", n: int, moduli: list[int], rows: int = 1,
     ctx.set_height_caps(0)
     ctx.set_variant(2)
     ctx.set_low_cells(0, 16)
+    ctx.set_xcd_pieces(5)
     for lay in layouts:
         ctx.set_layout(lay)
         ctx.prepare(n, moduli)
@@ -133,6 +137,21 @@ def tune(ctx: "_lib.Context", n: int, moduli: list[int], rows: int = 1,
                     best = ent
     ctx.set_variant(best["variant"])
     ctx.set_low_cells(best["low_cells"], best["low_kc"])
+    best = dict(best, xcd_shift=5)
+    ctx.prepare(n, moduli)
+    for sh in XCD_SHIFTS:
+        ctx.set_xcd_pieces(sh)
+        ctx.row_sweep(1)
+        t0 = time.perf_counter()
+        st = ctx.row_sweep(rows)
+        row_ms = (time.perf_counter() - t0) / rows * 1e3
+        ent = dict(best, xcd_shift=sh, row_ms=row_ms, kernel_ms_per_row=st["kernel_s"] / rows * 1e3)
+        table.append(ent)
+        if log:
+            log(f"autotune n={n} k={len(moduli)} xcd_shift={sh}: {row_ms:.2f} ms/row")
+        if row_ms < best["row_ms"]:
+            best = ent
+    ctx.set_xcd_pieces(best["xcd_shift"])
     best = dict(best, cap_percent=_lib.DEFAULT_CAP_PERCENT)
     rows_capped = min(CAPPED_ROWS, n - 1)
     if best["layout"] == 1 and rows_capped >= 1:
@@ -158,6 +177,7 @@ def tune(ctx: "_lib.Context", n: int, moduli: list[int], rows: int = 1,
                                       "low_cells": best["low_cells"], "low_kc": best["low_kc"],
                                       "variant": best["variant"],
                                       "cap_percent": best["cap_percent"],
+                                      "xcd_shift": best["xcd_shift"],
                                       "row_ms": best["row_ms"], "tuned_at": time.time()}
         save_cache(data, path)
     return result

@@ -162,3 +162,18 @@ def test_height_caps_match_whole_state_sweep(gpu_ctx):
             assert gpu_ctx.read_main(0, len(mods)) == whole, n
     finally:
         gpu_ctx.set_height_caps(True)
+
+
+def test_xcd_pieces_same_residues(gpu_ctx):
+    """Dealing the chunks to the XCDs in pieces of any size changes no residue."""
+    mods = [4294967291, 4294966997, 2147483647, 65537]
+    try:
+        gpu_ctx.set_xcd_pieces(-1)
+        want, _ = gpu_ctx.solve(15, mods)
+        for shift in (0, 1, 4, 9):
+            print(f"case xcd pieces 2^{shift}", flush=True)
+            gpu_ctx.set_xcd_pieces(shift)
+            got, _ = gpu_ctx.solve(15, mods)
+            assert got == want, shift
+    finally:
+        gpu_ctx.set_xcd_pieces(-1)
