@@ -3,6 +3,7 @@
 Produces, in-tree:
   oneesan-gpu_amd/lib/libofp.so    the C-ABI library (HIP kernels + host runtime)
   oneesan-gpu_amd/lib/ofp_solver   the CLI (reference argv / result-line contract)
+  oneesan-gpu_amd/lib/ofp_solver.provenance.json  its build provenance sidecar
   oneesan-gpu_amd/lib/ofp_hbm_probe  streaming read / copy / read-modify-write kernels
                                      (the HBM ceiling the cell kernels are compared with)
 
@@ -80,6 +81,10 @@ def build(force: bool = True, jobs: int = 4) -> Path:
         cli = LIB / "ofp_solver"
         _run([HIPCC, *HIP_FLAGS, "-x", "c++", str(cli_src), f"-L{LIB}", "-lofp",
               f"-Wl,-rpath,$ORIGIN", "-o", str(cli)])
+        # provenance sidecar lib/ofp_solver.provenance.json (what the binary was built from)
+        sys.path.insert(0, str(PKG))
+        from ofp import provenance
+        provenance.create_for_build(cli, compile_args=HIP_FLAGS, compiler=HIPCC)
     probe_src = CSRC / "hbm_probe.hip"
     if probe_src.exists():
         _run([HIPCC, *HIP_FLAGS, str(probe_src), "-o", str(LIB / "ofp_hbm_probe")])

@@ -14,6 +14,11 @@ Fixes relative to the reference: the reference's batch runner calls an
 undefined solver_fingerprint() (solve_b300_exact_batch.py:106) and raises
 NameError at start; here the checkpoint identity (solver_identity) is used.
 
+Build provenance: when the solver binary has a sidecar next to it
+(<binary>.provenance.json, written by oneesan-gpu_amd/build.py), it is checked
+against the binary at start and copied into the work directory as
+build_provenance.json, which the manifest names and hashes.
+
 Exit codes: 0 complete (or partial with --max-runs), 2 partial, others are
 errors (SystemExit with a message).
 """
@@ -25,9 +30,10 @@ import subprocess
 import sys
 from pathlib import Path
 
+from . import provenance
 from .exact_safety import (
-    acquire_workdir_lock, crt_pair, load_checkpoint, save_checkpoint, solver_identity,
-    validate_exact_reconstruction, validate_residue, write_exact_result,
+    acquire_workdir_lock, crt_pair, durable_atomic_write, load_checkpoint, save_checkpoint,
+    solver_identity, validate_exact_reconstruction, validate_residue, write_exact_result,
 )
 from .path_bound import primes_for_bound, simple_path_upper_bound
 from .solver_output import parse_result_line
@@ -52,7 +58,7 @@ def reconstruct(prefix: list[int], residues: dict[int, dict]) -> tuple[int, int,
 
 def finish(work: Path, n: int, path_bound: int, prefix: list[int], residues: dict[int, dict],
            strip_partition: list[int], identity: dict[str, str], checkpoint: Path,
-           out=sys.stdout) -> int:
+           out=sys.stdout, build_provenance: Path | None = None) -> int:
     x, m, wall, used = reconstruct(prefix, residues)
     if m <= path_bound:
         raise RuntimeError("finish called before CRT capacity reached")
@@ -69,7 +75,7 @@ def finish(work: Path, n: int, path_bound: int, prefix: list[int], residues: dic
     exact_path, manifest = write_exact_result(
         work, n=n, exact=x, path_bound=path_bound, modulus_product=m, used_moduli=used_moduli,
         residues=residues, identity=identity, checkpoint_path=checkpoint, total_wall_s=wall,
-        strip_partition=strip_partition)
+        strip_partition=strip_partition, build_provenance=build_provenance)
     print(f"n={n}\nexact={x}\nbound_bits={path_bound.bit_length()}\nmodulus_bits={m.bit_length()}\n"
           f"primes_used={k}\nsolver_wall_s_sum={wall:.9f}\nresult_file={exact_path}\n"
           f"manifest_file={manifest}", file=out)
@@ -97,6 +103,15 @@ def run(n: int, binary: Path, work: Path, *, target_mib: int = 16384, max_window
         checkpoint = work / "checkpoint.json"
         identity = solver_identity(binary)
         print(f"solver identity: sha256={identity['binary_sha256']}", file=err)
+        prov = None
+        sidecar = Path(str(binary) + provenance.SIDECAR_SUFFIX)
+        if sidecar.is_file():
+            try:
+                provenance.load(sidecar, binary=binary)
+            except ValueError as exc:
+                raise SystemExit(f"solver build provenance does not match the binary: {exc}") from exc
+            prov = work / "build_provenance.json"
+            durable_atomic_write(prov, sidecar.read_text(encoding="utf-8"))
         try:
             residues = load_checkpoint(checkpoint, n=n, identity=identity)
         except ValueError as exc:
@@ -106,7 +121,7 @@ def run(n: int, binary: Path, work: Path, *, target_mib: int = 16384, max_window
             raise SystemExit(f"checkpoint contains unexpected moduli: {unexpected}")
         _, m, _, _ = reconstruct(prefix, residues)
         if m > path_bound:
-            return finish(work, n, path_bound, prefix, residues, strips, identity, checkpoint, out)
+            return finish(work, n, path_bound, prefix, residues, strips, identity, checkpoint, out, prov)
         missing = [p for p in prefix if p not in residues]
         if max_runs:
             missing = missing[:max_runs]
@@ -153,7 +168,7 @@ def run(n: int, binary: Path, work: Path, *, target_mib: int = 16384, max_window
             raise SystemExit(f"batch solver exited without residues for: {sorted(set(missing) - seen, reverse=True)}")
         _, m, _, _ = reconstruct(prefix, residues)
         if m > path_bound:
-            return finish(work, n, path_bound, prefix, residues, strips, identity, checkpoint, out)
+            return finish(work, n, path_bound, prefix, residues, strips, identity, checkpoint, out, prov)
         print(f"partial: CRT bits={m.bit_length()} bound_bits={bits}; cached={len(residues)}", file=err)
         return 0 if max_runs else 2
     finally:

@@ -250,7 +250,8 @@ def acquire_workdir_lock(work: Path) -> TextIO:
 def write_exact_result(work: Path, *, n: int, exact: int, path_bound: int, modulus_product: int,
                        used_moduli: list[int], residues: dict[int, dict], identity: dict[str, str],
                        checkpoint_path: Path, total_wall_s: float,
-                       strip_partition: list[int]) -> tuple[Path, Path]:
+                       strip_partition: list[int],
+                       build_provenance: Path | None = None) -> tuple[Path, Path]:
     validate_residue_records(residues)
     if len(set(used_moduli)) != len(used_moduli):
         raise ValueError("exact result contains duplicate CRT moduli")
@@ -299,8 +300,8 @@ def write_exact_result(work: Path, *, n: int, exact: int, path_bound: int, modul
         "checkpoint_sha256": file_sha256(checkpoint_path),
         "exact_file": exact_path.name,
         "exact_sha256": hashlib.sha256(exact_text.encode()).hexdigest(),
-        "build_provenance_file": None,
-        "build_provenance_sha256": None,
+        "build_provenance_file": Path(build_provenance).name if build_provenance else None,
+        "build_provenance_sha256": file_sha256(build_provenance) if build_provenance else None,
         "admission_certificate_file": None,
         "admission_certificate_sha256": None,
         "admission_certificate_schema": None,

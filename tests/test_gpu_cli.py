@@ -55,3 +55,7 @@ def test_crt_runner_exact_n12(tmp_path):
     out2 = io.StringIO()
     assert crt_runner.run(12, CLI, tmp_path / "w", out=out2, err=io.StringIO()) == 0
     assert "batch run" not in out2.getvalue()
+    # the result verifies from first principles, with the build's provenance sidecar
+    from ofp import verify
+    assert man["build_provenance_file"] == "build_provenance.json"
+    assert verify.verify(tmp_path / "w" / "exact_manifest.json", binary=CLI)["n"] == 12
