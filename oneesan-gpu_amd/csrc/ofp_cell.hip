@@ -31,6 +31,9 @@
 //
 // Residues: uint32 mod p, additions only (every transfer coefficient is 1);
 // a + b mod p is computed with one compare-and-subtract.
+#include <cstring>
+#include <vector>
+
 #include "ofp_cell_impl.h"
 
 namespace ofp {
@@ -249,17 +252,22 @@ hipError_t launch_set_one(uint32_t* main, uint64_t index, int K, int k, int layo
     return hipGetLastError();
 }
 
+// Packed into a properly aligned buffer, then copied out byte-wise (out may be
+// any host buffer).
+template <typename Idx>
+static size_t pack_into(const uint64_t (*d)[kDpCols], void* out, size_t cap) {
+    const size_t bytes = sizeof(Quad<Idx>) * kDpRows * kDpCols;
+    if (cap < bytes) return 0;
+    std::vector<Quad<Idx>> q(kDpRows * kDpCols);
+    pack_quads<Idx>(d, (Quad<Idx>(*)[kDpCols])q.data());
+    std::memcpy(out, q.data(), bytes);
+    return bytes;
+}
+
 size_t pack_height_table(bool wide, void* out, size_t cap) {
     uint64_t d[kDpRows][kDpCols];
     build_height_table(d);
-    if (wide) {
-        if (cap < sizeof(Quad<uint64_t>) * kDpRows * kDpCols) return 0;
-        pack_quads<uint64_t>(d, (Quad<uint64_t>(*)[kDpCols])out);
-        return sizeof(Quad<uint64_t>) * kDpRows * kDpCols;
-    }
-    if (cap < sizeof(Quad<uint32_t>) * kDpRows * kDpCols) return 0;
-    pack_quads<uint32_t>(d, (Quad<uint32_t>(*)[kDpCols])out);
-    return sizeof(Quad<uint32_t>) * kDpRows * kDpCols;
+    return wide ? pack_into<uint64_t>(d, out, cap) : pack_into<uint32_t>(d, out, cap);
 }
 
 void build_capped_table(int cap, uint64_t (*d)[kDpCols]) {
@@ -279,11 +287,18 @@ size_t pack_capped_tables(bool wide, void* out, size_t cap) {
     const size_t one = (wide ? sizeof(Pair<uint64_t>) : sizeof(Pair<uint32_t>)) * kDpRows * kDpCols;
     if (cap < one * kMaxCap) return 0;
     uint64_t d[kDpRows][kDpCols];
+    std::vector<Pair<uint64_t>> q64(kDpRows * kDpCols);
+    std::vector<Pair<uint32_t>> q32(kDpRows * kDpCols);
     for (int c = 1; c <= kMaxCap; ++c) {
         build_capped_table(c, d);
         char* dst = (char*)out + one * (c - 1);
-        if (wide) pack_pairs<uint64_t>(d, (Pair<uint64_t>(*)[kDpCols])dst);
-        else pack_pairs<uint32_t>(d, (Pair<uint32_t>(*)[kDpCols])dst);
+        if (wide) {
+            pack_pairs<uint64_t>(d, (Pair<uint64_t>(*)[kDpCols])q64.data());
+            std::memcpy(dst, q64.data(), one);
+        } else {
+            pack_pairs<uint32_t>(d, (Pair<uint32_t>(*)[kDpCols])q32.data());
+            std::memcpy(dst, q32.data(), one);
+        }
     }
     return one * kMaxCap;
 }

@@ -23,9 +23,59 @@ OFP_HD uint32_t addm(uint32_t a, uint32_t b, uint32_t m) {
 // rank (width W, every prefix position shifted up by one) needs one 16-byte
 // (or 32-byte) table read per position.
 template <typename Idx>
-struct Quad {
+struct alignas(4 * sizeof(Idx)) Quad {
     Idx zn, zr, mn, mr;
 };
+
+// Whole-entry table reads (16 or 32 bytes at once).  Left to itself the
+// compiler reads only the fields each branch of a step needs, one dependent
+// LDS access after another; the decode walks and closure scans are chains of
+// such steps, so one wide read per step shortens every chain.
+template <typename Idx>
+OFP_HD Quad<Idx> ldq(const Quad<Idx>& e) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (sizeof(Idx) == 4) {
+        const uint4 v = *reinterpret_cast<const uint4*>(&e);
+        return Quad<Idx>{v.x, v.y, v.z, v.w};
+    } else {
+        const ulonglong2* q = reinterpret_cast<const ulonglong2*>(&e);
+        const ulonglong2 a = q[0], b = q[1];
+        return Quad<Idx>{(Idx)a.x, (Idx)a.y, (Idx)b.x, (Idx)b.y};
+    }
+#else
+    return e;
+#endif
+}
+// The first half only (zn, zr): what the closure scans need.
+template <typename Idx>
+struct Z2 {
+    Idx zn, zr;
+};
+template <typename Idx>
+OFP_HD Z2<Idx> ldz(const Quad<Idx>& e) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (sizeof(Idx) == 4) {
+        const uint2 v = *reinterpret_cast<const uint2*>(&e);
+        return Z2<Idx>{v.x, v.y};
+    } else {
+        const ulonglong2 v = *reinterpret_cast<const ulonglong2*>(&e);
+        return Z2<Idx>{(Idx)v.x, (Idx)v.y};
+    }
+#else
+    return Z2<Idx>{e.zn, e.zr};
+#endif
+}
+// One position of a rank walk without branches: v = number of completion
+// blocks (N, then R) that r passes; contrib = their size; r -= contrib.
+// zn + zr never wraps: it is at most D[pos+1][h], a word count of the state.
+template <typename Idx>
+OFP_HD int step_z(Idx zn, Idx zr, Idx& r, Idx& contrib) {
+    const Idx b = zn + zr;
+    const int v = (r >= zn) + (r >= b);
+    contrib = v == 0 ? Idx(0) : (v == 1 ? zn : b);
+    r -= contrib;
+    return v;
+}
 
 template <typename Idx>
 struct Tab {
@@ -39,16 +89,11 @@ struct Tab {
     }
     // decode one position: consume r, return the value and its contribution
     OFP_HD int step(int pos, int h, Idx& r, Idx& contrib) const {
-        const Quad<Idx> e = q[pos][h];
-        return step_e(e, r, contrib);
+        const Z2<Idx> e = ldz(q[pos][h]);
+        return step_z(e.zn, e.zr, r, contrib);
     }
     OFP_HD static int step_e(const Quad<Idx>& e, Idx& r, Idx& contrib) {
-        if (r < e.zn) { contrib = 0; return 0; }
-        r -= e.zn;
-        if (r < e.zr) { contrib = e.zn; return 1; }
-        r -= e.zr;
-        contrib = e.zn + e.zr;
-        return 2;
+        return step_z(e.zn, e.zr, r, contrib);
     }
 };
 
@@ -145,16 +190,18 @@ struct Direct {
 };
 
 // Synthetic residues of states that live on other shards: a seeded hash of the
-// global index (blocked words in their own domain), reduced mod p.  Used for
-// shard measurements; the CPU oracle evaluates the same function.
-OFP_HD uint64_t mix64(uint64_t x) {
-    x += 0x9E3779B97F4A7C15ull;
-    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-    return x ^ (x >> 31);
-}
+// global index (blocked words in their own domain), mapped onto [0, p) by a
+// multiply-high.  32-bit arithmetic only, so that standing in for a read costs
+// little more than the read would.  The CPU oracle evaluates the same function.
 OFP_HD uint32_t remote_residue(uint64_t seed, uint64_t index, int j, uint32_t mod) {
-    return (uint32_t)(mix64(seed ^ (index * 0x100000001B3ull + (uint64_t)j)) >> 32) % mod;
+    uint32_t h = (uint32_t)index * 0x9E3779B1u ^ ((uint32_t)(index >> 32) + (uint32_t)seed) * 0x85EBCA77u ^
+                 ((uint32_t)j + (uint32_t)(seed >> 32)) * 0xC2B2AE3Du;
+    h ^= h >> 15;
+    h *= 0x2C1B3C6Du;
+    h ^= h >> 12;
+    h *= 0x297A2D39u;
+    h ^= h >> 15;
+    return (uint32_t)(((uint64_t)h * mod) >> 32);
 }
 constexpr uint64_t kBlockedDomain = 1ull << 62;
 
@@ -237,9 +284,10 @@ OFP_HD void gather_closures(const Tab<Idx>& T, int W, int p, int hP, Idx mainbas
             Idx rr = sr, s0 = 0, s2 = 0;
             int hh = hP, depth = 1;
             for (int pos = p - 2; pos >= 0; --pos) {
+                const Z2<Idx> e = ldz(T.q[pos][hh]);
+                const Z2<Idx> e2 = ldz(T.q[pos][hh + 2]);
                 Idx c0;
-                const int v = T.step(pos, hh, rr, c0);
-                const Quad<Idx> e2 = T.q[pos][hh + 2];
+                const int v = step_z(e.zn, e.zr, rr, c0);
                 if (v == 2 && depth == 1)
                     g.add(m, (uint64_t)(baseLL + s2 + e2.zn + (sr - s0 - c0)), mod);
                 s0 += c0;
@@ -256,11 +304,13 @@ OFP_HD void gather_closures(const Tab<Idx>& T, int W, int p, int hP, Idx mainbas
         const Idx tail = T.at(p, hP + 2) + T.at(p - 1, hP + 1) + sr;
         Idx u0 = 0, u2 = 0;
         int depth = 1, Ha = hP;
+        uint64_t w = word >> (2 * p);  // P from its lowest position up
         for (int mp = p + 1; mp <= W - 1; ++mp) {
-            const int v = (int)((word >> (2 * (mp - 1))) & 3u);
+            const int v = (int)(w & 3u);
+            w >>= 2;
             const int H = Ha + (v == 1) - (v == 2);
-            const Quad<Idx> e0 = T.q[mp][H];
-            const Quad<Idx> e2 = T.q[mp][H + 2];
+            const Z2<Idx> e0 = ldz(T.q[mp][H]);
+            const Z2<Idx> e2 = ldz(T.q[mp][H + 2]);
             const Idx c0 = (v == 0) ? Idx(0) : (v == 1 ? e0.zn : Idx(e0.zn + e0.zr));
             if (v == 1 && depth == 1)
                 g.add(m, (uint64_t)(mainbase - (u0 + c0) + e0.zn + e0.zr + u2 + tail), mod);
@@ -298,13 +348,13 @@ OFP_HD Decoded<Idx> decode_blocked(const Tab<Idx>& T, int W, int p, uint64_t bid
     Idx r = (Idx)bidx;
     int h = 1;
     Idx mainbase = 0;
-    uint64_t word = 0;
+    uint64_t word = 0;  // built top-down, then moved to bits 2*pos
     for (int pos = W - 2; pos >= p; --pos) {
-        const Quad<Idx> e = T.q[pos][h];
+        const Quad<Idx> e = ldq(T.q[pos][h]);
         Idx cb;
-        const int v = Tab<Idx>::step_e(e, r, cb);
+        const int v = step_z(e.zn, e.zr, r, cb);
         mainbase += (v == 0) ? Idx(0) : (v == 1 ? e.mn : Idx(e.mn + e.mr));
-        word |= (uint64_t)v << (2 * pos);
+        word = (word << 2) | (uint64_t)v;
         h += (v == 2) - (v == 1);
     }
     d.hP = h;
@@ -312,14 +362,14 @@ OFP_HD Decoded<Idx> decode_blocked(const Tab<Idx>& T, int W, int p, uint64_t bid
     d.x = T.step(p - 1, h, r, cx);
     d.sr = r;
     d.mainbase = mainbase;
-    d.word = word;
+    d.word = word << (2 * p);
     return d;
 }
 
 // Capped completions for the words whose heights stay <= a cap c: at position
 // pos entered at height h, n = Dc[pos][h] (choose N), r = Dc[pos][h-1] (choose R).
 template <typename Idx>
-struct Pair {
+struct alignas(2 * sizeof(Idx)) Pair {
     Idx n, r;
 };
 
@@ -336,13 +386,9 @@ OFP_HD Decoded<Idx> decode_capped(const Tab<Idx>& T, const Pair<Idx> (*C)[kDpCol
     int h = 1, x = 0, hP = 1;
     for (int pos = W - 2; pos >= 0; --pos) {
         const Pair<Idx> pc = C[pos][h];
-        const Quad<Idx> e = T.q[pos][h];
-        int v = 0;
-        if (r >= pc.n) {
-            r -= pc.n;
-            if (r < pc.r) v = 1;
-            else { r -= pc.r; v = 2; }
-        }
+        const Quad<Idx> e = ldq(T.q[pos][h]);
+        Idx skip;
+        const int v = step_z(pc.n, pc.r, r, skip);
         const Idx cb = (v == 0) ? Idx(0) : (v == 1 ? e.zn : Idx(e.zn + e.zr));
         bidx += cb;
         if (pos >= p) mainbase += (v == 0) ? Idx(0) : (v == 1 ? e.mn : Idx(e.mn + e.mr));

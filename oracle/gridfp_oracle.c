@@ -371,14 +371,15 @@ int or_num_threads(void) {
  * Synthetic residues of words that live on other shards: a seeded hash of the
  * global index (blocked words in their own domain, bit 62), reduced mod p --
  * the same definition the product's shard sweep uses.  */
-static uint64_t or_mix64(uint64_t x) {
-    x += 0x9E3779B97F4A7C15ull;
-    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-    return x ^ (x >> 31);
-}
 uint32_t or_remote_residue(uint64_t seed, uint64_t index, int j, uint32_t mod) {
-    return (uint32_t)(or_mix64(seed ^ (index * 0x100000001B3ull + (uint64_t)j)) >> 32) % mod;
+    uint32_t h = (uint32_t)index * 0x9E3779B1u ^ ((uint32_t)(index >> 32) + (uint32_t)seed) * 0x85EBCA77u ^
+                 ((uint32_t)j + (uint32_t)(seed >> 32)) * 0xC2B2AE3Du;
+    h ^= h >> 15;
+    h *= 0x2C1B3C6Du;
+    h ^= h >> 12;
+    h *= 0x297A2D39u;
+    h ^= h >> 15;
+    return (uint32_t)(((uint64_t)h * mod) >> 32);
 }
 
 /* Cells p_hi..p_lo on a shard: main window [m0, m0+M), blocked words

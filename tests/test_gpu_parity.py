@@ -177,3 +177,15 @@ def test_xcd_pieces_same_residues(gpu_ctx):
             assert got == want, shift
     finally:
         gpu_ctx.set_xcd_pieces(-1)
+
+
+@pytest.mark.parametrize("n", [22, 23])
+def test_largest_reference_residues(gpu_ctx, n):
+    """n=22 (the bench size; 32-bit index path) and n=23 (64-bit index path, 7.8e9 states)
+    one-modulus solves against the residues the reference publishes."""
+    g = load_golden("reference_residues.json")
+    pairs = [(r["modulus"], r["residue"]) for r in g["residues"] if r["n"] == n]
+    assert pairs
+    print(f"case reference n={n} (whole solve, one modulus)", flush=True)
+    got, st = gpu_ctx.solve(n, [pairs[0][0]])
+    assert got == [pairs[0][1]], (n, got)
