@@ -169,8 +169,9 @@ def main() -> int:
     log(f"bench: launch config {tuned}")
     plan = ctx.plan(n, k)
     log(f"bench: state {plan.state_bytes / 2**30:.1f} GiB, free {plan.free_bytes / 2**30:.1f} GiB")
-    # first prepare allocates the state (the driver may clear the pages first: that is
-    # allocation, not solve time); the timed solve starts from the resident state
+    # first prepare allocates the state (the operating system may clear freshly mapped
+    # device pages first: allocation, not solve time); the timed solve starts from the
+    # resident state
     ctx.prepare(n, mods)
     tw0 = time.perf_counter()
     ctx.prepare(n, mods)
