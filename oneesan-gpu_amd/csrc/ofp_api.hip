@@ -1,11 +1,14 @@
 // ofp_api.hip -- host runtime behind include/ofp.h.
 //
 // One context = one GPU, one non-blocking stream (default priority), the
-// height table on the device, and one device-resident frontier state
-// (main + blocked words, k moduli interleaved in 64-state tiles).
-// A solve initialises the state (reference: fullmate_dropN_batch.cu:508-511),
-// runs W rows of cells p = W-1..1 (each row captured once in a hipGraph and
-// replayed), and reads main[rank(R)] = main[0] (fullmate_dropN_batch.cu:514).
+// height tables on the device, and one device-resident frontier state
+// (main + blocked words, k moduli interleaved per state).  A solve initialises
+// the state (reference: fullmate_dropN_batch.cu:508-511), runs W rows of cells
+// p = W-1..1 -- rows whose cells all sweep the whole state are captured once in
+// a hipGraph and replayed, the first rows and the last two run height-capped
+// cells directly -- and reads main[rank(R)] = main[0]
+// (fullmate_dropN_batch.cu:514).  Jobs (serving) hold their own state and
+// stream; shard sweeps run one GPU's window of an n=27 row.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

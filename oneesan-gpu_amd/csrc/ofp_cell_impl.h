@@ -174,9 +174,6 @@ OFP_HD void add_into(Vals<KC>& acc, const Vals<KC>& x, const uint32_t* mod) {
     for (int j = 0; j < KC; ++j) acc.v[j] = addm(acc.v[j], x.v[j], mod[j]);
 }
 
-// Closure predecessors are found one at a time by a scan; their residues are
-// accumulated one candidate late, so the loads of candidate c+1 are issued
-// before the values of candidate c are added (no memory latency per candidate).
 // Memory policy of a launch: Direct addresses the whole device-resident state.
 template <int KC, int L>
 struct Direct {
@@ -241,6 +238,9 @@ struct Windowed {
     OFP_HD void sb(uint64_t b, const Vals<KC>& v) const { store_vals<KC, 1>(blk, b - b0, K, off, v); }
 };
 
+// Closure predecessors are found one at a time by a scan; their residues are
+// accumulated one candidate late, so the loads of candidate c+1 are issued
+// before the values of candidate c are added (no memory latency per candidate).
 template <int KC, int L>
 struct Gather {
     Vals<KC> acc, pend;
