@@ -57,7 +57,7 @@ __global__ __launch_bounds__(256) void cell_kernel(CellLaunch a) {
     for (int t = threadIdx.x; t < kDpRows * kDpCols; t += blockDim.x) (&tab[0][0])[t] = g[t];
     __syncthreads();
     Tab<Idx> T{tab};
-    const Direct<KC, L> mem{a.main, a.blk, a.K, a.off};
+    const Direct<KC, L, Idx> mem{a.main, a.blk, a.K, a.off};
     const Sched sc(a.nblk, blockDim.x, gridDim.x, blockIdx.x, a.xcd_shift);
     for (uint64_t t = sc.slot; t < sc.count; t += sc.nx) {
         const uint64_t b = sc.chunk(t) * blockDim.x + threadIdx.x;
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(256) void cell_capped_kernel(CellLaunch a) {
     }
     __syncthreads();
     Tab<Idx> T{tab};
-    const Direct<KC, kAoS> mem{a.main, a.blk, a.K, a.off};
+    const Direct<KC, kAoS, Idx> mem{a.main, a.blk, a.K, a.off};
     const Sched sc(a.nblk, blockDim.x, gridDim.x, blockIdx.x, a.xcd_shift);
     for (uint64_t u = sc.slot; u < sc.count; u += sc.nx) {
         const uint64_t t = sc.chunk(u) * blockDim.x + threadIdx.x;

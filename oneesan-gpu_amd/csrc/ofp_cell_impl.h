@@ -11,9 +11,12 @@
 namespace ofp {
 namespace cell {
 
+// a + b mod m for a, b < m <= 2^32 - 1, without forming the possibly wrapping
+// sum: a + b >= m exactly when a >= m - b, and then a - (m - b) is the answer.
+// (m is uniform across a launch, so m - b is one subtract from a scalar.)
 OFP_HD uint32_t addm(uint32_t a, uint32_t b, uint32_t m) {
-    uint32_t s = a + b;
-    return (s < a || s >= m) ? s - m : s;
+    const uint32_t mb = m - b;
+    return a >= mb ? a - mb : a + b;
 }
 
 // One entry of the packed height table, for position pos and height h:
@@ -130,7 +133,7 @@ enum Layout : int { kAoSoA = 0, kAoS = 1 };
 template <int L>
 OFP_HD uint64_t elem(uint64_t i, int K, int j) {
     if constexpr (L == kAoSoA) return aosoa(i, K, j);
-    else return i * (uint64_t)K + (uint64_t)j;
+    else return i * (uint64_t)(uint32_t)K + (uint64_t)(uint32_t)j;
 }
 
 template <int KC>
@@ -175,15 +178,17 @@ OFP_HD void add_into(Vals<KC>& acc, const Vals<KC>& x, const uint32_t* mod) {
 }
 
 // Memory policy of a launch: Direct addresses the whole device-resident state.
-template <int KC, int L>
+// Idx: the index width of the launch; a 32-bit index makes every element
+// offset one 32x32->64 multiply-add.
+template <int KC, int L, typename Idx = uint64_t>
 struct Direct {
     uint32_t* __restrict__ main;
     uint32_t* __restrict__ blk;
     int K, off;
-    OFP_HD void lm(uint64_t i, Vals<KC>& v) const { load_vals<KC, L>(main, i, K, off, v); }
-    OFP_HD void sm(uint64_t i, const Vals<KC>& v) const { store_vals<KC, L>(main, i, K, off, v); }
-    OFP_HD void lb(uint64_t b, Vals<KC>& v) const { load_vals<KC, L>(blk, b, K, off, v); }
-    OFP_HD void sb(uint64_t b, const Vals<KC>& v) const { store_vals<KC, L>(blk, b, K, off, v); }
+    OFP_HD void lm(uint64_t i, Vals<KC>& v) const { load_vals<KC, L>(main, (uint64_t)(Idx)i, K, off, v); }
+    OFP_HD void sm(uint64_t i, const Vals<KC>& v) const { store_vals<KC, L>(main, (uint64_t)(Idx)i, K, off, v); }
+    OFP_HD void lb(uint64_t b, Vals<KC>& v) const { load_vals<KC, L>(blk, (uint64_t)(Idx)b, K, off, v); }
+    OFP_HD void sb(uint64_t b, const Vals<KC>& v) const { store_vals<KC, L>(blk, (uint64_t)(Idx)b, K, off, v); }
 };
 
 // Synthetic residues of states that live on other shards: a seeded hash of the
