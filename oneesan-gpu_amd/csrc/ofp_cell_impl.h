@@ -13,10 +13,13 @@ namespace cell {
 
 // a + b mod m for a, b < m <= 2^32 - 1, without forming the possibly wrapping
 // sum: a + b >= m exactly when a >= m - b, and then a - (m - b) is the answer.
-// (m is uniform across a launch, so m - b is one subtract from a scalar.)
+// The borrow of a - (m - b) is that comparison, so the whole is four VALU
+// operations (m - b from a scalar, subtract with borrow, add, select).
 OFP_HD uint32_t addm(uint32_t a, uint32_t b, uint32_t m) {
     const uint32_t mb = m - b;
-    return a >= mb ? a - mb : a + b;
+    uint32_t t;
+    const bool lt = __builtin_sub_overflow(a, mb, &t);
+    return lt ? a + b : t;
 }
 
 // One entry of the packed height table, for position pos and height h:
@@ -244,8 +247,11 @@ struct Windowed {
 };
 
 // Closure predecessors are found one at a time by a scan; their residues are
-// accumulated one candidate late, so the loads of candidate c+1 are issued
-// before the values of candidate c are added (no memory latency per candidate).
+// accumulated one candidate late: candidate c's values are added when candidate
+// c+1 is found (or at finish), so the scan runs on while c's loads are in flight.
+// The loads go straight into `pend` after its previous contents are added: a
+// load into a temporary followed by a copy into `pend` would make the copy wait
+// for the load at once.
 template <int KC, int L>
 struct Gather {
     Vals<KC> acc, pend;
@@ -257,10 +263,8 @@ struct Gather {
     }
     template <class Mem>
     OFP_HD void add(const Mem& m, uint64_t i, const uint32_t* mod) {
-        Vals<KC> cur;
-        m.lm(i, cur);
         if (have) add_into<KC>(acc, pend, mod);
-        pend = cur;
+        m.lm(i, pend);
         have = true;
     }
     template <class Mem>
