@@ -22,15 +22,19 @@ OFP_HD uint32_t addm(uint32_t a, uint32_t b, uint32_t m) {
     return lt ? a + b : t;
 }
 
-// One entry of the packed height table, for position pos and height h:
-//   zn = D[pos][h], zr = D[pos][h-1] (0 at h=0)      -- decode at pos
-//   mn = D[pos+1][h], mr = D[pos+1][h-1] (0 at h=0)  -- the same value one position up
-// so that decoding a blocked word (width W-1) and accumulating its main-numbering
-// rank (width W, every prefix position shifted up by one) needs one 16-byte
-// (or 32-byte) table read per position.
+// One entry of the packed height table, for position pos and height h, with
+// D[.][-1] = 0:
+//   zn = D[pos][h], zc = D[pos][h] + D[pos][h-1]            -- decode at pos
+//   mn = D[pos+1][h], mc = D[pos+1][h] + D[pos+1][h-1]      -- the same one position up
+// zn is the rank contribution of an R at pos (the N completions it passes), zc
+// that of an L (N and R completions), so a walk step compares against and adds
+// table values only.  Decoding a blocked word (width W-1) and accumulating its
+// main-numbering rank (width W, every prefix position shifted up by one) needs
+// one 16-byte (or 32-byte) table read per position.  With 32-bit Idx the sums
+// are taken mod 2^32, exactly as a sum of the two truncated counts would be.
 template <typename Idx>
 struct alignas(4 * sizeof(Idx)) Quad {
-    Idx zn, zr, mn, mr;
+    Idx zn, zc, mn, mc;
 };
 
 // Whole-entry table reads (16 or 32 bytes at once).  Left to itself the
@@ -52,10 +56,10 @@ OFP_HD Quad<Idx> ldq(const Quad<Idx>& e) {
     return e;
 #endif
 }
-// The first half only (zn, zr): what the closure scans need.
+// The first half only (zn, zc): what the closure scans need.
 template <typename Idx>
 struct Z2 {
-    Idx zn, zr;
+    Idx zn, zc;
 };
 template <typename Idx>
 OFP_HD Z2<Idx> ldz(const Quad<Idx>& e) {
@@ -68,38 +72,43 @@ OFP_HD Z2<Idx> ldz(const Quad<Idx>& e) {
         return Z2<Idx>{(Idx)v.x, (Idx)v.y};
     }
 #else
-    return Z2<Idx>{e.zn, e.zr};
+    return Z2<Idx>{e.zn, e.zc};
 #endif
+}
+// Rank contribution of value v (0=N, 1=R, 2=L) from a table entry's pair.
+template <typename Idx>
+OFP_HD Idx pick(int v, Idx n, Idx c) {
+    return v == 0 ? Idx(0) : (v == 1 ? n : c);
 }
 // One position of a rank walk without branches: v = number of completion
 // blocks (N, then R) that r passes; contrib = their size; r -= contrib.
-// zn + zr never wraps: it is at most D[pos+1][h], a word count of the state.
+// zc = zn + D[pos][h-1] never wraps: it is at most D[pos+1][h], a word count
+// of the state.
 template <typename Idx>
-OFP_HD int step_z(Idx zn, Idx zr, Idx& r, Idx& contrib) {
-    const Idx b = zn + zr;
-    const int v = (r >= zn) + (r >= b);
-    contrib = v == 0 ? Idx(0) : (v == 1 ? zn : b);
+OFP_HD int step_z(Idx zn, Idx zc, Idx& r, Idx& contrib) {
+    const bool g0 = r >= zn, g1 = r >= zc;
+    contrib = g1 ? zc : (g0 ? zn : Idx(0));
     r -= contrib;
-    return v;
+    return (int)g0 + (int)g1;
 }
 
 template <typename Idx>
 struct Tab {
     const Quad<Idx> (*q)[kDpCols];
     OFP_HD Idx at(int pos, int h) const { return q[pos][h].zn; }
-    OFP_HD Idx cL(int pos, int h) const { return q[pos][h].zn + q[pos][h].zr; }
+    OFP_HD Idx cL(int pos, int h) const { return q[pos][h].zc; }
     // rank contribution of value v (0=N,1=R,2=L) at position pos entered at height h
     OFP_HD Idx c(int pos, int v, int h) const {
         const Quad<Idx> e = q[pos][h];
-        return v == 0 ? Idx(0) : (v == 1 ? e.zn : Idx(e.zn + e.zr));
+        return pick(v, e.zn, e.zc);
     }
     // decode one position: consume r, return the value and its contribution
     OFP_HD int step(int pos, int h, Idx& r, Idx& contrib) const {
         const Z2<Idx> e = ldz(q[pos][h]);
-        return step_z(e.zn, e.zr, r, contrib);
+        return step_z(e.zn, e.zc, r, contrib);
     }
     OFP_HD static int step_e(const Quad<Idx>& e, Idx& r, Idx& contrib) {
-        return step_z(e.zn, e.zr, r, contrib);
+        return step_z(e.zn, e.zc, r, contrib);
     }
 };
 
@@ -296,11 +305,11 @@ OFP_HD void gather_closures(const Tab<Idx>& T, int W, int p, int hP, Idx mainbas
                 const Z2<Idx> e = ldz(T.q[pos][hh]);
                 const Z2<Idx> e2 = ldz(T.q[pos][hh + 2]);
                 Idx c0;
-                const int v = step_z(e.zn, e.zr, rr, c0);
+                const int v = step_z(e.zn, e.zc, rr, c0);
                 if (v == 2 && depth == 1)
                     g.add(m, (uint64_t)(baseLL + s2 + e2.zn + (sr - s0 - c0)), mod);
                 s0 += c0;
-                s2 += (v == 0) ? Idx(0) : (v == 1 ? e2.zn : Idx(e2.zn + e2.zr));
+                s2 += pick(v, e2.zn, e2.zc);
                 if (v == 2) ++depth;
                 else if (v == 1) { if (--depth == 0) break; }
                 hh += (v == 2) - (v == 1);
@@ -320,11 +329,11 @@ OFP_HD void gather_closures(const Tab<Idx>& T, int W, int p, int hP, Idx mainbas
             const int H = Ha + (v == 1) - (v == 2);
             const Z2<Idx> e0 = ldz(T.q[mp][H]);
             const Z2<Idx> e2 = ldz(T.q[mp][H + 2]);
-            const Idx c0 = (v == 0) ? Idx(0) : (v == 1 ? e0.zn : Idx(e0.zn + e0.zr));
+            const Idx c0 = pick(v, e0.zn, e0.zc);
             if (v == 1 && depth == 1)
-                g.add(m, (uint64_t)(mainbase - (u0 + c0) + e0.zn + e0.zr + u2 + tail), mod);
+                g.add(m, (uint64_t)(mainbase - (u0 + c0) + e0.zc + u2 + tail), mod);
             u0 += c0;
-            u2 += (v == 0) ? Idx(0) : (v == 1 ? e2.zn : Idx(e2.zn + e2.zr));
+            u2 += pick(v, e2.zn, e2.zc);
             if (v == 2) { if (--depth == 0) break; }
             else if (v == 1) ++depth;
             Ha = H;
@@ -360,11 +369,13 @@ OFP_HD Decoded<Idx> decode_blocked(const Tab<Idx>& T, int W, int p, uint64_t bid
     uint64_t word = 0;  // built top-down, then moved to bits 2*pos
     for (int pos = W - 2; pos >= p; --pos) {
         const Quad<Idx> e = ldq(T.q[pos][h]);
-        Idx cb;
-        const int v = step_z(e.zn, e.zr, r, cb);
-        mainbase += (v == 0) ? Idx(0) : (v == 1 ? e.mn : Idx(e.mn + e.mr));
-        word = (word << 2) | (uint64_t)v;
-        h += (v == 2) - (v == 1);
+        // the two comparisons select everything: v = g0 + g1, and R (g0 only)
+        // lowers the height by one, L (both) raises it by one
+        const bool g0 = r >= e.zn, g1 = r >= e.zc;
+        r -= g1 ? e.zc : (g0 ? e.zn : Idx(0));
+        mainbase += g1 ? e.mc : (g0 ? e.mn : Idx(0));
+        word = (word << 2) | (uint64_t)((int)g0 + (int)g1);
+        h += g1 ? 1 : (g0 ? -1 : 0);
     }
     d.hP = h;
     Idx cx;
@@ -376,10 +387,11 @@ OFP_HD Decoded<Idx> decode_blocked(const Tab<Idx>& T, int W, int p, uint64_t bid
 }
 
 // Capped completions for the words whose heights stay <= a cap c: at position
-// pos entered at height h, n = Dc[pos][h] (choose N), r = Dc[pos][h-1] (choose R).
+// pos entered at height h, n = Dc[pos][h] (after choosing N), c = Dc[pos][h] +
+// Dc[pos][h-1] (after choosing N or R), Dc[.][-1] = 0.
 template <typename Idx>
 struct alignas(2 * sizeof(Idx)) Pair {
-    Idx n, r;
+    Idx n, c;
 };
 
 // Decode the t-th (in rank order) blocked word whose heights stay <= the cap
@@ -397,10 +409,10 @@ OFP_HD Decoded<Idx> decode_capped(const Tab<Idx>& T, const Pair<Idx> (*C)[kDpCol
         const Pair<Idx> pc = C[pos][h];
         const Quad<Idx> e = ldq(T.q[pos][h]);
         Idx skip;
-        const int v = step_z(pc.n, pc.r, r, skip);
-        const Idx cb = (v == 0) ? Idx(0) : (v == 1 ? e.zn : Idx(e.zn + e.zr));
+        const int v = step_z(pc.n, pc.c, r, skip);
+        const Idx cb = pick(v, e.zn, e.zc);
         bidx += cb;
-        if (pos >= p) mainbase += (v == 0) ? Idx(0) : (v == 1 ? e.mn : Idx(e.mn + e.mr));
+        if (pos >= p) mainbase += pick(v, e.mn, e.mc);
         else if (pos == p - 1) { x = v; hP = h; }
         else sr += cb;
         word |= (uint64_t)v << (2 * pos);
@@ -588,7 +600,7 @@ inline void pack_pairs(const uint64_t (*dc)[kDpCols], Pair<Idx> (*q)[kDpCols]) {
     for (int pos = 0; pos < kDpRows; ++pos)
         for (int h = 0; h < kDpCols; ++h) {
             q[pos][h].n = (Idx)dc[pos][h];
-            q[pos][h].r = h > 0 ? (Idx)dc[pos][h - 1] : Idx(0);
+            q[pos][h].c = (Idx)(dc[pos][h] + (h > 0 ? dc[pos][h - 1] : 0));
         }
 }
 
@@ -598,10 +610,12 @@ inline void pack_quads(const uint64_t (*d)[kDpCols], Quad<Idx> (*q)[kDpCols]) {
     for (int pos = 0; pos < kDpRows; ++pos)
         for (int h = 0; h < kDpCols; ++h) {
             Quad<Idx> e;
+            const uint64_t up = pos + 1 < kDpRows ? d[pos + 1][h] : 0;
+            const uint64_t up1 = (pos + 1 < kDpRows && h > 0) ? d[pos + 1][h - 1] : 0;
             e.zn = (Idx)d[pos][h];
-            e.zr = h > 0 ? (Idx)d[pos][h - 1] : Idx(0);
-            e.mn = pos + 1 < kDpRows ? (Idx)d[pos + 1][h] : Idx(0);
-            e.mr = (pos + 1 < kDpRows && h > 0) ? (Idx)d[pos + 1][h - 1] : Idx(0);
+            e.zc = (Idx)(d[pos][h] + (h > 0 ? d[pos][h - 1] : 0));
+            e.mn = (Idx)up;
+            e.mc = (Idx)(up + up1);
             q[pos][h] = e;
         }
 }

@@ -2,6 +2,7 @@
 
   python -m ofp.rocprof_summary trace   <kernel_trace.csv>          [--cells C]
   python -m ofp.rocprof_summary counter <counter_collection.csv>    [--cells C] [--scale F]
+  python -m ofp.rocprof_summary valu    <counter_collection.csv>    --cells C [--cus N] [--xcds X]
 
 `trace` prints per-kernel call counts and mean durations.  `counter` prints the
 mean counter value per kernel (KiB per dispatch for FETCH_SIZE / WRITE_SIZE)
@@ -15,6 +16,18 @@ launches: `trace` lists each row's time and how many of its cells ran capped,
 then the per-cell times (p = W-1 .. 1) of the median full-state row; `counter`
 gives the mean over full-state rows' dispatches and the per-cell values of the
 last full-state row.
+
+`valu` reads one run collected with
+  --pmc SQ_ACTIVE_INST_VALU SQ_THREAD_CYCLES_VALU SQ_INSTS_VALU SQ_INSTS_SALU
+        SQ_INSTS_LDS SQ_INSTS_VMEM_RD SQ_INSTS_VMEM_WR SQ_WAVES GRBM_GUI_ACTIVE GRBM_COUNT
+and prints, per cell of the last full-state row, the integer-VALU figures:
+  busy  = SQ_ACTIVE_INST_VALU / CUs / (GRBM_GUI_ACTIVE / XCDs): the share of the
+          CUs' VALU issue cycles used (one wave64 VALU instruction per CU per cycle;
+          GRBM_GUI_ACTIVE is summed over the XCDs);
+  lanes = SQ_THREAD_CYCLES_VALU / (SQ_ACTIVE_INST_VALU * 64): active lanes per
+          VALU instruction (divergence);
+and VALU / SALU / LDS / VMEM instruction counts per 64 blocked words
+(SQ_WAVES counts the persistent waves, so per-word figures use --words).
 """
 from __future__ import annotations
 
@@ -105,17 +118,65 @@ def counter(path: str, scale: float, cells: int | None) -> None:
               + " ".join(f"{x[1] * 1024 * scale / 1e9:.1f}" for x in full[-1]))
 
 
+VALU_COUNTERS = ("SQ_ACTIVE_INST_VALU", "SQ_THREAD_CYCLES_VALU", "SQ_INSTS_VALU", "SQ_INSTS_SALU",
+                 "SQ_INSTS_LDS", "SQ_INSTS_VMEM_RD", "SQ_INSTS_VMEM_WR", "SQ_WAVES", "GRBM_GUI_ACTIVE")
+
+
+def valu(path: str, cells: int, cus: int, xcds: int, words: int | None) -> None:
+    by = collections.defaultdict(dict)
+    meta = {}
+    for r in csv.DictReader(open(path)):
+        d = int(r["Dispatch_Id"])
+        by[d][r["Counter_Name"]] = float(r["Counter_Value"])
+        meta[d] = (_short(r["Kernel_Name"]), int(r["Start_Timestamp"]), int(r["End_Timestamp"]))
+    fam = collections.defaultdict(list)
+    for d, (name, t0, t1) in meta.items():
+        kc = _kc(name)
+        if kc is not None and all(c in by[d] for c in VALU_COUNTERS):
+            fam[kc].append((d, d, "capped" in name))
+    for kc, seq in sorted(fam.items(), reverse=True):
+        full = [r for r in _rows(seq, cells) if not any(x[2] for x in r)]
+        if not full:
+            continue
+        row = full[-1]
+        print(f"KC={kc}: last full-state row ({len(full)} full-state rows); per cell p={cells}..1")
+        print(f"{'p':>3s} {'ms':>7s} {'busy%':>6s} {'lanes%':>6s} {'VALU/64w':>9s} {'SALU/64w':>9s} "
+              f"{'LDS/64w':>8s} {'VMRD/64w':>9s} {'VMWR/64w':>9s}")
+        tot_valu = tot_cyc = 0.0
+        for i, x in enumerate(row):
+            d = x[0]
+            b = by[d]
+            _, t0, t1 = meta[d]
+            cyc = b["GRBM_GUI_ACTIVE"] / xcds
+            busy = 100.0 * b["SQ_ACTIVE_INST_VALU"] / cus / cyc
+            lanes = 100.0 * b["SQ_THREAD_CYCLES_VALU"] / (b["SQ_ACTIVE_INST_VALU"] * 64)
+            w64 = (words / 64.0) if words else b["SQ_WAVES"]
+            tot_valu += b["SQ_ACTIVE_INST_VALU"]
+            tot_cyc += cyc
+            print(f"{cells - i:3d} {(t1 - t0) / 1e6:7.2f} {busy:6.1f} {lanes:6.1f} "
+                  f"{b['SQ_INSTS_VALU'] / w64:9.0f} {b['SQ_INSTS_SALU'] / w64:9.0f} {b['SQ_INSTS_LDS'] / w64:8.1f} "
+                  f"{b['SQ_INSTS_VMEM_RD'] / w64:9.1f} {b['SQ_INSTS_VMEM_WR'] / w64:9.1f}")
+        print(f"  row: VALU busy {100.0 * tot_valu / cus / tot_cyc:.1f}% of the CUs' VALU issue cycles")
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["trace", "counter"])
+    ap.add_argument("mode", choices=["trace", "counter", "valu"])
     ap.add_argument("csv")
     ap.add_argument("--cells", type=int, default=None, help="cells per row (W-1) for the per-row listing")
     ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--cus", type=int, default=256, help="compute units (MI355X: 256)")
+    ap.add_argument("--xcds", type=int, default=8, help="XCDs GRBM_GUI_ACTIVE is summed over (MI355X: 8)")
+    ap.add_argument("--words", type=int, default=None, help="blocked words per launch (per-64-word counts)")
     a = ap.parse_args(argv)
     if a.mode == "trace":
         trace(a.csv, a.cells)
-    else:
+    elif a.mode == "counter":
         counter(a.csv, a.scale, a.cells)
+    else:
+        if not a.cells:
+            ap.error("valu needs --cells")
+        valu(a.csv, a.cells, a.cus, a.xcds, a.words)
     return 0
 
 
