@@ -297,45 +297,49 @@ OFP_HD void gather_closures(const Tab<Idx>& T, int W, int p, int hP, Idx mainbas
         }
         // LL predecessors: [P,L,L,S*], S* = S with a top-level L at q turned into R,
         // entered at height hP+2 (reverse_block_predecessors, gridfp_reverse.hpp:62-68)
+        // The scan walks S downwards decoding it from its rank rr; an L opens and
+        // an R closes one level, so the bracket depth below x is 1 + hh - hP and
+        // the top level is hh == hP.  After a step rr is the rank of the rest of
+        // S, which S* keeps; s2 is the rank of S*'s part above q (entered two
+        // levels higher).
         if (p >= 3) {
             const Idx baseLL = mainbase + T.cL(p, hP) + T.cL(p - 1, hP + 1);
-            Idx rr = sr, s0 = 0, s2 = 0;
-            int hh = hP, depth = 1;
+            Idx rr = sr, s2 = 0;
+            int hh = hP;
             for (int pos = p - 2; pos >= 0; --pos) {
                 const Z2<Idx> e = ldz(T.q[pos][hh]);
                 const Z2<Idx> e2 = ldz(T.q[pos][hh + 2]);
-                Idx c0;
-                const int v = step_z(e.zn, e.zc, rr, c0);
-                if (v == 2 && depth == 1)
-                    g.add(m, (uint64_t)(baseLL + s2 + e2.zn + (sr - s0 - c0)), mod);
-                s0 += c0;
-                s2 += pick(v, e2.zn, e2.zc);
-                if (v == 2) ++depth;
-                else if (v == 1) { if (--depth == 0) break; }
-                hh += (v == 2) - (v == 1);
+                const bool g0 = rr >= e.zn, g1 = rr >= e.zc;  // L: g1; R: g0 and not g1
+                rr -= g1 ? e.zc : (g0 ? e.zn : Idx(0));
+                if (g1 && hh == hP) g.add(m, (uint64_t)(baseLL + s2 + e2.zn + rr), mod);
+                if (g0 && !g1 && hh == hP) break;
+                s2 += g1 ? e2.zc : (g0 ? e2.zn : Idx(0));
+                hh += g1 ? 1 : (g0 ? -1 : 0);
             }
         }
     }
     // RR predecessors: [P*,R,R,S], P* = P with a top-level R at q turned into L
     // (gridfp_reverse.hpp:70-82); at p=1 they are the closure sources of NN.
+    // The scan walks P upwards from its lowest position; an R opens and an L
+    // closes one level, so the depth is 1 + Ha - hP and the top level Ha == hP.
+    // ub is the main-numbering base without the part of P scanned so far, u2 the
+    // rank of that part in P* (entered two levels higher below q).
     {
-        const Idx tail = T.at(p, hP + 2) + T.at(p - 1, hP + 1) + sr;
-        Idx u0 = 0, u2 = 0;
-        int depth = 1, Ha = hP;
+        Idx ub = mainbase + T.at(p, hP + 2) + T.at(p - 1, hP + 1) + sr;
+        Idx u2 = 0;
+        int Ha = hP;
         uint64_t w = word >> (2 * p);  // P from its lowest position up
         for (int mp = p + 1; mp <= W - 1; ++mp) {
-            const int v = (int)(w & 3u);
+            const uint32_t v = (uint32_t)w & 3u;
             w >>= 2;
-            const int H = Ha + (v == 1) - (v == 2);
+            const bool isR = v == 1, isL = v == 2;
+            const int H = Ha + (isR ? 1 : (isL ? -1 : 0));
             const Z2<Idx> e0 = ldz(T.q[mp][H]);
             const Z2<Idx> e2 = ldz(T.q[mp][H + 2]);
-            const Idx c0 = pick(v, e0.zn, e0.zc);
-            if (v == 1 && depth == 1)
-                g.add(m, (uint64_t)(mainbase - (u0 + c0) + e0.zc + u2 + tail), mod);
-            u0 += c0;
-            u2 += pick(v, e2.zn, e2.zc);
-            if (v == 2) { if (--depth == 0) break; }
-            else if (v == 1) ++depth;
+            ub -= isL ? e0.zc : (isR ? e0.zn : Idx(0));
+            if (isR && Ha == hP) g.add(m, (uint64_t)(ub + e0.zc + u2), mod);
+            if (isL && Ha == hP) break;
+            u2 += isL ? e2.zc : (isR ? e2.zn : Idx(0));
             Ha = H;
         }
     }
