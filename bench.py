@@ -51,6 +51,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def _cpu_model() -> str | None:
+    """The host CPU's model name (first 'model name' line of /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(n: int, k: int, seconds: float) -> dict:
     """Time the CPU oracle's pull-form cell step on a bounded sample of the
     n-sized state (consecutive target windows of a middle cell), at one
@@ -98,6 +110,7 @@ def cpu_baseline(n: int, k: int, seconds: float) -> dict:
     cores = int(lib.or_num_threads())
     rate = done / el if el > 0 else None
     return {"value": rate, "unit": "transitions/s", "cores": cores, "kind": "port",
+            "cpu_model": _cpu_model(),
             "sample": (f"oracle pull-form step (oracle/gridfp_oracle.c or_cell_pull_range), n={n} cell "
                        f"p={p}, {windows} windows of {win_m} main + {win_b} blocked target states from "
                        f"rank 1/3 on, one modulus, {el:.1f} s on {cores} threads; per-residue CPU "

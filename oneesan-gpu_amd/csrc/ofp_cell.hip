@@ -30,7 +30,8 @@
 // O(1) beyond the scan.
 //
 // Residues: uint32 mod p, additions only (every transfer coefficient is 1);
-// a + b mod p is computed with one compare-and-subtract.
+// a + b mod p is one conditional subtract, written as a subtract with borrow
+// and a select (addm in ofp_cell_impl.h).
 #include <cstring>
 #include <vector>
 
