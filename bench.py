@@ -24,8 +24,12 @@ process:
     (8 B per state per modulus: read + write of one uint32 residue) divided by
     the average duration of the full-state launches (rows without capped
     cells), from HIP events on the library's stream;
+  * plain_small: whole one-modulus solves at n=12 and n=18 (the small plain-form
+    configurations; median of three, the n=18 residue checked against the
+    reference's published value);
   * cpu_baseline: the CPU oracle's pull-form step timed on the host cores on
-    a bounded sample of the same n=22 cell step.
+    a bounded sample of the same n=22 cell step, and its whole n=12 solve
+    (median of three, residue compared with the GPU's).
 
 Prints ONE JSON line on stdout.
 """
@@ -118,6 +122,22 @@ def cpu_baseline(n: int, k: int, seconds: float) -> dict:
             "residues_per_hour_extrapolated": (3600.0 * rate / (W * (W - 1) * (nm + nb))) if rate else None}
 
 
+def cpu_solve_n12(mod: int) -> dict:
+    """The CPU oracle's whole n=12 solve (pull form, one modulus): one warm-up,
+    then the median of three."""
+    from oracle import binding
+    binding.solve(12, mod)
+    times, res = [], None
+    for _ in range(3):
+        t0 = time.perf_counter()
+        res = binding.solve(12, mod)
+        times.append(time.perf_counter() - t0)
+    med = sorted(times)[1]
+    T = 13 * 12 * 98123
+    return {"solve_s": med, "residue": res, "transitions_per_s": T / med,
+            "sample": "oracle whole n=12 solve (oracle/gridfp_oracle.c or_solve, pull form), median of 3"}
+
+
 def copy_calibration(nbytes: int) -> int:
     """One device-to-device copy of nbytes (HIP runtime blit kernel), so the
     rocprofv3 FETCH_SIZE / WRITE_SIZE counters can be scaled: a copy of B bytes
@@ -153,6 +173,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-shard", action="store_true", help="skip the n=27 one-GPU-share measurement")
+    ap.add_argument("--no-small", action="store_true",
+                    help="skip the whole plain-form solves at n=12 and n=18 (k=1)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -248,6 +270,31 @@ def main() -> int:
             f"(residue {pres[0]})")
     ctx.close()
 
+    # whole plain-form solves of the small configurations (n=12, n=18, one modulus):
+    # one untimed solve (graph capture), then the median of three
+    small = None
+    if not args.no_small:
+        small = {}
+        for ns in (12, 18):
+            with ofp.Context(device=0) as c:
+                autotune.apply(c, ns, 1)
+                c.solve(ns, [PRIMES[0]])
+                times, res = [], None
+                for _ in range(3):
+                    t0s = time.perf_counter()
+                    r, _ = c.solve(ns, [PRIMES[0]])
+                    times.append(time.perf_counter() - t0s)
+                    res = r[0]
+            med = sorted(times)[1]
+            Ws = ns + 1
+            Ss = sum(ofp.state_counts(ns))
+            want = GOLDEN_FIRST.get(ns)
+            small[f"n{ns}"] = {"solve_s": med, "residue": res, "modulus": PRIMES[0],
+                               "verified": (res == want[1]) if want and want[0] == PRIMES[0] else None,
+                               "transitions_per_s": Ws * (Ws - 1) * Ss / med,
+                               "residues_per_hour": 3600.0 / med}
+            log(f"bench: plain n={ns} solve {med * 1e3:.1f} ms, residue {res}")
+
     shard = None
     if not args.no_shard:
         from ofp import shard as shard_mod
@@ -261,6 +308,9 @@ def main() -> int:
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(n, k, args.cpu_seconds)
         log(f"bench: cpu baseline {cpu['value']:.3e} transitions/s on {cpu['cores']} threads")
+        cpu["n12"] = cpu_solve_n12(PRIMES[0])
+        if small:
+            cpu["n12"]["residue_matches_gpu"] = cpu["n12"]["residue"] == small["n12"]["residue"]
 
     out = {
         "metric": "frontier transitions/s & residues/hour, batched vs 1-residue, n=22, bit-exact",
@@ -293,6 +343,7 @@ def main() -> int:
                      "avg_launch_ms": avg_launch_s * 1e3, "launch_basis": launch_basis,
                      "capped_kernel_s": st["capped_kernel_s"], "capped_launches": st["capped_launches"]},
         "cpu_baseline": cpu,
+        "plain_small": small,
         "shard_n27": shard,
     }
     print(json.dumps(out), flush=True)
