@@ -1,6 +1,7 @@
 # Builds checkers from the reference's own sources, where they lie, into
-# oracle/_ref/ (git-ignored; travels to the GPU box with the tree).  Nothing
-# is copied from the reference.  Needs /root/reference (this container only).
+# oracle/_ref/ (git-ignored, but kept with the working tree so the built
+# checkers are available wherever the tree is copied).  Nothing is copied from
+# the reference.  Needs the reference tree at /root/reference.
 #   _ref/libref_gridfp.so : the reference's src/common/gridfp_transition.hpp and
 #                           gridfp_reverse.hpp behind ref_gridfp_harness.cpp
 #   _ref/ggcount_single   : third_party/ggcount/src/ggcount.cpp (MIT), an
