@@ -183,7 +183,7 @@ def main() -> int:
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(json.dumps({"error": "this build measures one GPU: the frontier state of one solve "
+        print(json.dumps({"error": "this benchmark measures one GPU: the frontier state of one solve "
                                    "lives in one GPU's HBM; run with --gpus 1"}), flush=True)
         return 2
 

@@ -10,7 +10,7 @@ auxiliary dependencies (role, path, SHA-256, size), the compiler and its
 version, the compile arguments and the git commit; a SHA-256 over the
 canonical JSON of all of that guards the file.  The layout is the reference's
 ONEESAN_BUILD_PROVENANCE_V2 (scripts/solve/build_provenance.py:15-18,141-190),
-so the reference's loader reads it; this build's solver is one binary for all
+so the reference's loader reads it; this solver is one binary for all
 n, so it carries no -DTARGET_W argument (the reference's verifier asks for one,
 scripts/tools/verify_exact_result.py:358).
 """

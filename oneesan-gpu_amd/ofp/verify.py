@@ -223,7 +223,7 @@ def verify(manifest: Path, binary: Path | None = None, verify_sources: bool = Fa
     adm = [data["admission_certificate_file"], data["admission_certificate_sha256"],
            data["admission_certificate_schema"]]
     if any(v is not None for v in adm):
-        raise ValueError("admission certificates are not produced by this build")
+        raise ValueError("admission certificates are not produced by this solver")
 
     ck = manifest.parent / _basename(data, "checkpoint_file")
     if not ck.is_file():
