@@ -1,0 +1,70 @@
+"""The rocprofv3 summariser on synthetic CSVs (no GPU): rows of cells, full-state
+rows, calibrated bytes and the VALU figures are computed as documented."""
+from __future__ import annotations
+
+import csv
+
+from ofp import rocprof_summary as rs
+
+CELL = "void ofp::(anonymous namespace)::cell_kernel<14, unsigned int, 1, 2>(ofp::CellLaunch)"
+CAPPED = "void ofp::(anonymous namespace)::cell_capped_kernel<14, unsigned int>(ofp::CellLaunch)"
+
+
+def _trace(path, launches):
+    with open(path, "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=["Kernel_Name", "Start_Timestamp", "End_Timestamp"])
+        w.writeheader()
+        t = 0
+        for name, ms in launches:
+            w.writerow({"Kernel_Name": name, "Start_Timestamp": t, "End_Timestamp": t + int(ms * 1e6)})
+            t += int(ms * 1e6) + 1000
+
+
+def test_trace_rows_and_full_state_mean(tmp_path, capsys):
+    cells = 3
+    # row 0 capped, rows 1 and 2 full-state
+    launches = [(CAPPED, 1.0)] * cells + [(CELL, 10.0), (CELL, 11.0), (CELL, 12.0)] + [(CELL, 20.0)] * cells
+    p = tmp_path / "t.csv"
+    _trace(p, launches)
+    rs.trace(str(p), cells)
+    out = capsys.readouterr().out
+    assert "KC=14: 3 rows of 3 cells" in out
+    assert "3.0(3) 33.0(0) 60.0(0)" in out
+    assert "mean full-state launch 15.500 ms over 6 launches" in out  # (10+11+12+3x20)/6
+
+
+def test_counter_calibration(tmp_path, capsys):
+    p = tmp_path / "c.csv"
+    with open(p, "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=["Kernel_Name", "Counter_Name", "Counter_Value", "Dispatch_Id"])
+        w.writeheader()
+        for d in range(4):
+            w.writerow({"Kernel_Name": CELL, "Counter_Name": "FETCH_SIZE", "Counter_Value": 1000.0 * (d + 1),
+                        "Dispatch_Id": d})
+    rs.counter(str(p), 2.0, 2)
+    out = capsys.readouterr().out
+    # mean of 1000..4000 KiB = 2500 KiB, x1024 x2.0 bytes
+    assert f"calibrated_GB={2500 * 1024 * 2.0 / 1e9:10.3f}" in out
+    assert "KC=14: full-state rows 2" in out
+
+
+def test_valu_busy_and_lanes(tmp_path, capsys):
+    p = tmp_path / "v.csv"
+    cells = 2
+    fields = ["Kernel_Name", "Counter_Name", "Counter_Value", "Dispatch_Id", "Start_Timestamp", "End_Timestamp"]
+    with open(p, "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=fields)
+        w.writeheader()
+        for d in range(cells):
+            vals = {"SQ_ACTIVE_INST_VALU": 256 * 1000.0, "SQ_THREAD_CYCLES_VALU": 256 * 1000.0 * 32,
+                    "SQ_INSTS_VALU": 256 * 1000.0, "SQ_INSTS_SALU": 0.0, "SQ_INSTS_LDS": 0.0,
+                    "SQ_INSTS_VMEM_RD": 0.0, "SQ_INSTS_VMEM_WR": 0.0, "SQ_WAVES": 64.0,
+                    "GRBM_GUI_ACTIVE": 8 * 4000.0}
+            for name, v in vals.items():
+                w.writerow({"Kernel_Name": CELL, "Counter_Name": name, "Counter_Value": v, "Dispatch_Id": d,
+                            "Start_Timestamp": 0, "End_Timestamp": 1000000})
+    rs.valu(str(p), cells, 256, 8, None)
+    out = capsys.readouterr().out
+    # 1000 VALU issue cycles per CU out of 4000 per XCD: 25 %; half the lanes active: 50 %
+    assert "  25.0   50.0" in out
+    assert "row: VALU busy 25.0%" in out
