@@ -120,7 +120,9 @@ ofp_status ofp_set_launch(ofp_ctx* ctx, int threads, int blocks_per_cu);
  * registers live during the scans), 2 = one branch-free orbit path for every
  * kind of blocked word (all lanes' loads issued before the scans; the
  * default), 3 = as 2 with the orbit results stored before the scans (fewer live
- * registers).  Set by the autotuner; no reference counterpart. */
+ * registers), 4 = as 2 with the sorted assignment on every cell (see
+ * ofp_set_sorted_cells; shard sweeps run 2).  Set by the autotuner; no
+ * reference counterpart. */
 ofp_status ofp_set_variant(ofp_ctx* ctx, int variant);
 
 /* Work distribution over the 8 XCDs: the blocked words are cut into chunks of
@@ -128,6 +130,13 @@ ofp_status ofp_set_variant(ofp_ctx* ctx, int variant);
  * round-robin (default 5); -1 gives each XCD one contiguous eighth.  Set by the
  * autotuner; no reference counterpart. */
 ofp_status ofp_set_xcd_pieces(ofp_ctx* ctx, int log2_chunks);
+
+/* Full-state cells p >= from_cell (state-major layout) run the sorted kernel
+ * (variant 4): each workgroup decodes its 256 blocked words, then deals them
+ * back with the x=N words first, so that a wave holds one kind of word and its
+ * orbit loads and stores are contiguous.  0 (default) = never.  Results do not
+ * depend on it.  Set by the autotuner; no reference counterpart. */
+ofp_status ofp_set_sorted_cells(ofp_ctx* ctx, int from_cell);
 
 /* Height caps.  In a sweep from the initial state (ofp_solve, ofp_prepare +
  * ofp_row_sweep, jobs) the cells of the first rows only touch words of small

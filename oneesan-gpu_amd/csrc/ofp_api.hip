@@ -63,6 +63,7 @@ struct ofp_ctx {
     void* dpc64 = nullptr;
     int cap_percent = 85;  // height caps: cap a cell when it then sweeps <= this % of the blocked words
     int xcd_shift = 5;     // chunk pieces per XCD: 2^xcd_shift chunks, < 0 = one eighth each
+    int sorted_from = 0;   // full-state cells p >= sorted_from run the sorted kernel (0: none)
     std::string err;
     Sweep st;              // the context's own state (solve / prepare / row_sweep)
     int jobs = 0;          // live jobs (serving)
@@ -276,7 +277,8 @@ ofp_status enqueue_cells(ofp_ctx* c, Sweep& s, int p_hi, int p_lo, bool first_re
             a.nblk = s.nblk;
             a.first = 0;
             a.layout = s.layout;
-            a.variant = c->variant;
+            // the sorted assignment (variant 4) on the cells from sorted_from up
+            a.variant = (c->sorted_from > 0 && p >= c->sorted_from && s.layout == 1) ? 4 : c->variant;
             a.xcd_shift = c->xcd_shift;
             a.W = s.W;
             a.p = p;
@@ -314,7 +316,8 @@ ofp_status full_rows(ofp_ctx* c, Sweep& s, int rows, int* launches) {
     }
     char key[256];
     snprintf(key, sizeof key, "%p/%p/%d/%d/%d/%d/%d/%d/%d/%d/%d", (void*)s.main, (void*)s.blk, s.n, s.K,
-             c->threads, c->blocks_per_cu, s.layout, c->low_cells, c->low_kc, c->variant, c->xcd_shift);
+             c->threads, c->blocks_per_cu, s.layout, c->low_cells, c->low_kc, c->variant * 100 + c->sorted_from,
+             c->xcd_shift);
     std::string k = key;
     for (uint32_t m : s.mods) k += "/" + std::to_string(m);
     if (k != s.row_key || !s.row_exec) {
@@ -591,7 +594,7 @@ ofp_status ofp_set_launch(ofp_ctx* c, int threads, int blocks_per_cu) {
 
 ofp_status ofp_set_variant(ofp_ctx* c, int variant) {
     if (!c) return OFP_E_ARG;
-    if (variant < 0 || variant > 3) return fail(c, OFP_E_ARG, "variant must be 0..3");
+    if (variant < 0 || variant > 4) return fail(c, OFP_E_ARG, "variant must be 0..4");
     if (variant != c->variant) drop_graph(c->st);
     c->variant = variant;
     return OFP_OK;
@@ -602,6 +605,14 @@ ofp_status ofp_set_xcd_pieces(ofp_ctx* c, int log2_chunks) {
     if (log2_chunks < -1 || log2_chunks > 24) return fail(c, OFP_E_ARG, "log2_chunks must be -1..24");
     if (log2_chunks != c->xcd_shift) drop_graph(c->st);
     c->xcd_shift = log2_chunks;
+    return OFP_OK;
+}
+
+ofp_status ofp_set_sorted_cells(ofp_ctx* c, int from_cell) {
+    if (!c) return OFP_E_ARG;
+    if (from_cell < 0 || from_cell > kMaxW) return fail(c, OFP_E_ARG, "from_cell must be 0..28");
+    if (from_cell != c->sorted_from) drop_graph(c->st);
+    c->sorted_from = from_cell;
     return OFP_OK;
 }
 

@@ -66,6 +66,58 @@ __global__ __launch_bounds__(256) void cell_kernel(CellLaunch a) {
     }
 }
 
+// Sorted variant (state-major, uniform orbit body): the 256 words of a chunk are
+// decoded, then dealt back to the workgroup's threads with the x=N words first,
+// so the closure scans (x=N only) fill whole waves and the R/L words finish
+// together in the other waves.  Each word still has exactly one owner, so the
+// in-place argument is unchanged.  Three workgroup barriers per chunk.
+template <int KC, typename Idx>
+__global__ __launch_bounds__(256) void cell_sorted_kernel(CellLaunch a) {
+    __shared__ Quad<Idx> tab[kDpRows][kDpCols];
+    __shared__ Decoded<Idx> slot[256];
+    __shared__ uint32_t cnt[2][4];
+    const Quad<Idx>* g = (const Quad<Idx>*)a.dp;
+    for (int t = threadIdx.x; t < kDpRows * kDpCols; t += blockDim.x) (&tab[0][0])[t] = g[t];
+    __syncthreads();
+    Tab<Idx> T{tab};
+    const Direct<KC, kAoS, Idx> mem{a.main, a.blk, a.K, a.off};
+    const Sched sc(a.nblk, blockDim.x, gridDim.x, blockIdx.x, a.xcd_shift);
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const uint64_t below = lane ? (~0ull >> (64u - lane)) : 0ull;  // lanes under this one
+    for (uint64_t t = sc.slot; t < sc.count; t += sc.nx) {
+        const uint64_t b = sc.chunk(t) * blockDim.x + threadIdx.x;
+        Decoded<Idx> d{};
+        int key = 2;  // 0: x = N, 1: x = R or L, 2: past the end
+        if (b < a.nblk) {
+            d = decode_blocked<Idx>(T, a.W, a.p, a.first + b);
+            key = d.x == 0 ? 0 : 1;
+        }
+        const uint64_t m0 = __ballot(key == 0), m1 = __ballot(key == 1);
+        if (lane == 0) {
+            cnt[0][w] = (uint32_t)__popcll(m0);
+            cnt[1][w] = (uint32_t)__popcll(m1);
+        }
+        __syncthreads();
+        uint32_t o0 = 0, o1 = 0, n0 = 0, n1 = 0;
+        for (uint32_t i = 0; i < nw; ++i) {
+            if (i < w) {
+                o0 += cnt[0][i];
+                o1 += cnt[1][i];
+            }
+            n0 += cnt[0][i];
+            n1 += cnt[1][i];
+        }
+        if (key == 0) slot[o0 + (uint32_t)__popcll(m0 & below)] = d;
+        else if (key == 1) slot[n0 + o1 + (uint32_t)__popcll(m1 & below)] = d;
+        __syncthreads();
+        if (threadIdx.x < n0 + n1) {
+            const Decoded<Idx> e = slot[threadIdx.x];
+            cell_body<KC, Idx, kAoS, 2>(a, T, e, mem);
+        }
+        __syncthreads();  // slot and cnt are rewritten for the next chunk
+    }
+}
+
 // Shard variant: same per-thread body, windowed memory policy (state-major).
 template <int KC, typename Idx, int V>
 __global__ __launch_bounds__(256) void cell_shard_kernel(CellLaunch a) {
@@ -143,7 +195,8 @@ hipError_t launch_kc(const CellLaunch& a, int threads, int blocks, hipStream_t s
             hipLaunchKernelGGL((cell_kernel<KC, Idx, kAoS, 2>), dim3(blocks), dim3(threads), 0, s, a);
         else if (a.variant == 3)
             hipLaunchKernelGGL((cell_kernel<KC, Idx, kAoS, 3>), dim3(blocks), dim3(threads), 0, s, a);
-
+        else if (a.variant == 4)
+            hipLaunchKernelGGL((cell_sorted_kernel<KC, Idx>), dim3(blocks), dim3(threads), 0, s, a);
         else
             hipLaunchKernelGGL((cell_kernel<KC, Idx, kAoS, 0>), dim3(blocks), dim3(threads), 0, s, a);
     } else {
@@ -156,7 +209,7 @@ template <typename Idx, int KC>
 hipError_t launch_shard_kc(const CellLaunch& a, int threads, int blocks, hipStream_t s) {
     if (a.variant == 1)
         hipLaunchKernelGGL((cell_shard_kernel<KC, Idx, 1>), dim3(blocks), dim3(threads), 0, s, a);
-    else if (a.variant == 2)
+    else if (a.variant == 2 || a.variant == 4)  // no sorted shard kernel: the uniform body
         hipLaunchKernelGGL((cell_shard_kernel<KC, Idx, 2>), dim3(blocks), dim3(threads), 0, s, a);
     else if (a.variant == 3)
         hipLaunchKernelGGL((cell_shard_kernel<KC, Idx, 3>), dim3(blocks), dim3(threads), 0, s, a);

@@ -29,6 +29,7 @@ EXPORTS = (
     "ofp_last_error", "ofp_version", "ofp_destroy",
     "ofp_job_create", "ofp_job_enqueue", "ofp_job_info_get", "ofp_job_result", "ofp_job_destroy",
     "ofp_shard_plan", "ofp_shard_sweep", "ofp_set_height_caps", "ofp_height_cap", "ofp_set_xcd_pieces",
+    "ofp_set_sorted_cells",
 )
 
 
@@ -152,6 +153,7 @@ def load() -> ctypes.CDLL:
         "ofp_set_variant": (ctypes.c_int32, [vp, ctypes.c_int]),
         "ofp_set_height_caps": (ctypes.c_int32, [vp, ctypes.c_int]),
         "ofp_set_xcd_pieces": (ctypes.c_int32, [vp, ctypes.c_int]),
+        "ofp_set_sorted_cells": (ctypes.c_int32, [vp, ctypes.c_int]),
         "ofp_height_cap": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, u64p]),
         "ofp_state_counts": (ctypes.c_int32, [ctypes.c_int, u64p, u64p]),
         "ofp_last_error": (ctypes.c_char_p, [vp]),
@@ -331,13 +333,19 @@ class Context:
 
     def set_variant(self, variant: int) -> None:
         """Cell-kernel variant (state-major layout): 0 = early orbit loads, 1 = late,
-        2 = one branch-free orbit path, 3 = as 2 with early stores."""
+        2 = one branch-free orbit path, 3 = as 2 with early stores, 4 = as 2 with the
+        sorted assignment on every cell."""
         self._check(self._lib.ofp_set_variant(self._h, variant), "ofp_set_variant")
 
     def set_xcd_pieces(self, log2_chunks: int) -> None:
         """Deal the blocked-word chunks to the 8 XCDs in pieces of 2**log2_chunks
         chunks, round-robin (-1: one contiguous eighth per XCD)."""
         self._check(self._lib.ofp_set_xcd_pieces(self._h, log2_chunks), "ofp_set_xcd_pieces")
+
+    def set_sorted_cells(self, from_cell: int) -> None:
+        """Full-state cells p >= from_cell run the sorted kernel (x=N words dealt to
+        their own waves); 0 = never."""
+        self._check(self._lib.ofp_set_sorted_cells(self._h, from_cell), "ofp_set_sorted_cells")
 
     def set_height_caps(self, max_percent: int | bool) -> None:
         """Skip, in the first and last rows of a solve, the blocked words whose

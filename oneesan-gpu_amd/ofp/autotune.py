@@ -34,6 +34,8 @@ LOW_CELLS = ((0, 16), (6, 8), (4, 8))
 VARIANTS = (0, 1, 2, 3)
 # chunks are dealt to the 8 XCDs in pieces of 2**shift chunks (-1: one eighth each)
 XCD_SHIFTS = (-1, 0, 5, 8)
+# full-state cells p >= this run the sorted kernel (0: none)
+SORTED_FROM = (0, 4, 6, 8, 12)
 # third stage: cap a cell when it then sweeps at most this % of the blocked words;
 # timed on rows 1..CAPPED_ROWS of a solve (every row the forward bound caps at n <= 22)
 CAP_PERCENTS = (75, 85, 95)
@@ -82,6 +84,7 @@ def apply(ctx: "_lib.Context", n: int, k: int) -> dict | None:
         ctx.set_low_cells(int(ent.get("low_cells", 0)), int(ent.get("low_kc", 16)))
         ctx.set_height_caps(int(ent.get("cap_percent", _lib.DEFAULT_CAP_PERCENT)))
         ctx.set_xcd_pieces(int(ent.get("xcd_shift", 5)))
+        ctx.set_sorted_cells(int(ent.get("sorted_from", 0)))
     return ent
 
 
@@ -95,6 +98,7 @@ def tune(ctx: "_lib.Context", n: int, moduli: list[int], rows: int = 1,
     ctx.set_variant(2)
     ctx.set_low_cells(0, 16)
     ctx.set_xcd_pieces(5)
+    ctx.set_sorted_cells(0)
     for lay in layouts:
         ctx.set_layout(lay)
         ctx.prepare(n, moduli)
@@ -152,6 +156,22 @@ def tune(ctx: "_lib.Context", n: int, moduli: list[int], rows: int = 1,
         if row_ms < best["row_ms"]:
             best = ent
     ctx.set_xcd_pieces(best["xcd_shift"])
+    best = dict(best, sorted_from=0)
+    if best["layout"] == 1:
+        ctx.prepare(n, moduli)
+        for sf in SORTED_FROM:
+            ctx.set_sorted_cells(sf)
+            ctx.row_sweep(1)
+            t0 = time.perf_counter()
+            st = ctx.row_sweep(rows)
+            row_ms = (time.perf_counter() - t0) / rows * 1e3
+            ent = dict(best, sorted_from=sf, row_ms=row_ms, kernel_ms_per_row=st["kernel_s"] / rows * 1e3)
+            table.append(ent)
+            if log:
+                log(f"autotune n={n} k={len(moduli)} sorted_from={sf}: {row_ms:.2f} ms/row")
+            if sf == 0 or row_ms < best["row_ms"]:  # sf = 0 re-times the baseline on this state
+                best = ent
+    ctx.set_sorted_cells(best["sorted_from"])
     best = dict(best, cap_percent=_lib.DEFAULT_CAP_PERCENT)
     rows_capped = min(CAPPED_ROWS, n - 1)
     if best["layout"] == 1 and rows_capped >= 1:
@@ -178,6 +198,7 @@ def tune(ctx: "_lib.Context", n: int, moduli: list[int], rows: int = 1,
                                       "variant": best["variant"],
                                       "cap_percent": best["cap_percent"],
                                       "xcd_shift": best["xcd_shift"],
+                                      "sorted_from": best["sorted_from"],
                                       "row_ms": best["row_ms"], "tuned_at": time.time()}
         save_cache(data, path)
     return result

@@ -46,9 +46,9 @@ def test_every_cell_random_vectors(gpu_ctx, oracle, n):
                 assert np.array_equal(g_blk[:, j], eb), (n, p, mod, "blocked")
 
 
-@pytest.fixture(scope="module", params=[(0, 0), (1, 0), (1, 1), (1, 2), (1, 3)],
+@pytest.fixture(scope="module", params=[(0, 0), (1, 0), (1, 1), (1, 2), (1, 3), (1, 4)],
                 ids=["tiles64", "statemajor-early", "statemajor-late", "statemajor-uniform",
-                     "statemajor-early-retire"])
+                     "statemajor-early-retire", "statemajor-sorted"])
 def layout_ctx(request):
     import ofp
     layout, variant = request.param
@@ -162,6 +162,26 @@ def test_height_caps_match_whole_state_sweep(gpu_ctx):
             assert gpu_ctx.read_main(0, len(mods)) == whole, n
     finally:
         gpu_ctx.set_height_caps(True)
+
+
+@pytest.mark.parametrize("n", [16, 23])
+def test_sorted_cells_same_residues(gpu_ctx, n):
+    """The sorted assignment on the upper cells changes no residue: n=16 with several
+    moduli (32-bit ranks) and n=23 with one (64-bit ranks, the reference's residue)."""
+    mods = [4294967291, 4294966997, 2147483647, 65537] if n == 16 else [4294967291]
+    try:
+        gpu_ctx.set_sorted_cells(0)
+        if n == 23:
+            want = [2762394459]
+        else:
+            want, _ = gpu_ctx.solve(n, mods)
+        for sf in ((1, 6, 12) if n == 16 else (6,)):
+            print(f"case sorted cells from {sf}, n={n}", flush=True)
+            gpu_ctx.set_sorted_cells(sf)
+            got, _ = gpu_ctx.solve(n, mods)
+            assert got == want, (n, sf)
+    finally:
+        gpu_ctx.set_sorted_cells(0)
 
 
 def test_xcd_pieces_same_residues(gpu_ctx):
