@@ -10,9 +10,9 @@ and, with --scale, the calibrated bytes (the scale is what a device-to-device
 copy of known size reads for the same counter).
 
 With --cells C (cells per row, W-1) the cell launches of each moduli count KC
-(full-state `cell_kernel` and height-capped `cell_capped_kernel` together, in
-dispatch order, one launch per cell when KC <= 16) are cut into rows of C
-launches: `trace` lists each row's time and how many of its cells ran capped,
+(full-state `cell_kernel` / `cell_sorted_kernel` and height-capped
+`cell_capped_kernel` together, in dispatch order, one launch per cell when
+KC <= 16) are cut into rows of C launches (--rows R keeps the first R): `trace` lists each row's time and how many of its cells ran capped,
 then the per-cell times (p = W-1 .. 1) of the median full-state row; `counter`
 gives the mean over full-state rows' dispatches and the per-cell values of the
 last full-state row.
@@ -44,14 +44,18 @@ def _short(name: str) -> str:
 
 
 def _kc(name: str) -> int | None:
-    m = re.search(r"cell_(?:capped_)?kernel<(\d+)", name)
+    m = re.search(r"cell_(?:capped_|sorted_)?kernel<(\d+)", name)
     return int(m.group(1)) if m else None
+
+
+MAX_ROWS = None  # only the first rows of each moduli count (--rows); None = all
 
 
 def _rows(seq, cells):
     """seq: [(order, value, capped)] -> rows of `cells` consecutive dispatches."""
     seq = sorted(seq)
-    return [seq[i:i + cells] for i in range(0, len(seq) - cells + 1, cells)]
+    rows = [seq[i:i + cells] for i in range(0, len(seq) - cells + 1, cells)]
+    return rows[:MAX_ROWS] if MAX_ROWS else rows
 
 
 def trace(path: str, cells: int | None) -> None:
@@ -164,11 +168,16 @@ def main(argv=None) -> int:
     ap.add_argument("mode", choices=["trace", "counter", "valu"])
     ap.add_argument("csv")
     ap.add_argument("--cells", type=int, default=None, help="cells per row (W-1) for the per-row listing")
+    ap.add_argument("--rows", type=int, default=None,
+                    help="only the first ROWS rows of each moduli count (one solve's W rows; later "
+                         "launches of other sizes are left out)")
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--cus", type=int, default=256, help="compute units (MI355X: 256)")
     ap.add_argument("--xcds", type=int, default=8, help="XCDs GRBM_GUI_ACTIVE is summed over (MI355X: 8)")
     ap.add_argument("--words", type=int, default=None, help="blocked words per launch (per-64-word counts)")
     a = ap.parse_args(argv)
+    global MAX_ROWS
+    MAX_ROWS = a.rows
     if a.mode == "trace":
         trace(a.csv, a.cells)
     elif a.mode == "counter":
