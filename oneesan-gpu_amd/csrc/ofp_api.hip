@@ -852,7 +852,7 @@ ofp_status ofp_shard_sweep(ofp_ctx* c, const ofp_shard_spec* sp, ofp_shard_info*
             a.nblk = B;
             a.first = info->first_blocked;
             a.layout = 1;
-            a.variant = c->variant;
+            a.variant = c->variant;  // shard cells are never sorted (slower at 64-bit ranks)
             a.xcd_shift = c->xcd_shift;
             a.W = W;
             a.p = p;

@@ -66,21 +66,15 @@ __global__ __launch_bounds__(256) void cell_kernel(CellLaunch a) {
     }
 }
 
-// Sorted variant (state-major, uniform orbit body): the 256 words of a chunk are
-// decoded, then dealt back to the workgroup's threads with the x=N words first,
-// so the closure scans (x=N only) fill whole waves and the R/L words finish
-// together in the other waves.  Each word still has exactly one owner, so the
-// in-place argument is unchanged.  Three workgroup barriers per chunk.
-template <int KC, typename Idx>
-__global__ __launch_bounds__(256) void cell_sorted_kernel(CellLaunch a) {
-    __shared__ Quad<Idx> tab[kDpRows][kDpCols];
-    __shared__ Decoded<Idx> slot[256];
-    __shared__ uint32_t cnt[2][4];
-    const Quad<Idx>* g = (const Quad<Idx>*)a.dp;
-    for (int t = threadIdx.x; t < kDpRows * kDpCols; t += blockDim.x) (&tab[0][0])[t] = g[t];
-    __syncthreads();
-    Tab<Idx> T{tab};
-    const Direct<KC, kAoS, Idx> mem{a.main, a.blk, a.K, a.off};
+// Sorted assignment (state-major, uniform orbit body): the 256 words of a chunk
+// are decoded, then dealt back to the workgroup's threads with the x=N words
+// first, so the closure scans (x=N only) fill whole waves and every wave loads
+// and stores one kind of orbit word for consecutive words.  Each word still has
+// exactly one owner, so the in-place argument is unchanged.  Three workgroup
+// barriers per chunk; slot and cnt are the workgroup's LDS.
+template <int KC, typename Idx, class Mem>
+__device__ void sorted_sweep(const CellLaunch& a, const Tab<Idx>& T, const Mem& mem, Decoded<Idx>* slot,
+                             uint32_t (*cnt)[4]) {
     const Sched sc(a.nblk, blockDim.x, gridDim.x, blockIdx.x, a.xcd_shift);
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const uint64_t below = lane ? (~0ull >> (64u - lane)) : 0ull;  // lanes under this one
@@ -116,6 +110,19 @@ __global__ __launch_bounds__(256) void cell_sorted_kernel(CellLaunch a) {
         }
         __syncthreads();  // slot and cnt are rewritten for the next chunk
     }
+}
+
+template <int KC, typename Idx>
+__global__ __launch_bounds__(256) void cell_sorted_kernel(CellLaunch a) {
+    __shared__ Quad<Idx> tab[kDpRows][kDpCols];
+    __shared__ Decoded<Idx> slot[256];
+    __shared__ uint32_t cnt[2][4];
+    const Quad<Idx>* g = (const Quad<Idx>*)a.dp;
+    for (int t = threadIdx.x; t < kDpRows * kDpCols; t += blockDim.x) (&tab[0][0])[t] = g[t];
+    __syncthreads();
+    Tab<Idx> T{tab};
+    const Direct<KC, kAoS, Idx> mem{a.main, a.blk, a.K, a.off};
+    sorted_sweep<KC, Idx>(a, T, mem, slot, cnt);
 }
 
 // Shard variant: same per-thread body, windowed memory policy (state-major).
@@ -209,7 +216,7 @@ template <typename Idx, int KC>
 hipError_t launch_shard_kc(const CellLaunch& a, int threads, int blocks, hipStream_t s) {
     if (a.variant == 1)
         hipLaunchKernelGGL((cell_shard_kernel<KC, Idx, 1>), dim3(blocks), dim3(threads), 0, s, a);
-    else if (a.variant == 2 || a.variant == 4)  // no sorted shard kernel: the uniform body
+    else if (a.variant == 2 || a.variant == 4)  // shards are never sorted: the uniform body
         hipLaunchKernelGGL((cell_shard_kernel<KC, Idx, 2>), dim3(blocks), dim3(threads), 0, s, a);
     else if (a.variant == 3)
         hipLaunchKernelGGL((cell_shard_kernel<KC, Idx, 3>), dim3(blocks), dim3(threads), 0, s, a);
