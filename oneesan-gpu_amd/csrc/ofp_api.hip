@@ -63,7 +63,8 @@ struct ofp_ctx {
     void* dpc64 = nullptr;
     int cap_percent = 85;  // height caps: cap a cell when it then sweeps <= this % of the blocked words
     int xcd_shift = 5;     // chunk pieces per XCD: 2^xcd_shift chunks, < 0 = one eighth each
-    int sorted_from = 0;   // full-state cells p >= sorted_from run the sorted kernel (0: none)
+    int sorted_from = -1;  // full-state cells p >= sorted_from run the sorted kernel (0: none,
+                           // -1: from cell 6 for 32-bit ranks and 8 or more moduli, else none)
     std::string err;
     Sweep st;              // the context's own state (solve / prepare / row_sweep)
     int jobs = 0;          // live jobs (serving)
@@ -278,7 +279,9 @@ ofp_status enqueue_cells(ofp_ctx* c, Sweep& s, int p_hi, int p_lo, bool first_re
             a.first = 0;
             a.layout = s.layout;
             // the sorted assignment (variant 4) on the cells from sorted_from up
-            a.variant = (c->sorted_from > 0 && p >= c->sorted_from && s.layout == 1) ? 4 : c->variant;
+            const int sf = c->sorted_from >= 0 ? c->sorted_from
+                                                : (!wide && std::min(step, s.K - off) >= 8 ? 6 : 0);
+            a.variant = (sf > 0 && p >= sf && s.layout == 1) ? 4 : c->variant;
             a.xcd_shift = c->xcd_shift;
             a.W = s.W;
             a.p = p;
@@ -610,7 +613,7 @@ ofp_status ofp_set_xcd_pieces(ofp_ctx* c, int log2_chunks) {
 
 ofp_status ofp_set_sorted_cells(ofp_ctx* c, int from_cell) {
     if (!c) return OFP_E_ARG;
-    if (from_cell < 0 || from_cell > kMaxW) return fail(c, OFP_E_ARG, "from_cell must be 0..28");
+    if (from_cell < -1 || from_cell > kMaxW) return fail(c, OFP_E_ARG, "from_cell must be -1..28");
     if (from_cell != c->sorted_from) drop_graph(c->st);
     c->sorted_from = from_cell;
     return OFP_OK;

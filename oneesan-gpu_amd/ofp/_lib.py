@@ -344,7 +344,7 @@ class Context:
 
     def set_sorted_cells(self, from_cell: int) -> None:
         """Full-state cells p >= from_cell run the sorted kernel (x=N words dealt to
-        their own waves); 0 = never."""
+        their own waves); 0 = never, -1 = the library's choice by moduli per launch."""
         self._check(self._lib.ofp_set_sorted_cells(self._h, from_cell), "ofp_set_sorted_cells")
 
     def set_height_caps(self, max_percent: int | bool) -> None:

@@ -84,7 +84,7 @@ def apply(ctx: "_lib.Context", n: int, k: int) -> dict | None:
         ctx.set_low_cells(int(ent.get("low_cells", 0)), int(ent.get("low_kc", 16)))
         ctx.set_height_caps(int(ent.get("cap_percent", _lib.DEFAULT_CAP_PERCENT)))
         ctx.set_xcd_pieces(int(ent.get("xcd_shift", 5)))
-        ctx.set_sorted_cells(int(ent.get("sorted_from", 0)))
+        ctx.set_sorted_cells(int(ent.get("sorted_from", -1)))
     return ent
 
 

@@ -181,7 +181,7 @@ def test_sorted_cells_same_residues(gpu_ctx, n):
             got, _ = gpu_ctx.solve(n, mods)
             assert got == want, (n, sf)
     finally:
-        gpu_ctx.set_sorted_cells(0)
+        gpu_ctx.set_sorted_cells(-1)  # the library's default
 
 
 def test_xcd_pieces_same_residues(gpu_ctx):
