@@ -266,6 +266,12 @@ def main() -> int:
                  "full_row_ms": (sp["full_kernel_s"] / sp["full_launches"] * (W - 1) * 1e3
                                  if sp["full_launches"] else None),
                  "residue": pres[0], "residue_matches_batched": residues is not None and pres[0] == residues[0]}
+        if sp["full_launches"]:
+            # the same declared bytes as the batched roofline, at one modulus; this form
+            # is bound by its integer index work, not by HBM
+            pl_s = sp["full_kernel_s"] / sp["full_launches"]
+            plain.update({"avg_launch_ms": pl_s * 1e3, "declared_GBps": 8.0 * S / pl_s / 1e9,
+                          "hbm_frac": 8.0 * S / pl_s / 1e9 / HBM_PEAK_GBS})
         log(f"bench: plain k=1 solve {tp:.2f} s -> {plain['residues_per_hour']:.0f} residues/h "
             f"(residue {pres[0]})")
     ctx.close()

@@ -127,8 +127,9 @@ ofp_status ofp_set_variant(ofp_ctx* ctx, int variant);
 
 /* Work distribution over the 8 XCDs: the blocked words are cut into chunks of
  * one workgroup and dealt to the XCDs in pieces of 2^log2_chunks chunks,
- * round-robin (default 5); -1 gives each XCD one contiguous eighth.  Set by the
- * autotuner; no reference counterpart. */
+ * round-robin (default 5; a launch with fewer than 64 such pieces uses smaller
+ * ones); -1 gives each XCD one contiguous eighth.  Set by the autotuner; no
+ * reference counterpart. */
 ofp_status ofp_set_xcd_pieces(ofp_ctx* ctx, int log2_chunks);
 
 /* Full-state cells p >= from_cell (state-major layout) run the sorted kernel

@@ -243,6 +243,17 @@ uint64_t capped_blocked_count(int W, int cap) {
     return d[W - 1][1];
 }
 
+// Piece size (log2 chunks) of one launch: the context's, lowered until the launch
+// has at least 64 pieces.  With fewer pieces than XCDs the pieces land on a few
+// XCDs and their workgroups walk the chunks one after another (a launch of 8
+// chunks in one 32-chunk piece ran on one workgroup); 64 pieces keep the XCDs
+// within one piece of each other.
+int piece_shift(int shift, uint64_t nwords, int threads) {
+    const uint64_t chunks = (nwords + threads - 1) / threads;
+    while (shift > 0 && (chunks >> shift) < 64) --shift;
+    return shift;
+}
+
 // The cap a cell runs with: the bound above, when the capped kernel has few
 // enough words to pay for its longer decode (<= percent% of the blocked words).
 int effective_cap(int W, int r, int p, uint64_t nblk, uint64_t* count, int percent = 85) {
@@ -282,7 +293,7 @@ ofp_status enqueue_cells(ofp_ctx* c, Sweep& s, int p_hi, int p_lo, bool first_re
             const int sf = c->sorted_from >= 0 ? c->sorted_from
                                                 : (!wide && std::min(step, s.K - off) >= 8 ? 6 : 0);
             a.variant = (sf > 0 && p >= sf && s.layout == 1) ? 4 : c->variant;
-            a.xcd_shift = c->xcd_shift;
+            a.xcd_shift = piece_shift(c->xcd_shift, cap ? ncap : s.nblk, c->threads);
             a.W = s.W;
             a.p = p;
             a.read_blk = (p == p_hi) ? (first_reads_blk ? 1 : 0) : 1;
@@ -856,7 +867,7 @@ ofp_status ofp_shard_sweep(ofp_ctx* c, const ofp_shard_spec* sp, ofp_shard_info*
             a.first = info->first_blocked;
             a.layout = 1;
             a.variant = c->variant;  // shard cells are never sorted (slower at 64-bit ranks)
-            a.xcd_shift = c->xcd_shift;
+            a.xcd_shift = piece_shift(c->xcd_shift, B, c->threads);
             a.W = W;
             a.p = p;
             a.read_blk = (p == W - 1) ? 0 : 1;
