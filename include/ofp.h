@@ -147,7 +147,8 @@ ofp_status ofp_set_sorted_cells(ofp_ctx* ctx, int from_cell);
  * height (every other word is still zero), and in the last two rows only such
  * words can still reach the answer word; a capped cell enumerates just those
  * blocked words.  A cell is capped when it then sweeps at most max_percent % of
- * the blocked words (default 85; 0 turns caps off).  The answer is the same;
+ * the blocked words (default, and -1: 75 at one modulus, 85 with more; 0 turns
+ * caps off).  The answer is the same;
  * other main words read after the last two rows are not final.  Reference
  * analogue: the row-6 prefix initialiser and the mirrored last row
  * (row6crt20.cu:1434-1688,1787-1814,935-952), which likewise skip the first

@@ -61,7 +61,8 @@ struct ofp_ctx {
     void* dp64 = nullptr;  // packed uint64 height table
     void* dpc32 = nullptr; // packed capped tables, caps 1..kMaxCap (uint32, uint64)
     void* dpc64 = nullptr;
-    int cap_percent = 85;  // height caps: cap a cell when it then sweeps <= this % of the blocked words
+    int cap_percent = -1;  // height caps: cap a cell when it then sweeps <= this % of the blocked words
+                           // (-1: 75 at one modulus, else 85)
     int xcd_shift = 5;     // chunk pieces per XCD: 2^xcd_shift chunks, < 0 = one eighth each
     int sorted_from = -1;  // full-state cells p >= sorted_from run the sorted kernel (0: none,
                            // -1: from cell 6 for 32-bit ranks and 8 or more moduli, else none)
@@ -265,16 +266,23 @@ int effective_cap(int W, int r, int p, uint64_t nblk, uint64_t* count, int perce
     return cap;
 }
 
+// The height-cap threshold of a sweep: the context's, else 75 % at one modulus
+// (index-bound: the capped kernel's longer decode weighs more) and 85 % above.
+int cap_pct(const ofp_ctx* c, const Sweep& s) {
+    return c->cap_percent >= 0 ? c->cap_percent : (s.K == 1 ? 75 : 85);
+}
+
 // Enqueue cells p_hi..p_lo of s on s.stream; row >= 0 names the row of a sweep
 // from the initial state (height caps apply), -1 an arbitrary state.
 ofp_status enqueue_cells(ofp_ctx* c, Sweep& s, int p_hi, int p_lo, bool first_reads_blk, int* launches,
                          int row = -1) {
     const bool wide = s.W > 23;
     const int blocks = grid_blocks(c, s.nblk);
-    const bool caps = c->cap_percent > 0 && row >= 0 && s.layout == 1;
+    const int pct = cap_pct(c, s);
+    const bool caps = pct > 0 && row >= 0 && s.layout == 1;
     for (int p = p_hi; p >= p_lo; --p) {
         uint64_t ncap = 0;
-        const int cap = caps ? effective_cap(s.W, row, p, s.nblk, &ncap, c->cap_percent) : 0;
+        const int cap = caps ? effective_cap(s.W, row, p, s.nblk, &ncap, pct) : 0;
         // moduli per launch: all (up to 16) at once, or fewer on the low cells,
         // whose scattered closure reads are latency-bound (fewer registers per
         // thread -> more waves in flight).  Chunks keep an even offset.
@@ -365,9 +373,10 @@ ofp_status full_rows(ofp_ctx* c, Sweep& s, int rows, int* launches) {
 
 // True when some cell of row r (of a sweep from the initial state) runs capped.
 bool row_capped(const ofp_ctx* c, const Sweep& s, int r) {
-    if (c->cap_percent <= 0 || r < 0 || s.layout != 1) return false;
+    const int pct = cap_pct(c, s);
+    if (pct <= 0 || r < 0 || s.layout != 1) return false;
     for (int p = s.W - 1; p >= 1; --p)
-        if (effective_cap(s.W, r, p, s.nblk, nullptr, c->cap_percent)) return true;
+        if (effective_cap(s.W, r, p, s.nblk, nullptr, pct)) return true;
     return false;
 }
 
@@ -632,7 +641,7 @@ ofp_status ofp_set_sorted_cells(ofp_ctx* c, int from_cell) {
 
 ofp_status ofp_set_height_caps(ofp_ctx* c, int max_percent) {
     if (!c) return OFP_E_ARG;
-    if (max_percent < 0 || max_percent > 100) return fail(c, OFP_E_ARG, "max_percent must be 0..100");
+    if (max_percent < -1 || max_percent > 100) return fail(c, OFP_E_ARG, "max_percent must be -1..100");
     c->cap_percent = max_percent;
     return OFP_OK;
 }

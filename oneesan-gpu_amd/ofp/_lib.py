@@ -350,9 +350,10 @@ class Context:
     def set_height_caps(self, max_percent: int | bool) -> None:
         """Skip, in the first and last rows of a solve, the blocked words whose
         height exceeds what the row can hold or still use, in cells where that
-        leaves at most max_percent % of them (True = the default 85, False/0 = off)."""
+        leaves at most max_percent % of them (True or -1 = the library's default: 75 at
+        one modulus, 85 with more; False/0 = off)."""
         if max_percent is True:
-            max_percent = DEFAULT_CAP_PERCENT
+            max_percent = -1
         self._check(self._lib.ofp_set_height_caps(self._h, int(max_percent)), "ofp_set_height_caps")
 
     def set_low_cells(self, low_cells: int, low_kc: int) -> None:

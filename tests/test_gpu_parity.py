@@ -160,6 +160,14 @@ def test_height_caps_match_whole_state_sweep(gpu_ctx):
             gpu_ctx.row_sweep(W - 3)
             gpu_ctx.row_sweep(1)
             assert gpu_ctx.read_main(0, len(mods)) == whole, n
+        # one modulus: the default threshold is lower (75 %)
+        for n in (11, 17):
+            print(f"case caps n={n} k=1", flush=True)
+            gpu_ctx.set_height_caps(False)
+            whole, _ = gpu_ctx.solve(n, mods[:1])
+            gpu_ctx.set_height_caps(True)
+            capped, st = gpu_ctx.solve(n, mods[:1])
+            assert capped == whole and st["capped_launches"] > 0, n
     finally:
         gpu_ctx.set_height_caps(True)
 
