@@ -8,7 +8,7 @@ pytestmark = pytest.mark.gpu
 MODS = [4294967291, 4294966997, 2147483647, 1000003]
 
 
-@pytest.mark.parametrize("variant", [0, 3])
+@pytest.mark.parametrize("variant", [0, 2, 3])
 @pytest.mark.parametrize("n,slices,picks", [(12, 1, [0]), (12, 5, [0, 2, 4]), (16, 16, [3, 15]),
                                             (27, 1 << 30, [7, (1 << 29) + 12345])])
 def test_shard_sweep_matches_oracle(gpu_ctx, oracle, n, slices, picks, variant):
