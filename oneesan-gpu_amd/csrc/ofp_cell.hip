@@ -43,6 +43,13 @@ namespace {
 
 using namespace cell;
 
+// Copy the first kLdsRows x C entries of a kDpRows x kDpCols table into LDS
+// (row stride C there).
+template <int C, typename T>
+__device__ void load_table(T* dst, const T* src) {
+    for (int t = threadIdx.x; t < kLdsRows * C; t += blockDim.x) dst[t] = src[(t / C) * kDpCols + t % C];
+}
+
 // Work distribution: the blocked range is cut into chunks of blockDim.x words;
 // workgroup g runs on XCD g%8 (round-robin dispatch).  The chunks are dealt to
 // the XCDs in pieces of 2^shift consecutive chunks, round-robin (piece i to XCD
@@ -53,9 +60,9 @@ using namespace cell;
 // blocked range) is shared by all XCDs.
 template <int KC, typename Idx, int L, int V>
 __global__ __launch_bounds__(256) void cell_kernel(CellLaunch a) {
-    __shared__ Quad<Idx> tab[kDpRows][kDpCols];
+    __shared__ Quad<Idx> tab[kLdsRows][LdsCols<Idx>::value];
     const Quad<Idx>* g = (const Quad<Idx>*)a.dp;
-    for (int t = threadIdx.x; t < kDpRows * kDpCols; t += blockDim.x) (&tab[0][0])[t] = g[t];
+    load_table<LdsCols<Idx>::value>(&tab[0][0], g);
     __syncthreads();
     Tab<Idx> T{tab};
     const Direct<KC, L, Idx> mem{a.main, a.blk, a.K, a.off};
@@ -114,11 +121,11 @@ __device__ void sorted_sweep(const CellLaunch& a, const Tab<Idx>& T, const Mem& 
 
 template <int KC, typename Idx>
 __global__ __launch_bounds__(256) void cell_sorted_kernel(CellLaunch a) {
-    __shared__ Quad<Idx> tab[kDpRows][kDpCols];
+    __shared__ Quad<Idx> tab[kLdsRows][LdsCols<Idx>::value];
     __shared__ Decoded<Idx> slot[256];
     __shared__ uint32_t cnt[2][4];
     const Quad<Idx>* g = (const Quad<Idx>*)a.dp;
-    for (int t = threadIdx.x; t < kDpRows * kDpCols; t += blockDim.x) (&tab[0][0])[t] = g[t];
+    load_table<LdsCols<Idx>::value>(&tab[0][0], g);
     __syncthreads();
     Tab<Idx> T{tab};
     const Direct<KC, kAoS, Idx> mem{a.main, a.blk, a.K, a.off};
@@ -128,9 +135,9 @@ __global__ __launch_bounds__(256) void cell_sorted_kernel(CellLaunch a) {
 // Shard variant: same per-thread body, windowed memory policy (state-major).
 template <int KC, typename Idx, int V>
 __global__ __launch_bounds__(256) void cell_shard_kernel(CellLaunch a) {
-    __shared__ Quad<Idx> tab[kDpRows][kDpCols];
+    __shared__ Quad<Idx> tab[kLdsRows][LdsCols<Idx>::value];
     const Quad<Idx>* g = (const Quad<Idx>*)a.dp;
-    for (int t = threadIdx.x; t < kDpRows * kDpCols; t += blockDim.x) (&tab[0][0])[t] = g[t];
+    load_table<LdsCols<Idx>::value>(&tab[0][0], g);
     __syncthreads();
     Tab<Idx> T{tab};
     Windowed<KC> mem;
@@ -158,14 +165,12 @@ __global__ __launch_bounds__(256) void cell_shard_kernel(CellLaunch a) {
 // The uniform orbit body (V = 2) runs on each decoded word.
 template <int KC, typename Idx>
 __global__ __launch_bounds__(256) void cell_capped_kernel(CellLaunch a) {
-    __shared__ Quad<Idx> tab[kDpRows][kDpCols];
-    __shared__ Pair<Idx> ctab[kDpRows][kDpCols];
+    __shared__ Quad<Idx> tab[kLdsRows][LdsCols<Idx>::value];
+    __shared__ Pair<Idx> ctab[kLdsRows][LdsCols<Idx>::value];
     const Quad<Idx>* g = (const Quad<Idx>*)a.dp;
     const Pair<Idx>* gc = (const Pair<Idx>*)a.dpc;
-    for (int t = threadIdx.x; t < kDpRows * kDpCols; t += blockDim.x) {
-        (&tab[0][0])[t] = g[t];
-        (&ctab[0][0])[t] = gc[t];
-    }
+    load_table<LdsCols<Idx>::value>(&tab[0][0], g);
+    load_table<LdsCols<Idx>::value>(&ctab[0][0], gc);
     __syncthreads();
     Tab<Idx> T{tab};
     const Direct<KC, kAoS, Idx> mem{a.main, a.blk, a.K, a.off};

@@ -92,9 +92,21 @@ OFP_HD int step_z(Idx zn, Idx zc, Idx& r, Idx& contrib) {
     return (int)g0 + (int)g1;
 }
 
+// Rows and columns of a height table kept in LDS.  A word of W <= kMaxW positions
+// entered at height 1 is at height <= min(pos + 2, W - pos) <= 15 at every
+// position it can be completed from, and the scans read at most two heights above
+// that, so columns 0..21 and positions 0..kMaxW-1 hold every entry read.  At 64-bit
+// ranks that keeps the quad table at 19.7 KB instead of 30 KB, so eight workgroups
+// of 256 fit in a CU's 160 KB of LDS; 32-bit tables keep all columns.
+constexpr int kLdsRows = kMaxW;
+template <typename Idx>
+struct LdsCols {
+    static constexpr int value = sizeof(Idx) == 8 ? 22 : kDpCols;
+};
+
 template <typename Idx>
 struct Tab {
-    const Quad<Idx> (*q)[kDpCols];
+    const Quad<Idx> (*q)[LdsCols<Idx>::value];
     OFP_HD Idx at(int pos, int h) const { return q[pos][h].zn; }
     OFP_HD Idx cL(int pos, int h) const { return q[pos][h].zc; }
     // rank contribution of value v (0=N,1=R,2=L) at position pos entered at height h
@@ -404,7 +416,8 @@ struct alignas(2 * sizeof(Idx)) Pair {
 // table T, so the result is the same as decode_blocked(T, W, p, bidx) for the
 // word's own index bidx.
 template <typename Idx>
-OFP_HD Decoded<Idx> decode_capped(const Tab<Idx>& T, const Pair<Idx> (*C)[kDpCols], int W, int p, uint64_t t) {
+OFP_HD Decoded<Idx> decode_capped(const Tab<Idx>& T, const Pair<Idx> (*C)[LdsCols<Idx>::value], int W, int p,
+                                   uint64_t t) {
     Decoded<Idx> d;
     Idx r = (Idx)t, bidx = 0, mainbase = 0, sr = 0;
     uint64_t word = 0;
