@@ -93,9 +93,9 @@ OFP_HD int step_z(Idx zn, Idx zc, Idx& r, Idx& contrib) {
 }
 
 // Rows and columns of a height table kept in LDS.  A word of W <= kMaxW positions
-// entered at height 1 is at height <= min(pos + 2, W - pos) <= 15 at every
-// position it can be completed from, and the scans read at most two heights above
-// that, so columns 0..21 and positions 0..kMaxW-1 hold every entry read.  At 64-bit
+// entered at height 1 is at height <= 14 at every position it can still be
+// completed from, and the scans read at most two heights above that, so columns
+// 0..21 and positions 0..kMaxW-1 hold every entry read.  At 64-bit
 // ranks that keeps the quad table at 19.7 KB instead of 30 KB, so eight workgroups
 // of 256 fit in a CU's 160 KB of LDS; 32-bit tables keep all columns.
 constexpr int kLdsRows = kMaxW;
