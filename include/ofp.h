@@ -136,9 +136,8 @@ ofp_status ofp_set_xcd_pieces(ofp_ctx* ctx, int log2_chunks);
  * (variant 4): each workgroup decodes its 256 blocked words, then deals them
  * back with the x=N words first, so that a wave holds one kind of word and its
  * orbit loads and stores are contiguous.  0 = never; -1 (default) = from cell 6
- * for launches carrying 8 or more moduli with 32-bit ranks (n <= 22, HBM-bound),
- * never otherwise (bound by index math, or by occupancy with 64-bit ranks, where
- * the barriers cost more than the contiguity gains).
+ * for launches carrying 8 or more moduli (HBM-bound), never otherwise (bound by
+ * index math, where the barriers cost more than the contiguity gains).
  * Results do not depend on it.  Set by the autotuner; no reference counterpart. */
 ofp_status ofp_set_sorted_cells(ofp_ctx* ctx, int from_cell);
 

@@ -65,7 +65,7 @@ struct ofp_ctx {
                            // (-1: 75 at one modulus, else 85)
     int xcd_shift = 5;     // chunk pieces per XCD: 2^xcd_shift chunks, < 0 = one eighth each
     int sorted_from = -1;  // full-state cells p >= sorted_from run the sorted kernel (0: none,
-                           // -1: from cell 6 for 32-bit ranks and 8 or more moduli, else none)
+                           // -1: from cell 6 for launches of 8 or more moduli, else none)
     std::string err;
     Sweep st;              // the context's own state (solve / prepare / row_sweep)
     int jobs = 0;          // live jobs (serving)
@@ -299,7 +299,7 @@ ofp_status enqueue_cells(ofp_ctx* c, Sweep& s, int p_hi, int p_lo, bool first_re
             a.layout = s.layout;
             // the sorted assignment (variant 4) on the cells from sorted_from up
             const int sf = c->sorted_from >= 0 ? c->sorted_from
-                                                : (!wide && std::min(step, s.K - off) >= 8 ? 6 : 0);
+                                                : (std::min(step, s.K - off) >= 8 ? 6 : 0);
             a.variant = (sf > 0 && p >= sf && s.layout == 1) ? 4 : c->variant;
             a.xcd_shift = piece_shift(c->xcd_shift, cap ? ncap : s.nblk, c->threads);
             a.W = s.W;
