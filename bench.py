@@ -2,43 +2,56 @@
 """Benchmark: frontier transitions/s and residues/hour, batched vs one-residue,
 on one MI355X, for the corner-to-corner simple-path count at n=22.
 
-A *step* is one row sweep (cells W-1..1, W = n+1) of the interleaved frontier
-state: every main and blocked word, all k CRT moduli, in HBM.  With the
-defaults (--warmup 1 --steps 22) warmup + timed steps are the W = 23 rows of
-one complete n=22 solve, so the residues are read afterwards and checked
-against the reference's published value for the first prime.
-
-Workload: n=22, k = number of primes the checkerboard-strip bound needs (14),
-the largest primes below 2^32 -- an exact-count (CRT) run.  Inputs: the
+Workload: n=22, k = the number of primes the checkerboard-strip bound needs
+(14), the largest primes below 2^32 -- an exact-count (CRT) run.  Inputs: the
 solve's own initial state (synthetic in the sense that nothing is read from
-disk).  The first rows and the last two rows of a solve sweep only the words
-of small height (height caps; the answer is unchanged), so rows differ in
-cost: residues/hour comes from the time of the whole solve (initialisation,
-warmup row and timed rows; the state memory is allocated before), and `value`
-counts every row at its full
-algorithmic work W-1 cells x S states x k moduli.  Also measured in the same
-process:
-  * plain: the same solve with k=1 (one residue at a time), timed whole
-    -> residues/hour of the plain form;
-  * roofline of the cell kernel: declared algorithmic bytes per launch
-    (8 B per state per modulus: read + write of one uint32 residue) divided by
-    the average duration of the full-state launches (rows without capped
-    cells), from HIP events on the library's stream;
-  * plain_small: whole one-modulus solves at n=12 and n=18 (the small plain-form
-    configurations; median of three, the n=18 residue checked against the
-    reference's published value);
-  * cpu_baseline: the CPU oracle's pull-form step timed on the host cores on
-    a bounded sample of the same n=22 cell step, and its whole n=12 solve
-    (median of three, residue compared with the GPU's).
+disk); every buffer is device-resident before any timed region starts.
+
+Measured in one process, in this order:
+  * solve: one complete n=22 solve with all k moduli interleaved (state
+    initialisation + W = 23 rows; the memory is allocated before), timed whole.
+    All k residues are reported; the first is checked against the reference's
+    published value and every residue the committed ggcount fixtures hold for
+    n=22 is checked too.  residues_per_hour = 3600 k / solve time.  The first
+    rows and the last two rows of a solve sweep only the words of small height
+    (height caps; the answer is unchanged), so the solve does less than W full
+    rows of work.
+  * steps (the `value`): --warmup untimed and --steps timed row sweeps of the
+    same state with height caps off, so that every row does its full
+    algorithmic work (W-1 cells x S states x k moduli transitions); each cell is
+    launched on its own between HIP events on the library's stream, and the
+    timed region is bracketed by device synchronisation.  value = transitions
+    of the timed rows / their wall time.
+  * roofline: per kernel (cell_sorted_kernel: cells p >= 6, cell_kernel: p <= 5)
+    the mean launch duration over the timed rows' events; declared algorithmic
+    bytes per launch = 8 B x k x S (one 4-byte read and one 4-byte write per
+    state per modulus).  `roofline` is the dominant kernel's.  HBM traffic per
+    launch is measured before the GPU work starts, by running one row of the same
+    configuration in a child process under rocprofv3 with the DRAM request
+    counters TCC_EA0_RDREQ_DRAM_32B and TCC_EA0_WRREQ_WRITE_DRAM_32B (32-byte
+    units; no calibration needed); null if rocprofv3 is unavailable.  The
+    height-capped kernel of the solve is reported beside them.
+  * plain: the same solve with k=1 (one residue at a time), timed whole ->
+    residues/hour of the plain form, and its per-kernel launch times.
+  * plain_small: whole one-modulus solves at n=12 and n=18 (median of three).
+  * shard_n27: one GPU's share of an n=27 row (ofp.shard).
+  * cpu_baseline: the CPU oracle's pull-form step on a bounded sample of the
+    n=22 cell step, its whole n=12 solve (median of three) and one whole n=16
+    solve, on the host cores.
 
 Prints ONE JSON line on stdout.
 """
 from __future__ import annotations
 
 import argparse
+import csv
+import glob
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 from pathlib import Path
 
@@ -49,6 +62,7 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 GOLDEN_FIRST = {22: (4294967291, 1245069298), 21: (4294967291, 998035516),
                 20: (4294967291, 2308006916), 18: (4294967291, 503411004)}
+COUNTERS = ("TCC_EA0_RDREQ_DRAM_32B_sum", "TCC_EA0_WRREQ_WRITE_DRAM_32B_sum")
 
 
 def log(*a):
@@ -56,7 +70,14 @@ def log(*a):
 
 
 def _cpu_model() -> str | None:
-    """The host CPU's model name (first 'model name' line of /proc/cpuinfo)."""
+    """The host CPU's model name (lscpu, else /proc/cpuinfo)."""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
@@ -67,13 +88,35 @@ def _cpu_model() -> str | None:
     return None
 
 
-def cpu_baseline(n: int, k: int, seconds: float) -> dict:
+def golden_residues(n: int) -> dict[int, int]:
+    """Residues of n the committed fixtures hold: {modulus: residue}."""
+    out: dict[int, int] = {}
+    gd = ROOT / "tests" / "golden"
+    for name in ("reference_residues.json", "ggcount_residues.json", "ggcount_residues_large.json"):
+        try:
+            rows = json.loads((gd / name).read_text())["residues"]
+        except (OSError, ValueError, KeyError):
+            continue
+        for r in rows:
+            if r["n"] == n:
+                out[int(r["modulus"])] = int(r["residue"])
+    return out
+
+
+def capped_words(width: int, cap: int) -> int:
+    """Words of `width` positions entered at height 1 whose heights stay <= cap."""
+    d = [0] * (cap + 2)
+    d[0] = 1  # words of width 0 end at height 0
+    for _ in range(width):
+        d = [(d[h] + (d[h - 1] if h > 0 else 0) + d[h + 1]) if h <= cap else 0 for h in range(cap + 1)] + [0]
+    return d[1]
+
+
+def cpu_baseline(n: int, seconds: float) -> dict:
     """Time the CPU oracle's pull-form cell step on a bounded sample of the
     n-sized state (consecutive target windows of a middle cell), at one
     modulus; rate = target states processed per second."""
     import ctypes
-
-    import numpy as np
 
     from oracle import binding
     lib = binding.load()
@@ -122,51 +165,159 @@ def cpu_baseline(n: int, k: int, seconds: float) -> dict:
             "residues_per_hour_extrapolated": (3600.0 * rate / (W * (W - 1) * (nm + nb))) if rate else None}
 
 
-def cpu_solve_n12(mod: int) -> dict:
-    """The CPU oracle's whole n=12 solve (pull form, one modulus): one warm-up,
-    then the median of three."""
+def cpu_solve(n: int, mod: int, repeats: int) -> dict:
+    """The CPU oracle's whole solve (pull form, one modulus): one warm-up and the
+    median of `repeats` when repeats > 1, else one timed solve."""
     from oracle import binding
-    binding.solve(12, mod)
+    if repeats > 1:
+        binding.solve(n, mod)
     times, res = [], None
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        res = binding.solve(n, mod)
+        times.append(time.perf_counter() - t0)
+    med = sorted(times)[len(times) // 2]
+    W = n + 1
+    lib = binding.load()
+    S = int(lib.or_main_count(n)) + int(lib.or_blocked_count(n))
+    return {"solve_s": med, "residue": res, "transitions_per_s": W * (W - 1) * S / med,
+            "sample": f"oracle whole n={n} solve (oracle/gridfp_oracle.c or_solve, pull form), "
+                      + (f"median of {repeats}" if repeats > 1 else "one run")}
+
+
+def copy_rate(nbytes: int) -> dict | None:
+    """One device-to-device copy of nbytes (the HIP runtime's copy kernel): the
+    practical streaming ceiling next to the published 8 TB/s."""
+    import ctypes
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+    except OSError:
+        return None
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(nbytes)) != 0:
+        return None
+    if hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(nbytes)) != 0:
+        hip.hipFree(src)
+        return None
+    hip.hipMemset(src, 1, ctypes.c_size_t(nbytes))
+    hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3)
+    hip.hipDeviceSynchronize()
+    best = None
     for _ in range(3):
         t0 = time.perf_counter()
-        res = binding.solve(12, mod)
-        times.append(time.perf_counter() - t0)
-    med = sorted(times)[1]
-    T = 13 * 12 * 98123
-    return {"solve_s": med, "residue": res, "transitions_per_s": T / med,
-            "sample": "oracle whole n=12 solve (oracle/gridfp_oracle.c or_solve, pull form), median of 3"}
-
-
-def copy_calibration(nbytes: int) -> int:
-    """One device-to-device copy of nbytes (HIP runtime blit kernel), so the
-    rocprofv3 FETCH_SIZE / WRITE_SIZE counters can be scaled: a copy of B bytes
-    must read and write B bytes."""
-    import ctypes
-    hip = ctypes.CDLL("libamdhip64.so")
-    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
-    for ptr in (src, dst):
-        if hip.hipMalloc(ctypes.byref(ptr), ctypes.c_size_t(nbytes)) != 0:
-            log("copy calibration: hipMalloc failed")
-            return 1
-    hip.hipMemset(src, 1, ctypes.c_size_t(nbytes))
-    hip.hipDeviceSynchronize()
-    t0 = time.perf_counter()
-    rc = hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
-    hip.hipDeviceSynchronize()
-    el = time.perf_counter() - t0
+        rc = hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+        hip.hipDeviceSynchronize()
+        el = time.perf_counter() - t0
+        if rc == 0 and (best is None or el < best):
+            best = el
     hip.hipFree(src)
     hip.hipFree(dst)
-    print(json.dumps({"copy_bytes": nbytes, "seconds": el, "GBps_read_plus_write": 2 * nbytes / el / 1e9,
-                      "rc": rc}), flush=True)
-    return 0 if rc == 0 else 1
+    if best is None:
+        return None
+    return {"bytes": nbytes, "seconds": best, "GBps_read_plus_write": 2 * nbytes / best / 1e9,
+            "what": "hipMemcpy device-to-device, best of 3 (host clock around each copy)"}
+
+
+def counter_child(n: int, k: int) -> int:
+    """Run under rocprofv3 by measure_traffic(): one whole-state row of the bench
+    configuration (k moduli), then one of the plain form (k=1), each cell launched
+    on its own."""
+    import ofp
+    from ofp import autotune
+    from ofp.path_bound import PRIMES, primes_for_n
+    mods = primes_for_n(n) if k == 0 else PRIMES[:k]
+    with ofp.Context(device=0) as ctx:
+        for kk in (len(mods), 1):
+            autotune.apply(ctx, n, kk)
+            ctx.set_height_caps(0)
+            ctx.prepare(n, mods[:kk])
+            ctx.time_cells(n)
+    print(json.dumps({"counter_child": "done", "n": n, "k": len(mods)}), flush=True)
+    return 0
+
+
+def measure_traffic(n: int, k: int, timeout: float) -> dict:
+    """HBM bytes per launch for each cell kernel, from the DRAM request counters of
+    one row run in a child process under rocprofv3 (before this process touches the
+    GPU).  Returns {"kernels": {name: {"read": B, "write": B, "dispatches": d}}} or
+    {"error": ...}."""
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(prof):
+        return {"error": "rocprofv3 not found"}
+    tmp = tempfile.mkdtemp(prefix="ofp_ctr_", dir="/tmp")
+    env = dict(os.environ, TMPDIR="/tmp")
+    cmd = [prof, "--pmc", *COUNTERS, "-d", tmp, "-o", "ctr", "--output-format", "csv", "--",
+           sys.executable, str(Path(__file__).resolve()), "--counter-child", "--n", str(n), "--k", str(k)]
+    log(f"bench: counters: {' '.join(cmd[:4])} ... (one row of n={n}, k={k} and k=1 under rocprofv3)")
+    try:
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout, env=env, cwd="/tmp")
+    except subprocess.TimeoutExpired:
+        shutil.rmtree(tmp, ignore_errors=True)
+        return {"error": f"rocprofv3 child exceeded {timeout:.0f} s"}
+    files = glob.glob(os.path.join(tmp, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        shutil.rmtree(tmp, ignore_errors=True)
+        return {"error": f"no counter output (rc={r.returncode}): {r.stderr[-400:]}"}
+    per: dict[int, dict] = {}
+    for path in files:
+        with open(path) as f:
+            for row in csv.DictReader(f):
+                d = int(row["Dispatch_Id"])
+                e = per.setdefault(d, {"kernel": row["Kernel_Name"]})
+                e[row["Counter_Name"]] = float(row["Counter_Value"])
+    shutil.rmtree(tmp, ignore_errors=True)
+    from ofp.rocprof_summary import kernel_family
+    out: dict[str, dict] = {}
+    for d in sorted(per):
+        e = per[d]
+        fam = kernel_family(e["kernel"])
+        if fam is None or not all(c in e for c in COUNTERS):
+            continue
+        o = out.setdefault(fam, {"read": 0.0, "write": 0.0, "dispatches": 0})
+        o["read"] += 32.0 * e[COUNTERS[0]]
+        o["write"] += 32.0 * e[COUNTERS[1]]
+        o["dispatches"] += 1
+    for o in out.values():
+        o["read"] /= o["dispatches"]
+        o["write"] /= o["dispatches"]
+    return {"kernels": out, "counters": list(COUNTERS), "unit": "bytes per dispatch (counter x 32 B)",
+            "rc": r.returncode}
+
+
+def kernel_rows(cell_times: list[list[tuple[int, float, str]]], kc: int, states: int, k: int,
+                traffic: dict | None) -> dict:
+    """Per-kernel statistics over rows of time_cells() output: one entry per cell
+    (one launch when k <= 16; a cell of k > 16 moduli sums its launches)."""
+    by: dict[str, list[float]] = {}
+    cells: dict[str, set] = {}
+    for row in cell_times:
+        for p, ms, name in row:
+            by.setdefault(name, []).append(ms)
+            cells.setdefault(name, set()).add(p)
+    declared = 8.0 * states * k
+    out = {}
+    for name, v in by.items():
+        avg = sum(v) / len(v)
+        ent = {"kernel": f"{name}<{kc}>", "cells": sorted(cells[name], reverse=True), "launches": len(v),
+               "avg_launch_ms": avg, "bytes_per_launch": declared, "achieved": declared / (avg * 1e-3) / 1e9,
+               "unit": "GB/s"}
+        ent["frac"] = ent["achieved"] / HBM_PEAK_GBS
+        t = (traffic or {}).get(f"{name}<{kc}>")
+        if t:
+            tb = t["read"] + t["write"]
+            ent.update({"traffic": tb, "traffic_read": t["read"], "traffic_write": t["write"],
+                        "traffic_GBps": tb / (avg * 1e-3) / 1e9,
+                        "traffic_frac": tb / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                        "traffic_over_declared": tb / declared})
+        out[name] = ent
+    return out
 
 
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=22, help="timed row sweeps")
-    ap.add_argument("--warmup", type=int, default=1, help="untimed row sweeps before timing")
+    ap.add_argument("--steps", type=int, default=10, help="timed full-work row sweeps")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed row sweeps before timing")
     ap.add_argument("--n", type=int, default=22)
     ap.add_argument("--k", type=int, default=0, help="moduli (0 = as many as the CRT bound needs)")
     ap.add_argument("--no-plain", action="store_true", help="skip the k=1 plain-form solve")
@@ -175,17 +326,18 @@ def main() -> int:
     ap.add_argument("--no-shard", action="store_true", help="skip the n=27 one-GPU-share measurement")
     ap.add_argument("--no-small", action="store_true",
                     help="skip the whole plain-form solves at n=12 and n=18 (k=1)")
-    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
-                    help="only run one device-to-device copy of BYTES (counter calibration)")
+    ap.add_argument("--no-counters", action="store_true", help="skip the rocprofv3 traffic measurement")
+    ap.add_argument("--counter-timeout", type=float, default=240.0)
+    ap.add_argument("--counter-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
-    if args.copy_calibration:
-        return copy_calibration(args.copy_calibration)
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(json.dumps({"error": "this benchmark measures one GPU: the frontier state of one solve "
                                    "lives in one GPU's HBM; run with --gpus 1"}), flush=True)
         return 2
+    if args.counter_child:
+        return counter_child(args.n, args.k)
 
     import ofp
     from ofp.path_bound import PRIMES, primes_for_n
@@ -198,86 +350,101 @@ def main() -> int:
     S = nm + nb
     log(f"bench: n={n} W={W} k={k} states={S} ({nm} main + {nb} blocked)")
 
+    traffic = None
+    if not args.no_counters:
+        traffic = measure_traffic(n, k, args.counter_timeout)
+        log(f"bench: counters {json.dumps(traffic)[:300]}")
+
     from ofp import autotune
     ctx = ofp.Context(device=0)
     tuned = autotune.apply(ctx, n, k)
     log(f"bench: launch config {tuned}")
     plan = ctx.plan(n, k)
     log(f"bench: state {plan.state_bytes / 2**30:.1f} GiB, free {plan.free_bytes / 2**30:.1f} GiB")
-    # first prepare allocates the state (the operating system may clear freshly mapped
-    # device pages first: allocation, not solve time); the timed solve starts from the
-    # resident state
-    ctx.prepare(n, mods)
-    tw0 = time.perf_counter()
-    ctx.prepare(n, mods)
-    if args.warmup:
-        ctx.row_sweep(args.warmup)
-        log(f"bench: warmup {args.warmup} rows done")
-    warm_s = time.perf_counter() - tw0
-    # The library synchronises its stream before returning from row_sweep, so the
-    # timed region is bracketed by device synchronisation on both sides.
-    t0 = time.perf_counter()
-    st = ctx.row_sweep(args.steps)
-    t1 = time.perf_counter()
-    el = t1 - t0
-    log(f"bench: {args.steps} rows in {el:.3f} s (kernel {st['kernel_s']:.3f} s, {st['launches']} launches)")
-    verified = None
-    residues = None
-    if args.warmup + args.steps == W:
-        residues = ctx.read_main(0, k)
-        if n in GOLDEN_FIRST and GOLDEN_FIRST[n][0] == mods[0]:
-            verified = residues[0] == GOLDEN_FIRST[n][1]
-        log(f"bench: residues {residues} verified_first={verified}")
 
-    transitions = st["transitions"]
-    value = transitions / el
-    ms_per_step = el / args.steps * 1e3
-    whole = args.warmup + args.steps == W
-    # a whole solve: initialisation + warmup + timed rows; otherwise extrapolated
-    solve_s = (warm_s + el) if whole else el / args.steps * W
+    # ---- one complete solve, all k moduli (the state memory is allocated first) ----
+    ctx.prepare(n, mods)
+    ts0 = time.perf_counter()
+    ctx.prepare(n, mods)
+    ss = ctx.row_sweep(W)
+    residues = ctx.read_main(0, k)
+    solve_s = time.perf_counter() - ts0
+    golden = golden_residues(n)
+    checked = {str(m): (r == golden[m]) for m, r in zip(mods, residues) if m in golden}
+    verified = (residues[0] == GOLDEN_FIRST[n][1]) if n in GOLDEN_FIRST and GOLDEN_FIRST[n][0] == mods[0] else None
     residues_per_hour = 3600.0 * k / solve_s
-    if st["full_launches"]:
-        avg_launch_s = st["full_kernel_s"] / st["full_launches"]
-        launch_basis = f"{st['full_launches']} full-state launches (rows without capped cells)"
-    else:
-        avg_launch_s = st["kernel_s"] / st["launches"]
-        launch_basis = f"all {st['launches']} launches (no row of the timed region ran uncapped)"
-    bytes_per_launch = 8.0 * S * min(k, 16)  # one launch carries up to 16 moduli
-    achieved = bytes_per_launch / avg_launch_s / 1e9
-    # HBM bytes per full-state launch as measured with rocprofv3 counters for this
-    # exact launch configuration (recorded with the tuned entry), else unknown
-    traffic, traffic_source = None, None
-    if tuned and tuned.get("hbm_bytes_per_full_launch"):
-        traffic = float(tuned["hbm_bytes_per_full_launch"])
-        traffic_source = tuned.get("hbm_bytes_source")
+    log(f"bench: whole solve {solve_s:.2f} s -> {residues_per_hour:.0f} residues/h; residues {residues}; "
+        f"first verified={verified}; fixtures {checked}")
+    capped = None
+    if ss["capped_launches"]:
+        # declared bytes of the capped launches: 8 B x k x (blocked words swept + main
+        # words whose heights stay within the cap), summed over the solve's capped cells
+        lib = ofp.load()
+        decl = 0.0
+        for r in range(W):
+            for p in range(W - 1, 0, -1):
+                cap, nbc = ofp.height_cap(n, r, p, tuned.get("cap_percent", 85) if tuned else 85)
+                if cap:
+                    decl += 8.0 * k * (nbc + capped_words(W, cap))
+        avg = ss["capped_kernel_s"] / ss["capped_launches"]
+        capped = {"kernel": f"cell_capped_kernel<{min(k, 16)}>", "launches": ss["capped_launches"],
+                  "kernel_s": ss["capped_kernel_s"], "avg_launch_ms": avg * 1e3,
+                  "bytes_total": decl, "achieved": decl / ss["capped_kernel_s"] / 1e9, "unit": "GB/s",
+                  "frac": decl / ss["capped_kernel_s"] / 1e9 / HBM_PEAK_GBS,
+                  "basis": "solve's capped rows (events around runs of capped rows); declared bytes per "
+                           "capped launch = 8 B x k x (blocked words swept + main words within the cap)"}
+        del lib
 
-    # plain one-residue-at-a-time form (k=1): one whole solve (state memory already held)
+    # ---- timed steps: full-work rows (height caps off), each cell between events ----
+    ctx.set_height_caps(0)
+    ctx.prepare(n, mods)
+    for _ in range(args.warmup):
+        ctx.time_cells(n)
+    log(f"bench: warmup {args.warmup} rows done")
+    t0 = time.perf_counter()
+    rows = [ctx.time_cells(n) for _ in range(args.steps)]
+    el = time.perf_counter() - t0
+    ctx.set_height_caps(True)
+    kernel_ms = sum(ms for row in rows for _, ms, _ in row) * 1e-3
+    transitions = args.steps * (W - 1) * S * k
+    value = transitions / el
+    log(f"bench: {args.steps} full rows in {el:.3f} s (kernel {kernel_ms:.3f} s)")
+    kc = min(k, 16)
+    kern = kernel_rows(rows, kc, S, k, (traffic or {}).get("kernels"))
+    dom = max(kern.values(), key=lambda e: e["avg_launch_ms"] * e["launches"])
+    roof = {"bound": "hbm", "achieved": dom["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": dom["frac"], "traffic": dom.get("traffic"), "kernel": dom["kernel"],
+            "bytes_per_launch": dom["bytes_per_launch"], "avg_launch_ms": dom["avg_launch_ms"],
+            "launches": dom["launches"],
+            "launch_basis": f"{dom['launches']} launches of the timed rows, HIP events on the library's stream",
+            "traffic_source": ("rocprofv3 --pmc " + " ".join(COUNTERS) + " x 32 B, one row in a child process "
+                               "of this run") if dom.get("traffic") else None}
+    if dom.get("traffic"):
+        roof["traffic_frac"] = dom["traffic_frac"]
+
+    # ---- plain one-residue-at-a-time form (k=1): one whole solve, then one timed row ----
     plain = None
     if not args.no_plain:
-        autotune.apply(ctx, n, 1)
+        tuned1 = autotune.apply(ctx, n, 1)
         ctx.prepare(n, [mods[0]])
         tp0 = time.perf_counter()
         ctx.prepare(n, [mods[0]])
         sp = ctx.row_sweep(W)
-        tp = time.perf_counter() - tp0
         pres = ctx.read_main(0, 1)
-        plain = {"solve_s": tp, "residues_per_hour": 3600.0 / tp,
-                 "transitions_per_s": sp["transitions"] / tp,
-                 "full_row_ms": (sp["full_kernel_s"] / sp["full_launches"] * (W - 1) * 1e3
-                                 if sp["full_launches"] else None),
-                 "residue": pres[0], "residue_matches_batched": residues is not None and pres[0] == residues[0]}
-        if sp["full_launches"]:
-            # the same declared bytes as the batched roofline, at one modulus; this form
-            # is bound by its integer index work, not by HBM
-            pl_s = sp["full_kernel_s"] / sp["full_launches"]
-            plain.update({"avg_launch_ms": pl_s * 1e3, "declared_GBps": 8.0 * S / pl_s / 1e9,
-                          "hbm_frac": 8.0 * S / pl_s / 1e9 / HBM_PEAK_GBS})
+        tp = time.perf_counter() - tp0
+        ctx.set_height_caps(0)
+        prow = [ctx.time_cells(n) for _ in range(2)]
+        autotune.apply(ctx, n, 1)
+        pk = kernel_rows(prow, 1, S, 1, (traffic or {}).get("kernels"))
+        plain = {"solve_s": tp, "residues_per_hour": 3600.0 / tp, "transitions_per_s": sp["transitions"] / tp,
+                 "residue": pres[0], "residue_matches_batched": pres[0] == residues[0],
+                 "full_row_ms": sum(ms for _, ms, _ in prow[-1]), "kernels": pk, "launch": tuned1}
         log(f"bench: plain k=1 solve {tp:.2f} s -> {plain['residues_per_hour']:.0f} residues/h "
             f"(residue {pres[0]})")
     ctx.close()
 
-    # whole plain-form solves of the small configurations (n=12, n=18, one modulus):
-    # one untimed solve (graph capture), then the median of three
+    copy = copy_rate(16 << 30)
+
     small = None
     if not args.no_small:
         small = {}
@@ -312,9 +479,10 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(n, k, args.cpu_seconds)
+        cpu = cpu_baseline(n, args.cpu_seconds)
         log(f"bench: cpu baseline {cpu['value']:.3e} transitions/s on {cpu['cores']} threads")
-        cpu["n12"] = cpu_solve_n12(PRIMES[0])
+        cpu["n12"] = cpu_solve(12, PRIMES[0], 3)
+        cpu["n16"] = cpu_solve(16, PRIMES[0], 1)
         if small:
             cpu["n12"]["residue_matches_gpu"] = cpu["n12"]["residue"] == small["n12"]["residue"]
 
@@ -325,29 +493,33 @@ def main() -> int:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": el / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic (the solve's own initial frontier; nothing loaded)",
-        "config": {"workload": f"exact CRT count, n={n} ({W}x{W} vertices), {k} moduli interleaved, "
-                               f"one row sweep per step", "n": n, "moduli": k, "states": S,
+        "config": {"workload": f"exact CRT count, n={n} ({W}x{W} vertices), {k} moduli interleaved; a step is "
+                               f"one full-work row sweep (height caps off)", "n": n, "moduli": k, "states": S,
                    "launch": tuned,
                    "main_states": nm, "blocked_states": nb, "state_bytes": plan.state_bytes,
                    "parallelism": "single GPU"},
+        "value_basis": "transitions of the timed full-work rows (W-1 cells x S states x k moduli each) / "
+                       "their wall time",
         "residues_per_hour": residues_per_hour,
+        "residues_per_hour_basis": "one measured whole solve (initialisation + all W rows, height caps on)",
         "solve_s": solve_s,
+        "solve_kernel_s": ss["kernel_s"],
         "verified_first_residue": verified,
         "residues": residues,
+        "moduli": mods,
+        "fixture_checks": checked,
         "plain_k1": plain,
         "batched_vs_plain_residues_per_hour": (residues_per_hour / plain["residues_per_hour"]) if plain else None,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "traffic_source": traffic_source,
-                     "kernel": "cell_kernel", "bytes_per_launch": bytes_per_launch,
-                     "avg_launch_ms": avg_launch_s * 1e3, "launch_basis": launch_basis,
-                     "capped_kernel_s": st["capped_kernel_s"], "capped_launches": st["capped_launches"]},
+        "roofline": roof,
+        "roofline_kernels": list(kern.values()) + ([capped] if capped else []),
+        "counter_run": {kk: vv for kk, vv in (traffic or {}).items() if kk != "kernels"} if traffic else None,
+        "hbm_copy": copy,
         "cpu_baseline": cpu,
         "plain_small": small,
         "shard_n27": shard,

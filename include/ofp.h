@@ -90,6 +90,20 @@ ofp_status ofp_plan(ofp_ctx* ctx, int n, int k, ofp_plan_info* out);
 ofp_status ofp_solve(ofp_ctx* ctx, int n, const uint32_t* moduli, int k,
                      uint32_t* residues_out, ofp_stats* stats);
 
+/* Moduli per sweep of ofp_solve.  A solve whose k-moduli state does not fit
+ * HBM (or exceeds max_moduli, when that is > 0) runs its moduli in several
+ * sweeps one after another, each of the largest count that fits, split as
+ * ofp_moduli_groups() says; the residues are the same.  0 (default) = as many
+ * as fit.  Reference analogue: its residue loop runs one modulus per sweep
+ * (row6crt20.cu:1784-1841). */
+ofp_status ofp_set_max_batch(ofp_ctx* ctx, int max_moduli);
+
+/* The sweeps ofp_solve makes of k moduli with at most max_per_sweep each:
+ * fewest sweeps, near-equal sizes, even sizes where that adds no sweep.
+ * Writes up to `capacity` sizes and returns the number of sweeps (-1 on bad
+ * arguments).  Host-only, no GPU. */
+int ofp_moduli_groups(int k, int max_per_sweep, int* sizes, int capacity);
+
 /* Apply cells p_hi..p_lo (W-1 >= p_hi >= p_lo >= 1) of one row to caller
  * state, on the device, in the product layout.  main_io holds
  * main_states*k residues and blocked_io blocked_states*k residues, both
@@ -107,8 +121,27 @@ ofp_status ofp_apply_cells(ofp_ctx* ctx, int n, const uint32_t* moduli, int k,
  * state is not re-initialised, so repeated calls advance it row by row. */
 ofp_status ofp_prepare(ofp_ctx* ctx, int n, const uint32_t* moduli, int k);
 ofp_status ofp_row_sweep(ofp_ctx* ctx, int rows, ofp_stats* stats);
-/* Read residues of main state `index` (k values) from the prepared state. */
-ofp_status ofp_read_main(ofp_ctx* ctx, uint64_t index, uint32_t* out_k);
+/* Read the residues of main state `index` from the prepared state: one per
+ * modulus of the last ofp_prepare() / ofp_solve(), in order.  `capacity` is
+ * the number of uint32 slots at out; fewer than the prepared moduli count is
+ * refused with OFP_E_ARG (nothing is written).  The answer word is index 0
+ * (rank_full(R), fullmate_dropN_batch.cu:514). */
+ofp_status ofp_read_main(ofp_ctx* ctx, uint64_t index, uint32_t* out, int capacity);
+
+/* One whole-state row of the prepared state with every cell launched on its
+ * own and timed with HIP events (no graph, no height caps): ms_out[i] and
+ * kind_out[i] for cell p = W-1-i, i < W-1 (kind 0 = cell_kernel, 1 =
+ * cell_sorted_kernel; a cell with several launches, k > 16, sums them).
+ * `capacity` slots each; fewer than W-1 is refused.  Advances the state by
+ * one row.  Used by the benchmark for per-kernel rooflines; no reference
+ * counterpart. */
+ofp_status ofp_time_cells(ofp_ctx* ctx, float* ms_out, int* kind_out, int capacity);
+
+/* Index width of the cell kernels: 0 = by n (32-bit ranks for n <= 22,
+ * 64-bit above), 64 = 64-bit ranks at every n.  Results do not depend on it;
+ * lets tests run the 64-bit kernels at sizes the CPU oracle checks.  No
+ * reference counterpart. */
+ofp_status ofp_set_index_width(ofp_ctx* ctx, int bits);
 
 /* Kernel launch tuning (threads per workgroup, workgroups per CU); set by
  * the autotuner, 0 = default.  Reference analogue: GRIDFP_FRONTIER_THREADS /

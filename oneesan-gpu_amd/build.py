@@ -4,8 +4,6 @@ Produces, in-tree:
   oneesan-gpu_amd/lib/libofp.so    the C-ABI library (HIP kernels + host runtime)
   oneesan-gpu_amd/lib/ofp_solver   the CLI (reference argv / result-line contract)
   oneesan-gpu_amd/lib/ofp_solver.provenance.json  its build provenance sidecar
-  oneesan-gpu_amd/lib/ofp_hbm_probe  streaming read / copy / read-modify-write kernels
-                                     (the HBM ceiling the cell kernels are compared with)
 
 hipcc cross-compiles for gfx950 without a GPU.  Every HIP translation unit is
 built with packed-FP32 code generation disabled.
@@ -85,9 +83,6 @@ def build(force: bool = True, jobs: int = 4) -> Path:
         sys.path.insert(0, str(PKG))
         from ofp import provenance
         provenance.create_for_build(cli, compile_args=HIP_FLAGS, compiler=HIPCC)
-    probe_src = CSRC / "hbm_probe.hip"
-    if probe_src.exists():
-        _run([HIPCC, *HIP_FLAGS, str(probe_src), "-o", str(LIB / "ofp_hbm_probe")])
     return so
 
 

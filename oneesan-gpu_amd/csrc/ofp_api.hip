@@ -66,6 +66,8 @@ struct ofp_ctx {
     int xcd_shift = 5;     // chunk pieces per XCD: 2^xcd_shift chunks, < 0 = one eighth each
     int sorted_from = -1;  // full-state cells p >= sorted_from run the sorted kernel (0: none,
                            // -1: from cell 6 for launches of 8 or more moduli, else none)
+    int index_bits = 0;    // 0: 32-bit ranks for W <= 23, 64-bit above; 64: 64-bit ranks always
+    int max_batch = 0;     // moduli per sweep of ofp_solve (0: as many as fit HBM)
     std::string err;
     Sweep st;              // the context's own state (solve / prepare / row_sweep)
     int jobs = 0;          // live jobs (serving)
@@ -275,14 +277,16 @@ int cap_pct(const ofp_ctx* c, const Sweep& s) {
 // Enqueue cells p_hi..p_lo of s on s.stream; row >= 0 names the row of a sweep
 // from the initial state (height caps apply), -1 an arbitrary state.
 ofp_status enqueue_cells(ofp_ctx* c, Sweep& s, int p_hi, int p_lo, bool first_reads_blk, int* launches,
-                         int row = -1) {
-    const bool wide = s.W > 23;
+                         int row = -1, hipEvent_t* cell_ev = nullptr, int* cell_kind = nullptr) {
+    const bool wide = s.W > 23 || c->index_bits == 64;
     const int blocks = grid_blocks(c, s.nblk);
     const int pct = cap_pct(c, s);
     const bool caps = pct > 0 && row >= 0 && s.layout == 1;
     for (int p = p_hi; p >= p_lo; --p) {
         uint64_t ncap = 0;
         const int cap = caps ? effective_cap(s.W, row, p, s.nblk, &ncap, pct) : 0;
+        // per-cell timing (ofp_time_cells): events around each cell's launches
+        if (cell_ev) OFP_HIP(c, hipEventRecord(cell_ev[2 * (p_hi - p)], s.stream), "event record");
         // moduli per launch: all (up to 16) at once, or fewer on the low cells,
         // whose scattered closure reads are latency-bound (fewer registers per
         // thread -> more waves in flight).  Chunks keep an even offset.
@@ -321,7 +325,9 @@ ofp_status enqueue_cells(ofp_ctx* c, Sweep& s, int p_hi, int p_lo, bool first_re
             }
             if (e != hipSuccess) return hip_fail(c, e, "cell kernel launch");
             if (launches) ++*launches;
+            if (cell_kind) cell_kind[p_hi - p] = a.variant == 4 ? 1 : 0;
         }
+        if (cell_ev) OFP_HIP(c, hipEventRecord(cell_ev[2 * (p_hi - p) + 1], s.stream), "event record");
     }
     return OFP_OK;
 }
@@ -337,9 +343,9 @@ ofp_status full_rows(ofp_ctx* c, Sweep& s, int rows, int* launches) {
         return OFP_OK;
     }
     char key[256];
-    snprintf(key, sizeof key, "%p/%p/%d/%d/%d/%d/%d/%d/%d/%d/%d", (void*)s.main, (void*)s.blk, s.n, s.K,
+    snprintf(key, sizeof key, "%p/%p/%d/%d/%d/%d/%d/%d/%d/%d/%d/%d", (void*)s.main, (void*)s.blk, s.n, s.K,
              c->threads, c->blocks_per_cu, s.layout, c->low_cells, c->low_kc, c->variant * 100 + c->sorted_from,
-             c->xcd_shift);
+             c->xcd_shift, c->index_bits);
     std::string k = key;
     for (uint32_t m : s.mods) k += "/" + std::to_string(m);
     if (k != s.row_key || !s.row_exec) {
@@ -455,9 +461,29 @@ ofp_status init_state(ofp_ctx* c, Sweep& s) {
             "zero blocked state");
     s.row = 0;
     uint64_t start = dp_at(s.W - 1, 1);
-    hipError_t e = launch_set_one(s.main, start, s.Kp, s.K, s.layout, s.mods_dev, s.stream);
+    hipError_t e = s.mods_dev ? launch_set_one(s.main, start, s.Kp, s.K, s.layout, s.mods_dev, s.stream)
+                              : launch_set_one_values(s.main, start, s.Kp, s.K, s.layout, s.mods.data(), s.stream);
     if (e != hipSuccess) return hip_fail(c, e, "init kernel");
     return OFP_OK;
+}
+
+// Split k moduli into sweeps of at most g each (g >= 1): as few sweeps as
+// possible, sizes as equal as possible, and even sizes where that costs no
+// sweep (the state-major stride is padded to even, so an odd sweep carries
+// one idle slot).  Returns the number of sweeps; sizes[] gets the first cap.
+int moduli_groups(int k, int g, int* sizes, int cap) {
+    if (k < 1 || g < 1) return 0;
+    const int groups = (k + g - 1) / g;
+    int per = (k + groups - 1) / groups;
+    if ((per & 1) && per + 1 <= g && (k + per) / (per + 1) <= groups) ++per;
+    int left = k, i = 0;
+    while (left > 0) {
+        const int now = std::min(per, left);
+        if (sizes && i < cap) sizes[i] = now;
+        left -= now;
+        ++i;
+    }
+    return i;
 }
 
 // Host rank / unrank over the height table (lexicographic, N<R<L, entered at height 1).
@@ -505,9 +531,12 @@ void class_start(int W, int lc, uint64_t r, uint64_t* b_first, uint64_t* m_first
     *m_first = rank_word((prefix << 2) | min_suffix(lc + 1, h), W);
 }
 
-ofp_status read_main(ofp_ctx* c, Sweep& s, uint64_t index, uint32_t* out_k) {
+ofp_status read_main(ofp_ctx* c, Sweep& s, uint64_t index, uint32_t* out_k, int capacity) {
     if (!s.main) return fail(c, OFP_E_ARG, "no prepared state");
     if (index >= s.nmain) return fail(c, OFP_E_ARG, "index out of range");
+    if (capacity < s.K)
+        return fail(c, OFP_E_ARG, "output holds " + std::to_string(capacity) + " residues, the state has " +
+                                      std::to_string(s.K) + " moduli");
     OFP_HIP(c, hipStreamSynchronize(s.stream), "sync");
     for (int j = 0; j < s.K; ++j)
         OFP_HIP(c, hipMemcpy(out_k + j, s.main + state_elem(s.layout, index, s.Kp, j), 4, hipMemcpyDeviceToHost),
@@ -738,30 +767,109 @@ ofp_status ofp_row_sweep(ofp_ctx* c, int rows, ofp_stats* st) {
     return OFP_OK;
 }
 
-ofp_status ofp_read_main(ofp_ctx* c, uint64_t index, uint32_t* out_k) {
-    if (!c || !out_k) return OFP_E_ARG;
+ofp_status ofp_read_main(ofp_ctx* c, uint64_t index, uint32_t* out, int capacity) {
+    if (!c || !out) return OFP_E_ARG;
     if (c->st.layout != c->layout) return fail(c, OFP_E_ARG, "no prepared state");
-    return read_main(c, c->st, index, out_k);
+    return read_main(c, c->st, index, out, capacity);
+}
+
+ofp_status ofp_time_cells(ofp_ctx* c, float* ms_out, int* kind_out, int capacity) {
+    if (!c || !ms_out || !kind_out) return OFP_E_ARG;
+    Sweep& s = c->st;
+    if (!s.main || s.layout != c->layout) return fail(c, OFP_E_ARG, "no prepared state");
+    const int cells = s.W - 1;
+    if (capacity < cells) return fail(c, OFP_E_ARG, "capacity below the W-1 cells of a row");
+    while ((int)s.seg_ev.size() < 2 * cells) {
+        hipEvent_t e = nullptr;
+        OFP_HIP(c, hipEventCreate(&e), "event create");
+        s.seg_ev.push_back(e);
+    }
+    for (int i = 0; i < cells; ++i) kind_out[i] = 0;
+    // row -1: an arbitrary state, every cell over the whole state
+    ofp_status st = enqueue_cells(c, s, s.W - 1, 1, false, nullptr, -1, s.seg_ev.data(), kind_out);
+    if (st) return st;
+    OFP_HIP(c, hipEventSynchronize(s.seg_ev[2 * cells - 1]), "time cells sync");
+    for (int i = 0; i < cells; ++i) {
+        float ms = 0;
+        OFP_HIP(c, hipEventElapsedTime(&ms, s.seg_ev[2 * i], s.seg_ev[2 * i + 1]), "event time");
+        ms_out[i] = ms;
+    }
+    if (s.row >= 0) s.row += 1;
+    return OFP_OK;
+}
+
+ofp_status ofp_set_index_width(ofp_ctx* c, int bits) {
+    if (!c) return OFP_E_ARG;
+    if (bits != 0 && bits != 64) return fail(c, OFP_E_ARG, "index width must be 0 (by n) or 64");
+    if (bits != c->index_bits) drop_graph(c->st);
+    c->index_bits = bits;
+    return OFP_OK;
 }
 
 ofp_status ofp_solve(ofp_ctx* c, int n, const uint32_t* moduli, int k, uint32_t* residues_out,
                      ofp_stats* st) {
     if (!c || !moduli || !residues_out) return OFP_E_ARG;
     auto t0 = std::chrono::steady_clock::now();
-    ofp_status s = ensure_state(c, c->st, n, moduli, k);
-    if (s) return s;
-    s = init_state(c, c->st);
-    if (s) return s;
-    ofp_stats rs{};
-    s = ofp_row_sweep(c, c->st.W, &rs);
-    if (s) return s;
-    s = read_main(c, c->st, 0, residues_out);  // rank_full(R) = 0 (fullmate_dropN_batch.cu:508,514)
-    if (s) return s;
+    uint64_t nm, nb;
+    if (!counts(n, &nm, &nb)) return fail(c, OFP_E_RANGE, "n must be in 1..27");
+    if (k < 1 || k > 1024) return fail(c, OFP_E_ARG, "number of moduli must be in 1..1024");
+    // Moduli per sweep: all of them when their interleaved state fits HBM (memory this
+    // context already holds counts as free), else the largest count that fits, the
+    // moduli then running in several sweeps one after another -- what the reference
+    // does one modulus at a time (row6crt20.cu:1784).
+    size_t fr = 0, tot = 0;
+    OFP_HIP(c, hipSetDevice(c->device), "hipSetDevice");
+    OFP_HIP(c, hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
+    const uint64_t avail = fr + c->st.main_cap + c->st.blk_cap;
+    int g = c->max_batch > 0 ? std::min(k, c->max_batch) : k;
+    while (g > 1 && state_bytes(c->layout, n, g) + c->reserve > avail) --g;
+    if (state_bytes(c->layout, n, g) + c->reserve > avail)
+        return fail(c, OFP_E_MEMORY, "insufficient HBM: one modulus needs " +
+                                         std::to_string(state_bytes(c->layout, n, 1)) + " bytes, free " +
+                                         std::to_string(avail));
+    std::vector<int> sizes((size_t)k);
+    const int sweeps = moduli_groups(k, g, sizes.data(), k);
+    ofp_stats acc{};
+    for (int i = 0, off = 0; i < sweeps; off += sizes[i], ++i) {
+        ofp_status s = ensure_state(c, c->st, n, moduli + off, sizes[i]);
+        if (s) return s;
+        s = init_state(c, c->st);
+        if (s) return s;
+        ofp_stats rs{};
+        s = ofp_row_sweep(c, c->st.W, &rs);
+        if (s) return s;
+        s = read_main(c, c->st, 0, residues_out + off, sizes[i]);  // rank_full(R) = 0 (fullmate_dropN_batch.cu:508,514)
+        if (s) return s;
+        if (i == 0) {
+            acc = rs;
+        } else {
+            acc.kernel_s += rs.kernel_s;
+            acc.transitions += rs.transitions;
+            acc.rows += rs.rows;
+            acc.launches += rs.launches;
+            acc.full_kernel_s += rs.full_kernel_s;
+            acc.capped_kernel_s += rs.capped_kernel_s;
+            acc.full_launches += rs.full_launches;
+            acc.capped_launches += rs.capped_launches;
+        }
+    }
     if (st) {
-        *st = rs;
+        *st = acc;
         st->wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
     return OFP_OK;
+}
+
+ofp_status ofp_set_max_batch(ofp_ctx* c, int max_moduli) {
+    if (!c) return OFP_E_ARG;
+    if (max_moduli < 0 || max_moduli > 1024) return fail(c, OFP_E_ARG, "max_moduli must be 0..1024");
+    c->max_batch = max_moduli;
+    return OFP_OK;
+}
+
+int ofp_moduli_groups(int k, int max_per_sweep, int* sizes, int capacity) {
+    if (k < 1 || max_per_sweep < 1 || capacity < 0) return -1;
+    return moduli_groups(k, max_per_sweep, sizes, capacity);
 }
 
 ofp_status ofp_apply_cells(ofp_ctx* c, int n, const uint32_t* moduli, int k, uint32_t* main_io,
@@ -862,7 +970,7 @@ ofp_status ofp_shard_sweep(ofp_ctx* c, const ofp_shard_spec* sp, ofp_shard_info*
     hipError_t e = launch_fill_remote(dm, info->first_main, M, db, info->first_blocked, B, Kp, k, dmods,
                                       sp->seed, c->stream);
     if (e != hipSuccess) { release(); return hip_fail(c, e, "fill shard window"); }
-    const bool wide = W > 23;
+    const bool wide = W > 23 || c->index_bits == 64;
     const int blocks = grid_blocks(c, B);
     OFP_HIP_R(hipEventRecord(c->ev0, c->stream), "event");
     int launches = 0;
@@ -980,7 +1088,7 @@ ofp_status ofp_job_info_get(ofp_job* j, ofp_job_info* info) {
 ofp_status ofp_job_result(ofp_job* j, uint32_t* residues_out) {
     if (!j || !residues_out) return OFP_E_ARG;
     if (j->rows_enqueued != j->sw.W) return fail(j->ctx, OFP_E_ARG, "job has not run all W rows");
-    return read_main(j->ctx, j->sw, 0, residues_out);
+    return read_main(j->ctx, j->sw, 0, residues_out, j->sw.K);
 }
 
 void ofp_job_destroy(ofp_job* j) {

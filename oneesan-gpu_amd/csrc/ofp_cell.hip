@@ -310,6 +310,23 @@ hipError_t launch_fill_remote(uint32_t* main, uint64_t m0, uint64_t M, uint32_t*
     return hipGetLastError();
 }
 
+struct ModVals {
+    uint32_t m[kMaxKC];
+};
+__global__ void set_one_values_kernel(uint32_t* main, uint64_t index, int K, int k, int layout, ModVals mods) {
+    int j = threadIdx.x;
+    if (j < k) main[state_elem(layout, index, K, j)] = 1u % mods.m[j];
+}
+
+hipError_t launch_set_one_values(uint32_t* main, uint64_t index, int K, int k, int layout,
+                                 const uint32_t* mods_host, hipStream_t s) {
+    if (k < 1 || k > kMaxKC || K < k) return hipErrorInvalidValue;
+    ModVals mv{};
+    for (int j = 0; j < k; ++j) mv.m[j] = mods_host[j];
+    hipLaunchKernelGGL(set_one_values_kernel, dim3(1), dim3(64), 0, s, main, index, K, k, layout, mv);
+    return hipGetLastError();
+}
+
 hipError_t launch_set_one(uint32_t* main, uint64_t index, int K, int k, int layout,
                           const uint32_t* mods_dev, hipStream_t s) {
     if (k < 1 || k > 1024 || K < k) return hipErrorInvalidValue;

@@ -9,11 +9,13 @@
 // order, beginning with backend= and carrying n=, residue=, modulus=,
 // wall_s= exactly once each and in that order (row6crt20.cu:1841).
 //
-// Differences, by design: all moduli are computed together in one
-// interleaved sweep, so each line reports wall_s = batch wall time / k (its
-// share, so that summing wall_s over a batch gives the solver's real time, as
-// the exact runner's manifest check expects) and batch_seconds = the whole
-// batch; target_mib and max_window are
+// Differences, by design: the moduli are computed together in interleaved
+// sweeps -- all in one when their state fits HBM, else in sweeps of the largest
+// count that fits, one after another (the library's ofp_solve; the environment
+// variable OFP_BATCH_MODULI caps the count per sweep) -- so each line reports
+// wall_s = the whole run's wall time / k (its share, so that summing wall_s over a
+// run gives the solver's real time, as the exact runner's manifest check expects)
+// and batch_seconds = the whole run; target_mib and max_window are
 // validated and echoed but do not change the sweep (there is no scratch
 // window: the state is updated in place); one GPU is used.
 // Exit codes: 1 bad arguments / runtime error, 2 no GPU, 5 state does not fit
@@ -98,6 +100,14 @@ int main(int argc, char** argv) {
         std::cerr << "ofp_create failed (" << st << ")\n";
         return 1;
     }
+    if (const char* e = std::getenv("OFP_BATCH_MODULI")) {
+        long v;
+        if (!parse_int(e, 0, 1024, &v) || ofp_set_max_batch(ctx, (int)v) != OFP_OK) {
+            std::cerr << "OFP_BATCH_MODULI must be an integer in 0..1024\n";
+            ofp_destroy(ctx);
+            return 1;
+        }
+    }
     ofp_plan_info plan{};
     if (ofp_plan(ctx, (int)n, (int)mods.size(), &plan) != OFP_OK) {
         std::cerr << ofp_last_error(ctx) << "\n";
@@ -108,9 +118,13 @@ int main(int argc, char** argv) {
               << " moduli=" << mods.size() << " state_gib=" << double(plan.state_bytes) / (1ull << 30)
               << " free_gib=" << double(plan.free_bytes) / (1ull << 30) << "\n";
     if (!plan.fits) {
-        std::cerr << "insufficient HBM for the interleaved state\n";
-        ofp_destroy(ctx);
-        return 5;
+        ofp_plan_info one{};
+        if (ofp_plan(ctx, (int)n, 1, &one) != OFP_OK || !one.fits) {
+            std::cerr << "insufficient HBM for the state of one modulus\n";
+            ofp_destroy(ctx);
+            return 5;
+        }
+        std::cerr << "the state of all moduli does not fit: they run in several sweeps\n";
     }
     std::vector<uint32_t> res(mods.size());
     ofp_stats stats{};
@@ -120,13 +134,15 @@ int main(int argc, char** argv) {
         ofp_destroy(ctx);
         return st == OFP_E_MEMORY ? 5 : 1;
     }
+    const size_t sweeps = stats.rows > 0 ? (size_t)stats.rows / (size_t)(n + 1) : 1;
     for (size_t j = 0; j < mods.size(); ++j) {
         std::cout << "backend=ofp-mi355x-hbm32-interleaved n=" << n << " residue=" << res[j]
                   << " modulus=" << mods[j] << " residue_index=" << j << " residues_total=" << mods.size()
                   << " gpus=1 peers=0 main_states=" << plan.main_states
                   << " blocked_states=" << plan.blocked_states << " scratch_target_mib=" << target_mib
                   << " max_window=" << max_window << " requested_gpus=" << requested
-                  << " batch_moduli=" << mods.size() << " transitions=" << stats.transitions
+                  << " batch_moduli=" << (mods.size() + sweeps - 1) / sweeps << " sweeps=" << sweeps
+                  << " transitions=" << stats.transitions
                   << " kernel_s=" << stats.kernel_s << " launches=" << stats.launches
                   << " batch_seconds=" << stats.wall_s
                   << " wall_s=" << stats.wall_s / double(mods.size()) << std::endl;

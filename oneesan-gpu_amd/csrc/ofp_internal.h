@@ -84,5 +84,8 @@ hipError_t launch_fill_remote(uint32_t* main, uint64_t m0, uint64_t M, uint32_t*
 // Set main[(index, j)] = 1 mod moduli[j] for j < k (interleave stride K).
 hipError_t launch_set_one(uint32_t* main, uint64_t index, int K, int k, int layout,
                           const uint32_t* mods_dev, hipStream_t s);
+// The same with the moduli passed by value from host memory (k <= kMaxKC).
+hipError_t launch_set_one_values(uint32_t* main, uint64_t index, int K, int k, int layout,
+                                 const uint32_t* mods_host, hipStream_t s);
 
 }  // namespace ofp

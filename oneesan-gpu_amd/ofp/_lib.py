@@ -29,7 +29,8 @@ EXPORTS = (
     "ofp_last_error", "ofp_version", "ofp_destroy",
     "ofp_job_create", "ofp_job_enqueue", "ofp_job_info_get", "ofp_job_result", "ofp_job_destroy",
     "ofp_shard_plan", "ofp_shard_sweep", "ofp_set_height_caps", "ofp_height_cap", "ofp_set_xcd_pieces",
-    "ofp_set_sorted_cells",
+    "ofp_set_sorted_cells", "ofp_time_cells", "ofp_set_index_width", "ofp_set_max_batch",
+    "ofp_moduli_groups",
 )
 
 
@@ -146,7 +147,13 @@ def load() -> ctypes.CDLL:
                                              ctypes.c_int, ctypes.c_int, ctypes.c_int]),
         "ofp_prepare": (ctypes.c_int32, [vp, ctypes.c_int, u32p, ctypes.c_int]),
         "ofp_row_sweep": (ctypes.c_int32, [vp, ctypes.c_int, ctypes.POINTER(Stats)]),
-        "ofp_read_main": (ctypes.c_int32, [vp, ctypes.c_uint64, u32p]),
+        "ofp_read_main": (ctypes.c_int32, [vp, ctypes.c_uint64, u32p, ctypes.c_int]),
+        "ofp_time_cells": (ctypes.c_int32, [vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int),
+                                            ctypes.c_int]),
+        "ofp_set_index_width": (ctypes.c_int32, [vp, ctypes.c_int]),
+        "ofp_set_max_batch": (ctypes.c_int32, [vp, ctypes.c_int]),
+        "ofp_moduli_groups": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                             ctypes.c_int]),
         "ofp_set_launch": (ctypes.c_int32, [vp, ctypes.c_int, ctypes.c_int]),
         "ofp_set_layout": (ctypes.c_int32, [vp, ctypes.c_int]),
         "ofp_set_low_cells": (ctypes.c_int32, [vp, ctypes.c_int, ctypes.c_int]),
@@ -192,6 +199,17 @@ def state_counts(n: int) -> tuple[int, int]:
 
 
 DEFAULT_CAP_PERCENT = 85
+
+
+def moduli_groups(k: int, max_per_sweep: int) -> list[int]:
+    """Sizes of the sweeps a solve of k moduli makes with at most max_per_sweep each
+    (host computation, no GPU needed)."""
+    lib = load()
+    buf = (ctypes.c_int * max(k, 1))()
+    cnt = lib.ofp_moduli_groups(k, max_per_sweep, buf, max(k, 1))
+    if cnt < 0:
+        raise OfpError(-1, f"moduli_groups(k={k}, max_per_sweep={max_per_sweep})")
+    return list(buf[:cnt])
 
 
 def height_cap(n: int, row: int, p: int, max_percent: int = DEFAULT_CAP_PERCENT) -> tuple[int, int]:
@@ -292,9 +310,29 @@ class Context:
         return stats.as_dict()
 
     def read_main(self, index: int, k: int) -> list[int]:
+        """Residues of main word `index` of the prepared state; k must be its moduli
+        count (the library refuses a smaller buffer rather than overrun it)."""
         out = (ctypes.c_uint32 * k)()
-        self._check(self._lib.ofp_read_main(self._h, index, out), "ofp_read_main")
+        self._check(self._lib.ofp_read_main(self._h, index, out, k), "ofp_read_main")
         return list(out)
+
+    def time_cells(self, n: int) -> list[tuple[int, float, str]]:
+        """One whole-state row of the prepared n-state, each cell launched and timed on
+        its own: [(p, ms, kernel)] for p = n..1 (advances the state by one row)."""
+        cells = n
+        ms = (ctypes.c_float * cells)()
+        kind = (ctypes.c_int * cells)()
+        self._check(self._lib.ofp_time_cells(self._h, ms, kind, cells), "ofp_time_cells")
+        names = ("cell_kernel", "cell_sorted_kernel")
+        return [(cells - i, float(ms[i]), names[kind[i]]) for i in range(cells)]
+
+    def set_index_width(self, bits: int) -> None:
+        """0 = by n (32-bit ranks up to n=22), 64 = 64-bit ranks at every n."""
+        self._check(self._lib.ofp_set_index_width(self._h, bits), "ofp_set_index_width")
+
+    def set_max_batch(self, max_moduli: int) -> None:
+        """At most max_moduli moduli per sweep of solve() (0 = as many as fit HBM)."""
+        self._check(self._lib.ofp_set_max_batch(self._h, max_moduli), "ofp_set_max_batch")
 
     def set_launch(self, threads: int, blocks_per_cu: int) -> None:
         self._check(self._lib.ofp_set_launch(self._h, threads, blocks_per_cu), "ofp_set_launch")

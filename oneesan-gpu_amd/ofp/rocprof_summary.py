@@ -48,6 +48,13 @@ def _kc(name: str) -> int | None:
     return int(m.group(1)) if m else None
 
 
+def kernel_family(name: str) -> str | None:
+    """'cell_sorted_kernel<14>' etc. for a cell kernel's full name (index width and
+    other template arguments dropped), None for any other kernel."""
+    m = re.search(r"(cell_(?:capped_|sorted_|shard_)?kernel)<(\d+)", name)
+    return f"{m.group(1)}<{m.group(2)}>" if m else None
+
+
 MAX_ROWS = None  # only the first rows of each moduli count (--rows); None = all
 
 

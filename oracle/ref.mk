@@ -6,10 +6,12 @@
 #                           gridfp_reverse.hpp behind ref_gridfp_harness.cpp
 #   _ref/ggcount_single   : third_party/ggcount/src/ggcount.cpp (MIT), an
 #                           independent CPU frontier DP (Iwashita et al. 2013)
+#   _ref/ggcount_omp      : the same source built with its own OpenMP loops
+#                           (used for the n=20..22 fixtures)
 REF ?= /root/reference
 OUT := _ref
 
-all: $(OUT)/libref_gridfp.so $(OUT)/ggcount_single
+all: $(OUT)/libref_gridfp.so $(OUT)/ggcount_single $(OUT)/ggcount_omp
 
 $(OUT):
 	mkdir -p $(OUT)
@@ -19,5 +21,8 @@ $(OUT)/libref_gridfp.so: ref_gridfp_harness.cpp $(REF)/src/common/gridfp_transit
 
 $(OUT)/ggcount_single: $(REF)/third_party/ggcount/src/ggcount.cpp | $(OUT)
 	g++ -std=c++11 -DNDEBUG -O3 $< -o $@
+
+$(OUT)/ggcount_omp: $(REF)/third_party/ggcount/src/ggcount.cpp | $(OUT)
+	g++ -std=c++11 -DNDEBUG -O3 -fopenmp $< -o $@
 
 .PHONY: all

@@ -71,3 +71,44 @@ def test_cli_rejects_bad_arguments(argv, msg):
     assert res.returncode == 1
     assert msg in res.stderr
     assert res.stdout == ""
+
+
+def test_read_main_takes_a_capacity():
+    """ofp_read_main carries the caller's buffer size, so a buffer shorter than the
+    prepared moduli count is refused instead of overrun (the GPU side of this is
+    tests/test_gpu_parity.py::test_read_main_refuses_short_buffer)."""
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    decl = re.search(r"ofp_read_main\s*\(([^)]*)\)", text).group(1)
+    assert [a.split()[-1].lstrip("*") for a in decl.split(",")] == ["ctx", "index", "out", "capacity"]
+
+
+@pytest.mark.parametrize("k,g,want", [
+    (14, 14, [14]),
+    (14, 16, [14]),
+    (15, 8, [8, 7]),
+    (15, 9, [8, 7]),
+    (14, 8, [8, 6]),
+    (14, 7, [7, 7]),
+    (14, 6, [6, 6, 2]),
+    (20, 6, [6, 6, 6, 2]),
+    (13, 4, [4, 4, 4, 1]),
+    (5, 2, [2, 2, 1]),
+    (1, 1, [1]),
+    (7, 1, [1] * 7),
+    (9, 5, [5, 4]),
+])
+def test_moduli_groups(k, g, want):
+    """Moduli that do not fit one sweep: fewest sweeps of at most g, near-equal
+    sizes, even sizes where that adds no sweep."""
+    import ofp
+    got = ofp.moduli_groups(k, g)
+    assert got == want
+    assert sum(got) == k and max(got) <= g
+    assert len(got) == -(-k // g)
+
+
+def test_moduli_groups_rejects_bad_arguments():
+    import ofp
+    for k, g in ((0, 4), (4, 0), (-1, 2)):
+        with pytest.raises(ofp.OfpError):
+            ofp.moduli_groups(k, g)

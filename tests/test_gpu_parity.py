@@ -217,3 +217,75 @@ def test_largest_reference_residues(gpu_ctx, n):
     print(f"case reference n={n} (whole solve, one modulus)", flush=True)
     got, st = gpu_ctx.solve(n, [pairs[0][0]])
     assert got == [pairs[0][1]], (n, got)
+
+
+PRIMES16 = [4294967291, 4294967279, 4294967231, 4294967197, 4294967189, 4294967161, 4294967143,
+            4294967111, 4294967087, 4294967029, 4294966997, 4294966981, 4294966943, 4294966927,
+            4294966909, 4294966877]
+
+
+@pytest.mark.parametrize("k", [1, 8, 14])
+@pytest.mark.parametrize("sorted_cells", [False, True], ids=["unsorted", "sorted"])
+def test_64bit_ranks_every_cell(gpu_ctx, oracle, k, sorted_cells):
+    """The 64-bit-rank kernels (the path n >= 23 takes) forced at small n: every
+    cell on random vectors, k moduli interleaved, unsorted (cell_kernel<k, uint64>)
+    and sorted (cell_sorted_kernel<k, uint64>, the default of n >= 23 with 8 or more
+    moduli), against the CPU forward recurrence."""
+    import ofp
+    mods = ([2] + PRIMES16)[:k] if k > 1 else [4294967291]
+    try:
+        gpu_ctx.set_index_width(64)
+        gpu_ctx.set_sorted_cells(1 if sorted_cells else 0)
+        for n in (7, 10, 11):
+            nm, nb = ofp.state_counts(n)
+            W = n + 1
+            rng = np.random.default_rng(0x64B17 + 97 * n + k)
+            cells = range(W - 1, 0, -1) if n < 11 else (W - 1, W // 2, 3, 2, 1)
+            for p in cells:
+                main = np.stack([rng.integers(0, m, nm, dtype=np.uint64) for m in mods], 1).astype(np.uint32)
+                blk = np.stack([rng.integers(0, m, nb, dtype=np.uint64) for m in mods], 1).astype(np.uint32)
+                g_main, g_blk = main.copy(), blk.copy()
+                print(f"case 64-bit n={n} k={k} p={p} sorted={sorted_cells}", flush=True)
+                gpu_ctx.apply_cells(n, mods, g_main, g_blk, p, p, True)
+                for j in sorted({0, k // 2, k - 1}):
+                    em, eb = _oracle_cells(oracle, n, mods[j], main[:, j], blk[:, j], p, p, True)
+                    assert np.array_equal(g_main[:, j], em), (n, k, p, j, "main")
+                    assert np.array_equal(g_blk[:, j], eb), (n, k, p, j, "blocked")
+    finally:
+        gpu_ctx.set_index_width(0)
+        gpu_ctx.set_sorted_cells(-1)
+
+
+def test_64bit_ranks_whole_solves(gpu_ctx):
+    """Whole solves with 64-bit ranks forced (capped, unsorted and sorted kernels):
+    residues equal the 32-bit path's and the exact counts."""
+    mods = PRIMES16[:9]
+    try:
+        for n in (9, 12, 15):
+            gpu_ctx.set_index_width(0)
+            want, _ = gpu_ctx.solve(n, mods)
+            gpu_ctx.set_index_width(64)
+            print(f"case 64-bit whole solve n={n} k={len(mods)}", flush=True)
+            got, st = gpu_ctx.solve(n, mods)
+            assert got == want, n
+        exact = int(load_golden("exact_counts.json")["counts"]["12"])
+        gpu_ctx.set_index_width(64)
+        got, _ = gpu_ctx.solve(12, mods)
+        assert got == [exact % m for m in mods]
+    finally:
+        gpu_ctx.set_index_width(0)
+
+
+def test_n23_two_moduli():
+    """n=23 (64-bit ranks, 7.8e9 states) with two moduli interleaved: the first
+    residue is the reference's (docs/b300-hbm32.md:382-412), the second equals a
+    one-modulus solve."""
+    import ofp
+    mods = [4294967291, 4294967279]
+    with ofp.Context() as ctx:
+        print("case n=23 k=2 interleaved", flush=True)
+        got, _ = ctx.solve(23, mods)
+        assert got[0] == 2762394459
+        print("case n=23 k=1 second modulus", flush=True)
+        one, _ = ctx.solve(23, mods[1:])
+        assert one == got[1:]
