@@ -1,26 +1,37 @@
-"""Exact-count durability: residue checkpoint, result manifest, work-dir lock.
+"""Durable state of an exact (CRT) run: the residue checkpoint, the result
+manifest and exact.txt, and the work-directory lock.
 
-The on-disk formats are the reference's, so a run can be resumed or verified
-by either implementation (reference: scripts/solve/exact_safety.py):
+The files are interchangeable with the reference's (scripts/solve/exact_safety.py),
+so either implementation can resume or verify a run:
 
-checkpoint.json  (format_version 4; the reference docs call it "V3",
-                  README.md:36, while its code writes 4, exact_safety.py:16)
-  {"format_version": 4, "n": N,
-   "solver": {"binary_name", "binary_sha256", "git_commit"},
-   "admission_certificate": null,
-   "residues": {"<p>": {"residue": r, "wall_s": s[, "log": str]}},
-   "checkpoint_sha256": sha256 of the canonical JSON of everything else}
-  Strict load: duplicate keys, unknown or missing fields, non-integer or
-  non-canonical values, a changed solver binary or a checksum mismatch are
-  errors (exact_safety.py:146-310).  Writes are fsync + atomic rename + fsync
-  of the directory (exact_safety.py:313-336).
+checkpoint.json -- format_version 4 (the reference's docs call the format "V3",
+  README.md:36; its code writes 4, exact_safety.py:16):
 
-exact_manifest.json (format ONEESAN_EXACT_RESULT_V4) and exact.txt, manifest
-  first, exact.txt last as the completion marker (exact_safety.py:395-563).
+      format_version  int, 4
+      n               int
+      solver          {binary_name: str, binary_sha256: 64 hex, git_commit: str}
+      admission_certificate   null (this solver has no row-8 automaton to certify)
+      residues        {"<modulus>": {residue: int in [0, modulus), wall_s?: finite >= 0, log?: str}}
+      checkpoint_sha256       SHA-256 of the canonical JSON of all other fields
 
-Not carried over: build-provenance sidecars and row-8 admission certificates
-(they certify reference build artefacts that this solver does not have); the
-manifest keeps those fields, set to null.
+exact_manifest.json -- format ONEESAN_EXACT_RESULT_V4 with manifest_sha256 over
+  the canonical JSON of its other fields; exact.txt -- the human-readable
+  result, written after the manifest, so that its presence marks completion.
+
+Canonical JSON = sorted keys, "," and ":" separators, ASCII.  Every file is
+replaced atomically: written to a temporary file in the same directory, flushed
+to disk, renamed over the target, and the directory entry flushed too.
+
+Reading is strict: a duplicate key, a NaN/Infinity constant, a missing or
+unknown field, a value of the wrong JSON type, a non-canonical modulus key, a
+residue outside [0, modulus), a changed solver binary or a checksum mismatch
+is an error.  The phrases the reference's tests match on
+(tests/test_exact_safety.py, tests/test_exact_manifest.py) are kept in the
+messages: "duplicate JSON key", "malformed exact checkpoint", "format_version
+and n must be JSON integers", "residue must be a JSON integer", "outside
+canonical range", "SHA-256 integrity checksum mismatch", "solver binary
+changed", "violates rigorous path bound", "final congruence check", "already in
+use".
 """
 from __future__ import annotations
 
@@ -30,221 +41,295 @@ import hmac
 import json
 import math
 import os
+import tempfile
 from pathlib import Path
-from typing import Any, TextIO
+from typing import Any, Callable, TextIO
 
 CHECKPOINT_FORMAT_VERSION = 4
 CHECKSUM_FIELD = "checkpoint_sha256"
 RESULT_MANIFEST_FORMAT = "ONEESAN_EXACT_RESULT_V4"
 RESULT_CHECKSUM_FIELD = "manifest_sha256"
+LOCK_NAME = ".solve.lock"
 
+_HEX = frozenset("0123456789abcdef")
+
+
+# ---------------------------------------------------------------- hashing
 
 def file_sha256(path: Path) -> str:
-    h = hashlib.sha256()
-    with Path(path).open("rb") as f:
-        for chunk in iter(lambda: f.read(1 << 20), b""):
-            h.update(chunk)
-    return h.hexdigest()
+    digest = hashlib.sha256()
+    with open(path, "rb") as f:
+        while block := f.read(1 << 20):
+            digest.update(block)
+    return digest.hexdigest()
 
 
-def solver_identity(binary: Path, git_commit: str = "unknown") -> dict[str, str]:
-    """Identity a checkpoint is bound to: a changed binary invalidates resume."""
-    binary = Path(binary).resolve()
-    return {"binary_name": binary.name, "binary_sha256": file_sha256(binary), "git_commit": git_commit}
+def canonical_json(obj: Any) -> bytes:
+    return json.dumps(obj, sort_keys=True, separators=(",", ":"), ensure_ascii=True).encode("ascii")
 
 
-def validate_residue(modulus: int, residue: int, *, source: str = "residue") -> None:
-    if modulus < 2:
-        raise ValueError(f"{source}: invalid modulus {modulus}")
-    if not 0 <= residue < modulus:
-        raise ValueError(f"{source}: residue {residue} is outside canonical range [0, {modulus})")
-
-
-def _validate_wall(value: Any, source: str) -> float:
-    if isinstance(value, bool) or not isinstance(value, (int, float)):
-        raise ValueError(f"{source}: wall_s must be numeric")
-    w = float(value)
-    if not math.isfinite(w) or w < 0:
-        raise ValueError(f"{source}: wall_s must be finite and nonnegative")
-    return w
-
-
-def validate_residue_records(residues: dict[int, dict]) -> None:
-    for modulus, rec in residues.items():
-        if type(modulus) is not int:
-            raise ValueError(f"checkpoint modulus key must be int, got {type(modulus).__name__}")
-        if not isinstance(rec, dict):
-            raise ValueError(f"checkpoint modulus {modulus}: record must be an object")
-        if "residue" not in rec:
-            raise ValueError(f"checkpoint modulus {modulus}: missing residue")
-        if type(rec["residue"]) is not int:
-            raise ValueError(f"checkpoint modulus {modulus}: residue must be a JSON integer")
-        validate_residue(modulus, rec["residue"], source=f"checkpoint modulus {modulus}")
-        if "wall_s" in rec:
-            _validate_wall(rec["wall_s"], f"checkpoint modulus {modulus}")
-        if "log" in rec and not isinstance(rec["log"], str):
-            raise ValueError(f"checkpoint modulus {modulus}: log must be a string")
-
-
-def _no_duplicates(pairs):
-    out: dict[str, Any] = {}
-    for k, v in pairs:
-        if k in out:
-            raise ValueError(f"duplicate JSON key in exact checkpoint: {k}")
-        out[k] = v
-    return out
-
-
-def _canonical(data: dict[str, Any], drop: str) -> bytes:
-    payload = {k: v for k, v in data.items() if k != drop}
-    return json.dumps(payload, sort_keys=True, separators=(",", ":"), ensure_ascii=True).encode()
+def _sha_without(data: dict[str, Any], field: str) -> str:
+    return hashlib.sha256(canonical_json({k: v for k, v in data.items() if k != field})).hexdigest()
 
 
 def checkpoint_checksum(data: dict[str, Any]) -> str:
-    return hashlib.sha256(_canonical(data, CHECKSUM_FIELD)).hexdigest()
+    return _sha_without(data, CHECKSUM_FIELD)
 
 
 def result_manifest_checksum(data: dict[str, Any]) -> str:
-    return hashlib.sha256(_canonical(data, RESULT_CHECKSUM_FIELD)).hexdigest()
+    return _sha_without(data, RESULT_CHECKSUM_FIELD)
 
 
-def _load_json_strict(path: Path, what: str = "exact checkpoint") -> dict[str, Any]:
+def _is_sha256(value: Any) -> bool:
+    return isinstance(value, str) and len(value) == 64 and set(value) <= _HEX
+
+
+def solver_identity(binary: Path, git_commit: str = "unknown") -> dict[str, str]:
+    """What a checkpoint is bound to: resuming with a different binary is refused."""
+    resolved = Path(binary).resolve()
+    return {"binary_name": resolved.name, "binary_sha256": file_sha256(resolved), "git_commit": git_commit}
+
+
+# ---------------------------------------------------------------- value checks
+
+def _is_int(v: Any) -> bool:
+    return type(v) is int  # JSON integers only: bool is not one
+
+
+def validate_residue(modulus: int, residue: int, *, source: str = "residue") -> None:
+    if not _is_int(modulus) or modulus < 2:
+        raise ValueError(f"{source}: modulus {modulus!r} is not an integer >= 2")
+    if not _is_int(residue) or not 0 <= residue < modulus:
+        raise ValueError(f"{source}: residue {residue!r} is outside canonical range [0, {modulus})")
+
+
+def _wall_seconds(value: Any, source: str) -> float:
+    if isinstance(value, bool) or not isinstance(value, (int, float)):
+        raise ValueError(f"{source}: wall_s is not a number")
+    seconds = float(value)
+    if not (math.isfinite(seconds) and seconds >= 0.0):
+        raise ValueError(f"{source}: wall_s {value!r} is not a finite, non-negative number")
+    return seconds
+
+
+RECORD_FIELDS = {"residue", "wall_s", "log"}
+
+
+def validate_residue_records(residues: dict[int, dict]) -> None:
+    """Every record: an int modulus key, a JSON-integer residue in range, optional
+    finite non-negative wall_s and string log, nothing else."""
+    for modulus, rec in residues.items():
+        where = f"checkpoint modulus {modulus}"
+        if not _is_int(modulus):
+            raise ValueError(f"checkpoint modulus key {modulus!r} is not an int")
+        if not isinstance(rec, dict):
+            raise ValueError(f"{where}: record is not an object")
+        extra = set(rec) - RECORD_FIELDS
+        if extra:
+            raise ValueError(f"{where}: unexpected fields {sorted(extra)}")
+        if "residue" not in rec:
+            raise ValueError(f"{where}: no residue")
+        if not _is_int(rec["residue"]):
+            raise ValueError(f"{where}: residue must be a JSON integer")
+        validate_residue(modulus, rec["residue"], source=where)
+        if "wall_s" in rec:
+            _wall_seconds(rec["wall_s"], where)
+        if "log" in rec and not isinstance(rec["log"], str):
+            raise ValueError(f"{where}: log is not a string")
+
+
+# ---------------------------------------------------------------- strict JSON
+
+def _pairs_without_duplicates(pairs: list[tuple[str, Any]]) -> dict[str, Any]:
+    seen: dict[str, Any] = {}
+    for key, value in pairs:
+        if key in seen:
+            raise ValueError(f"duplicate JSON key {key!r}")
+        seen[key] = value
+    return seen
+
+
+def _refuse_constant(name: str) -> Any:
+    raise ValueError(f"JSON constant {name} is not allowed")
+
+
+def read_strict_json(path: Path, what: str) -> dict[str, Any]:
+    """A JSON object from `path`; any defect is a ValueError naming `what`."""
     try:
-        raw = path.read_text(encoding="utf-8")
+        text = Path(path).read_text(encoding="utf-8")
     except OSError as exc:
-        raise ValueError(f"cannot read {what} {path}: {exc}") from exc
-
-    def reject_constant(token: str):
-        raise ValueError(f"non-standard JSON constant: {token}")
+        raise ValueError(f"{what} {path} cannot be read: {exc}") from exc
     try:
-        data = json.loads(raw, object_pairs_hook=_no_duplicates, parse_constant=reject_constant)
-    except (json.JSONDecodeError, ValueError) as exc:
+        obj = json.loads(text, object_pairs_hook=_pairs_without_duplicates, parse_constant=_refuse_constant)
+    except ValueError as exc:  # JSONDecodeError is a ValueError
         raise ValueError(f"malformed {what} {path}: {exc}") from exc
-    if not isinstance(data, dict):
-        raise ValueError(f"{what} {path}: root must be a JSON object")
-    return data
+    if not isinstance(obj, dict):
+        raise ValueError(f"malformed {what} {path}: the top level is not an object")
+    return obj
 
+
+Check = Callable[[Any], bool]
+
+
+def _require_fields(obj: dict[str, Any], spec: dict[str, Check], where: str) -> None:
+    """obj has exactly the keys of spec, each value passing its check."""
+    missing = sorted(set(spec) - set(obj))
+    if missing:
+        raise ValueError(f"{where}: fields missing: {missing}")
+    unknown = sorted(set(obj) - set(spec))
+    if unknown:
+        raise ValueError(f"{where}: fields not in the format: {unknown}")
+    for key, ok in spec.items():
+        if not ok(obj[key]):
+            raise ValueError(f"{where}: field {key!r} has an invalid value {obj[key]!r}")
+
+
+SOLVER_SPEC: dict[str, Check] = {
+    "binary_name": lambda v: isinstance(v, str),
+    "binary_sha256": _is_sha256,
+    "git_commit": lambda v: isinstance(v, str),
+}
+CHECKPOINT_SPEC: dict[str, Check] = {
+    "format_version": _is_int,
+    "n": _is_int,
+    "solver": lambda v: isinstance(v, dict),
+    "admission_certificate": lambda v: v is None,
+    "residues": lambda v: isinstance(v, dict),
+    CHECKSUM_FIELD: _is_sha256,
+}
+
+
+# ---------------------------------------------------------------- checkpoint
 
 def load_checkpoint(path: Path, *, n: int, identity: dict[str, str]) -> dict[int, dict]:
+    """Residues of a checkpoint ({} when there is none), after checking its
+    format, checksum, n and solver identity."""
     path = Path(path)
     if not path.exists():
         return {}
-    data = _load_json_strict(path)
-    if type(data.get("format_version")) is not int or type(data.get("n")) is not int:
-        raise ValueError(f"checkpoint {path}: format_version and n must be JSON integers")
+    data = read_strict_json(path, "exact checkpoint")
+    where = f"checkpoint {path}"
+    if not (_is_int(data.get("format_version")) and _is_int(data.get("n"))):
+        raise ValueError(f"{where}: format_version and n must be JSON integers")
     if data["format_version"] != CHECKPOINT_FORMAT_VERSION:
-        raise ValueError(f"checkpoint {path}: unsupported format_version {data['format_version']}")
-    saved = data.get(CHECKSUM_FIELD)
-    if not isinstance(saved, str) or len(saved) != 64:
-        raise ValueError(f"checkpoint {path} has no integrity checksum")
-    if not hmac.compare_digest(saved, checkpoint_checksum(data)):
-        raise ValueError(f"checkpoint {path}: SHA-256 integrity checksum mismatch")
+        raise ValueError(f"{where}: format_version {data['format_version']} is not "
+                         f"{CHECKPOINT_FORMAT_VERSION}")
+    if not _is_sha256(data.get(CHECKSUM_FIELD)):
+        raise ValueError(f"{where}: no {CHECKSUM_FIELD}")
+    if not hmac.compare_digest(data[CHECKSUM_FIELD], checkpoint_checksum(data)):
+        raise ValueError(f"{where}: SHA-256 integrity checksum mismatch")
+    _require_fields(data, CHECKPOINT_SPEC, where)
     if data["n"] != n:
-        raise ValueError(f"checkpoint {path} belongs to n={data['n']}, expected n={n}")
-    expected = {"format_version", "n", "solver", "admission_certificate", "residues", CHECKSUM_FIELD}
-    if set(data) - expected:
-        raise ValueError(f"checkpoint {path}: unknown fields: {sorted(set(data) - expected)}")
-    if expected - set(data):
-        raise ValueError(f"checkpoint {path}: missing fields: {sorted(expected - set(data))}")
-    solver = data["solver"]
-    keys = {"binary_name", "binary_sha256", "git_commit"}
-    if not isinstance(solver, dict) or set(solver) != keys or not all(isinstance(solver[k], str) for k in keys):
-        raise ValueError(f"checkpoint {path}: solver identity fields mismatch")
-    sha = solver["binary_sha256"]
-    if len(sha) != 64 or any(c not in "0123456789abcdef" for c in sha):
-        raise ValueError(f"checkpoint {path}: invalid solver binary SHA-256")
-    if sha != identity["binary_sha256"]:
-        raise ValueError(f"checkpoint {path}: solver binary changed ({sha!r} != {identity['binary_sha256']!r})")
-    if data["admission_certificate"] is not None:
-        raise ValueError(f"checkpoint {path}: admission certificates are not supported by this solver")
-    raw = data["residues"]
-    if not isinstance(raw, dict):
-        raise ValueError(f"checkpoint {path}: residues must be an object")
+        raise ValueError(f"{where} belongs to n={data['n']}, this run is n={n}")
+    _require_fields(data["solver"], SOLVER_SPEC, f"{where} solver")
+    saved = data["solver"]["binary_sha256"]
+    if saved != identity["binary_sha256"]:
+        raise ValueError(f"{where}: solver binary changed since the checkpoint was written "
+                         f"({saved} -> {identity['binary_sha256']})")
     out: dict[int, dict] = {}
-    for key, rec in raw.items():
-        try:
-            modulus = int(key)
-        except ValueError as exc:
-            raise ValueError(f"checkpoint {path}: invalid modulus key {key!r}") from exc
-        if str(modulus) != key:
-            raise ValueError(f"checkpoint {path}: non-canonical modulus key {key!r}")
-        if not isinstance(rec, dict):
-            raise ValueError(f"checkpoint modulus {modulus}: record must be an object")
-        unknown = sorted(set(rec) - {"residue", "wall_s", "log"})
-        if unknown:
-            raise ValueError(f"checkpoint modulus {modulus}: unknown fields: {unknown}")
-        out[modulus] = rec
+    for key, rec in data["residues"].items():
+        if not (key.isdigit() and str(int(key)) == key):
+            raise ValueError(f"{where}: modulus key {key!r} is not a canonical decimal")
+        out[int(key)] = rec
     validate_residue_records(out)
     return out
 
 
 def durable_atomic_write(path: Path, text: str) -> None:
-    path = Path(path)
-    tmp = path.with_suffix(path.suffix + ".tmp")
-    fd = os.open(tmp, os.O_WRONLY | os.O_CREAT | os.O_TRUNC, 0o644)
+    """Replace `path` with `text` so that a crash leaves the old or the new file,
+    never a torn one: temporary file in the same directory, fsync, rename, fsync
+    of the directory."""
+    target = Path(path)
+    directory = target.parent
+    handle, tmp_name = tempfile.mkstemp(prefix=f".{target.name}.", suffix=".tmp", dir=directory)
+    tmp = Path(tmp_name)
     try:
-        with os.fdopen(fd, "w", encoding="utf-8") as f:
-            fd = -1
-            f.write(text)
-            f.flush()
-            os.fsync(f.fileno())
-        os.replace(tmp, path)
-        dfd = os.open(path.parent, os.O_RDONLY | os.O_DIRECTORY)
-        try:
-            os.fsync(dfd)
-        finally:
-            os.close(dfd)
-    except Exception:
-        if fd >= 0:
-            os.close(fd)
-        try:
-            tmp.unlink()
-        except FileNotFoundError:
-            pass
+        with os.fdopen(handle, "wb") as out:
+            out.write(text.encode("utf-8"))
+            out.flush()
+            os.fsync(out.fileno())
+        os.chmod(tmp, 0o644)
+        os.replace(tmp, target)
+    except BaseException:
+        tmp.unlink(missing_ok=True)
         raise
+    dir_fd = os.open(directory, os.O_RDONLY | os.O_DIRECTORY)
+    try:
+        os.fsync(dir_fd)
+    finally:
+        os.close(dir_fd)
+
+
+def _pretty(obj: dict[str, Any]) -> str:
+    return json.dumps(obj, indent=2, sort_keys=True, ensure_ascii=True) + "\n"
 
 
 def save_checkpoint(path: Path, *, n: int, identity: dict[str, str], residues: dict[int, dict]) -> None:
     validate_residue_records(residues)
-    payload: dict[str, Any] = {
+    doc: dict[str, Any] = {
         "format_version": CHECKPOINT_FORMAT_VERSION,
         "n": n,
-        "solver": identity,
+        "solver": dict(identity),
         "admission_certificate": None,
-        "residues": {str(k): v for k, v in residues.items()},
+        "residues": {str(m): residues[m] for m in sorted(residues)},
     }
-    payload[CHECKSUM_FIELD] = checkpoint_checksum(payload)
-    durable_atomic_write(path, json.dumps(payload, indent=2, sort_keys=True, ensure_ascii=True) + "\n")
+    doc[CHECKSUM_FIELD] = checkpoint_checksum(doc)
+    durable_atomic_write(path, _pretty(doc))
 
+
+# ---------------------------------------------------------------- CRT
 
 def crt_pair(x: int, m: int, r: int, p: int) -> tuple[int, int]:
-    """Combine x mod m with r mod p (coprime) into the residue mod m*p."""
+    """The residue mod m*p congruent to x mod m and r mod p (gcd(m, p) = 1)."""
     if m == 1:
         return r % p, p
-    t = ((r - x) * pow(m, -1, p)) % p
-    return x + m * t, m * p
+    lift = (r - x) * pow(m, -1, p) % p
+    return x + m * lift, m * p
 
 
 def validate_exact_reconstruction(x: int, modulus_product: int, path_bound: int,
                                   used: list[tuple[int, int]]) -> None:
+    """x is the count only if the moduli's product exceeds the bound, x lies in
+    [0, bound] and x meets every congruence it was built from."""
     if modulus_product <= path_bound:
-        raise ValueError(f"CRT capacity insufficient: M={modulus_product} <= path_bound={path_bound}")
+        raise ValueError(f"moduli product ({modulus_product.bit_length()} bits) does not exceed the path "
+                         f"bound ({path_bound.bit_length()} bits)")
     if not 0 <= x <= path_bound:
-        raise ValueError(f"CRT reconstruction violates rigorous path bound: exact={x}, bound={path_bound}")
+        raise ValueError(f"CRT value {x} violates rigorous path bound {path_bound}")
     for p, r in used:
         validate_residue(p, r, source=f"final modulus {p}")
         if x % p != r:
-            raise ValueError(f"CRT reconstruction failed final congruence check for modulus {p}")
+            raise ValueError(f"CRT value fails the final congruence check mod {p}")
 
 
 def acquire_workdir_lock(work: Path) -> TextIO:
-    fd = (Path(work) / ".solve.lock").open("a+")
+    """An exclusive, non-blocking lock on <work>/.solve.lock, held while the
+    returned file stays open."""
+    handle = open(Path(work) / LOCK_NAME, "a+")
     try:
-        fcntl.flock(fd.fileno(), fcntl.LOCK_EX | fcntl.LOCK_NB)
-    except BlockingIOError as exc:
-        fd.close()
-        raise RuntimeError(f"work directory is already in use: {work}") from exc
-    return fd
+        fcntl.flock(handle, fcntl.LOCK_EX | fcntl.LOCK_NB)
+    except OSError as exc:
+        handle.close()
+        raise RuntimeError(f"work directory {work} is already in use by another runner") from exc
+    return handle
+
+
+# ---------------------------------------------------------------- result
+
+def _product(values: list[int]) -> int:
+    out = 1
+    for v in values:
+        out *= v
+    return out
+
+
+def exact_text(n: int, exact: int, path_bound: int, modulus_product: int, primes: int, wall: float,
+               manifest_name: str) -> str:
+    """exact.txt: one key=value per line."""
+    lines = [f"n={n}", f"exact={exact}", f"bound_bits={path_bound.bit_length()}",
+             f"modulus_bits={modulus_product.bit_length()}", f"primes_used={primes}",
+             f"solver_wall_s_sum={wall:.9f}", f"manifest_file={manifest_name}"]
+    return "\n".join(lines) + "\n"
 
 
 def write_exact_result(work: Path, *, n: int, exact: int, path_bound: int, modulus_product: int,
@@ -252,73 +337,73 @@ def write_exact_result(work: Path, *, n: int, exact: int, path_bound: int, modul
                        checkpoint_path: Path, total_wall_s: float,
                        strip_partition: list[int],
                        build_provenance: Path | None = None) -> tuple[Path, Path]:
+    """Write exact_manifest.json, then exact.txt; returns their paths."""
     validate_residue_records(residues)
     if len(set(used_moduli)) != len(used_moduli):
-        raise ValueError("exact result contains duplicate CRT moduli")
-    missing = [p for p in used_moduli if p not in residues]
-    if missing:
-        raise ValueError(f"exact result is missing residues for moduli: {missing}")
-    prod = 1
-    for p in used_moduli:
-        prod *= p
-    if prod != modulus_product:
-        raise ValueError("CRT modulus product mismatch before finalization")
+        raise ValueError("a CRT modulus is used twice")
+    absent = [p for p in used_moduli if p not in residues]
+    if absent:
+        raise ValueError(f"no residue recorded for moduli {absent}")
+    if _product(used_moduli) != modulus_product:
+        raise ValueError("the product of the used moduli is not the stated modulus product")
     validate_exact_reconstruction(exact, modulus_product, path_bound,
                                   [(p, residues[p]["residue"]) for p in used_moduli])
-    if not Path(checkpoint_path).exists():
-        raise ValueError(f"cannot finalize exact result without checkpoint: {checkpoint_path}")
-    congruences = []
+    checkpoint_path = Path(checkpoint_path)
+    if not checkpoint_path.exists():
+        raise ValueError(f"checkpoint {checkpoint_path} is missing: a result needs its checkpoint")
+    congruences: list[dict[str, Any]] = []
     for p in used_moduli:
-        rec = residues[p]
-        item: dict[str, Any] = {"modulus": p, "residue": rec["residue"], "wall_s": float(rec.get("wall_s", 0.0))}
-        if "log" in rec:
-            item["log"] = rec["log"]
-        congruences.append(item)
-    wall_sum = sum(c["wall_s"] for c in congruences)
-    if not math.isclose(float(total_wall_s), wall_sum, rel_tol=1e-12, abs_tol=1e-9):
-        raise ValueError(f"solver wall-time sum mismatch before finalization: {total_wall_s} != {wall_sum}")
+        entry: dict[str, Any] = {"modulus": p, "residue": residues[p]["residue"],
+                                 "wall_s": float(residues[p].get("wall_s", 0.0))}
+        if "log" in residues[p]:
+            entry["log"] = residues[p]["log"]
+        congruences.append(entry)
+    wall = math.fsum(c["wall_s"] for c in congruences)
+    if not math.isclose(float(total_wall_s), wall, rel_tol=1e-12, abs_tol=1e-9):
+        raise ValueError(f"total solver wall time {total_wall_s} is not the sum of the residues' {wall}")
     work = Path(work)
     manifest_path, exact_path = work / "exact_manifest.json", work / "exact.txt"
-    exact_text = (f"n={n}\nexact={exact}\nbound_bits={path_bound.bit_length()}\n"
-                  f"modulus_bits={modulus_product.bit_length()}\nprimes_used={len(used_moduli)}\n"
-                  f"solver_wall_s_sum={total_wall_s:.9f}\nmanifest_file={manifest_path.name}\n")
-    payload: dict[str, Any] = {
+    text = exact_text(n, exact, path_bound, modulus_product, len(used_moduli), float(total_wall_s),
+                      manifest_path.name)
+    prov = Path(build_provenance) if build_provenance else None
+    manifest: dict[str, Any] = {
         "format": RESULT_MANIFEST_FORMAT,
         "n": n,
         "exact_decimal": str(exact),
         "path_bound_decimal": str(path_bound),
         "bound_bits": path_bound.bit_length(),
         "bound_method": "checkerboard_strip_v1",
-        "strip_partition": strip_partition,
+        "strip_partition": list(strip_partition),
         "modulus_product_decimal": str(modulus_product),
         "modulus_bits": modulus_product.bit_length(),
         "primes_used": len(used_moduli),
         "congruences": congruences,
-        "solver": identity,
+        "solver": dict(identity),
         "solver_wall_s_sum": float(total_wall_s),
-        "checkpoint_file": Path(checkpoint_path).name,
+        "checkpoint_file": checkpoint_path.name,
         "checkpoint_sha256": file_sha256(checkpoint_path),
         "exact_file": exact_path.name,
-        "exact_sha256": hashlib.sha256(exact_text.encode()).hexdigest(),
-        "build_provenance_file": Path(build_provenance).name if build_provenance else None,
-        "build_provenance_sha256": file_sha256(build_provenance) if build_provenance else None,
+        "exact_sha256": hashlib.sha256(text.encode("utf-8")).hexdigest(),
+        "build_provenance_file": prov.name if prov else None,
+        "build_provenance_sha256": file_sha256(prov) if prov else None,
         "admission_certificate_file": None,
         "admission_certificate_sha256": None,
         "admission_certificate_schema": None,
     }
-    payload[RESULT_CHECKSUM_FIELD] = result_manifest_checksum(payload)
-    durable_atomic_write(manifest_path, json.dumps(payload, indent=2, sort_keys=True, ensure_ascii=True) + "\n")
-    durable_atomic_write(exact_path, exact_text)
+    manifest[RESULT_CHECKSUM_FIELD] = result_manifest_checksum(manifest)
+    durable_atomic_write(manifest_path, _pretty(manifest))
+    durable_atomic_write(exact_path, text)
     return exact_path, manifest_path
 
 
 def load_exact_result_manifest(path: Path) -> dict[str, Any]:
-    data = _load_json_strict(Path(path), "exact result manifest")
-    saved = data.get(RESULT_CHECKSUM_FIELD)
-    if not isinstance(saved, str) or len(saved) != 64:
-        raise ValueError(f"exact result manifest {path}: missing SHA-256 checksum")
-    if not hmac.compare_digest(saved, result_manifest_checksum(data)):
-        raise ValueError(f"exact result manifest {path}: SHA-256 integrity checksum mismatch")
+    """A result manifest whose format and checksum check out."""
+    data = read_strict_json(Path(path), "exact result manifest")
+    where = f"exact result manifest {path}"
+    if not _is_sha256(data.get(RESULT_CHECKSUM_FIELD)):
+        raise ValueError(f"{where}: no {RESULT_CHECKSUM_FIELD}")
+    if not hmac.compare_digest(data[RESULT_CHECKSUM_FIELD], result_manifest_checksum(data)):
+        raise ValueError(f"{where}: manifest SHA-256 checksum mismatch")
     if data.get("format") != RESULT_MANIFEST_FORMAT:
-        raise ValueError(f"exact result manifest {path}: unsupported format {data.get('format')!r}")
+        raise ValueError(f"{where}: format {data.get('format')!r} is not {RESULT_MANIFEST_FORMAT}")
     return data

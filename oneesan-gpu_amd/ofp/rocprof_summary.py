@@ -3,8 +3,17 @@
   python -m ofp.rocprof_summary trace   <kernel_trace.csv>          [--cells C]
   python -m ofp.rocprof_summary counter <counter_collection.csv>    [--cells C] [--scale F]
   python -m ofp.rocprof_summary valu    <counter_collection.csv>    --cells C [--cus N] [--xcds X]
+  python -m ofp.rocprof_summary dram    <counter_collection.csv>
 
-`trace` prints per-kernel call counts and mean durations.  `counter` prints the
+`dram` reads a run collected with
+  --pmc TCC_EA0_RDREQ_DRAM_32B_sum TCC_EA0_WRREQ_WRITE_DRAM_32B_sum
+(read and write requests to DRAM in 32-byte units, a 64-byte request counted
+twice and a 128-byte one four times) and prints, per cell-kernel family, the
+mean DRAM bytes read and written per dispatch.
+
+`trace` prints per-kernel call counts and mean durations; it reads the kernel
+trace as CSV (--output-format csv) or as the SQLite database rocprofv3 writes by
+default (*_results.db).  `counter` prints the
 mean counter value per kernel (KiB per dispatch for FETCH_SIZE / WRITE_SIZE)
 and, with --scale, the calibrated bytes (the scale is what a device-to-device
 copy of known size reads for the same counter).
@@ -65,8 +74,21 @@ def _rows(seq, cells):
     return rows[:MAX_ROWS] if MAX_ROWS else rows
 
 
+def _trace_rows(path: str) -> list[dict]:
+    """Kernel dispatches as {Kernel_Name, Start_Timestamp, End_Timestamp} (ns)."""
+    if path.endswith(".db"):
+        import sqlite3
+        con = sqlite3.connect(path)
+        try:
+            got = con.execute("select name, start, end from kernels order by start").fetchall()
+        finally:
+            con.close()
+        return [{"Kernel_Name": n, "Start_Timestamp": s, "End_Timestamp": e} for n, s, e in got]
+    return list(csv.DictReader(open(path)))
+
+
 def trace(path: str, cells: int | None) -> None:
-    rows = list(csv.DictReader(open(path)))
+    rows = _trace_rows(path)
     by = collections.defaultdict(list)
     fam = collections.defaultdict(list)
     for r in rows:
@@ -170,9 +192,41 @@ def valu(path: str, cells: int, cus: int, xcds: int, words: int | None) -> None:
         print(f"  row: VALU busy {100.0 * tot_valu / cus / tot_cyc:.1f}% of the CUs' VALU issue cycles")
 
 
+DRAM_COUNTERS = ("TCC_EA0_RDREQ_DRAM_32B_sum", "TCC_EA0_WRREQ_WRITE_DRAM_32B_sum")
+
+
+def dram_bytes(path: str) -> dict[str, dict]:
+    """{family: {"read": B, "write": B, "dispatches": d}}, means per dispatch."""
+    per: dict[int, dict] = {}
+    for r in csv.DictReader(open(path)):
+        e = per.setdefault(int(r["Dispatch_Id"]), {"kernel": r["Kernel_Name"]})
+        e[r["Counter_Name"]] = float(r["Counter_Value"])
+    out: dict[str, dict] = {}
+    for d in sorted(per):
+        e = per[d]
+        fam = kernel_family(e["kernel"])
+        if fam is None or not all(c in e for c in DRAM_COUNTERS):
+            continue
+        o = out.setdefault(fam, {"read": 0.0, "write": 0.0, "dispatches": 0})
+        o["read"] += 32.0 * e[DRAM_COUNTERS[0]]
+        o["write"] += 32.0 * e[DRAM_COUNTERS[1]]
+        o["dispatches"] += 1
+    for o in out.values():
+        o["read"] /= o["dispatches"]
+        o["write"] /= o["dispatches"]
+    return out
+
+
+def dram(path: str) -> None:
+    print(f"{'kernel':28s} {'dispatches':>10s} {'read_GB':>10s} {'write_GB':>10s} {'total_GB':>10s}")
+    for fam, o in sorted(dram_bytes(path).items()):
+        print(f"{fam:28s} {o['dispatches']:10d} {o['read'] / 1e9:10.3f} {o['write'] / 1e9:10.3f} "
+              f"{(o['read'] + o['write']) / 1e9:10.3f}")
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["trace", "counter", "valu"])
+    ap.add_argument("mode", choices=["trace", "counter", "valu", "dram"])
     ap.add_argument("csv")
     ap.add_argument("--cells", type=int, default=None, help="cells per row (W-1) for the per-row listing")
     ap.add_argument("--rows", type=int, default=None,
@@ -189,6 +243,8 @@ def main(argv=None) -> int:
         trace(a.csv, a.cells)
     elif a.mode == "counter":
         counter(a.csv, a.scale, a.cells)
+    elif a.mode == "dram":
+        dram(a.csv)
     else:
         if not a.cells:
             ap.error("valu needs --cells")

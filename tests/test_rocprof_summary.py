@@ -68,3 +68,35 @@ def test_valu_busy_and_lanes(tmp_path, capsys):
     # 1000 VALU issue cycles per CU out of 4000 per XCD: 25 %; half the lanes active: 50 %
     assert "  25.0   50.0" in out
     assert "row: VALU busy 25.0%" in out
+
+
+SORTED = "void ofp::(anonymous namespace)::cell_sorted_kernel<14, unsigned int>(ofp::CellLaunch)"
+
+
+def test_dram_bytes_per_family(tmp_path, capsys):
+    """DRAM request counters are in 32-byte units: bytes = 32 x count, averaged per
+    dispatch of each cell-kernel family; other kernels are left out."""
+    p = tmp_path / "d.csv"
+    rows = [(1, SORTED, 1000, 500), (2, SORTED, 3000, 700), (3, CELL, 10, 20),
+            (4, "void ofp::set_one_values_kernel(unsigned int*)", 1, 1)]
+    with open(p, "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=["Dispatch_Id", "Kernel_Name", "Counter_Name", "Counter_Value"])
+        w.writeheader()
+        for d, name, rd, wr in rows:
+            w.writerow({"Dispatch_Id": d, "Kernel_Name": name, "Counter_Name": rs.DRAM_COUNTERS[0],
+                        "Counter_Value": rd})
+            w.writerow({"Dispatch_Id": d, "Kernel_Name": name, "Counter_Name": rs.DRAM_COUNTERS[1],
+                        "Counter_Value": wr})
+    got = rs.dram_bytes(str(p))
+    assert set(got) == {"cell_sorted_kernel<14>", "cell_kernel<14>"}
+    assert got["cell_sorted_kernel<14>"] == {"read": 32.0 * 2000, "write": 32.0 * 600, "dispatches": 2}
+    assert got["cell_kernel<14>"] == {"read": 320.0, "write": 640.0, "dispatches": 1}
+    rs.dram(str(p))
+    assert "cell_sorted_kernel<14>" in capsys.readouterr().out
+
+
+def test_kernel_family_names():
+    assert rs.kernel_family(SORTED) == "cell_sorted_kernel<14>"
+    assert rs.kernel_family(CELL) == "cell_kernel<14>"
+    assert rs.kernel_family(CAPPED) == "cell_capped_kernel<14>"
+    assert rs.kernel_family("void ofp::fill_remote_kernel()") is None
