@@ -90,6 +90,18 @@ ofp_status ofp_plan(ofp_ctx* ctx, int n, int k, ofp_plan_info* out);
 ofp_status ofp_solve(ofp_ctx* ctx, int n, const uint32_t* moduli, int k,
                      uint32_t* residues_out, ofp_stats* stats);
 
+/* ofp_solve for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid allocation): the
+ * same result, with the state held in a static device array inside the library
+ * (64 MiB per device the library runs on, kept until the process ends) and no
+ * hipMalloc, hipFree or graph creation on the way.  The state of (n, k) must fit
+ * the array (n <= 15 with up to 8 moduli, n = 16 with up to 2); larger requests
+ * fail with OFP_E_MEMORY; 1..16 moduli.  Each call re-initialises the array, and
+ * calls that overlap on one device (several contexts or threads) race on it.
+ * No counterpart in the reference, which allocates its state per process. */
+ofp_status ofp_solve_static(ofp_ctx* ctx, int n, const uint32_t* moduli, int k,
+                            uint32_t* residues_out);
+
 /* Moduli per sweep of ofp_solve.  A solve whose k-moduli state does not fit
  * HBM (or exceeds max_moduli, when that is > 0) runs its moduli in several
  * sweeps one after another, each of the largest count that fits, split as

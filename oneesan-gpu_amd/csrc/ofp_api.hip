@@ -15,6 +15,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -40,6 +41,7 @@ struct Sweep {
     hipGraph_t row_graph = nullptr;
     std::string row_key;
     bool async_alloc = false;  // buffers come from the stream-ordered allocator (jobs)
+    bool fixed = false;        // buffers are slices of the library's static device array (no graphs)
     int row = -1;              // rows swept since the initial state; -1: state of unknown origin
     std::vector<hipEvent_t> seg_ev;  // timing events of row segments (ofp_row_sweep)
 };
@@ -68,6 +70,8 @@ struct ofp_ctx {
                            // -1: from cell 6 for launches of 8 or more moduli, else none)
     int index_bits = 0;    // 0: 32-bit ranks for W <= 23, 64-bit above; 64: 64-bit ranks always
     int max_batch = 0;     // moduli per sweep of ofp_solve (0: as many as fit HBM)
+    void* scratch = nullptr;  // device address of the static state array on this device
+    Sweep fixed;           // ofp_solve_static's state, inside that array
     std::string err;
     Sweep st;              // the context's own state (solve / prepare / row_sweep)
     int jobs = 0;          // live jobs (serving)
@@ -79,6 +83,14 @@ struct ofp_job {
     int rows_enqueued = 0;
     hipEvent_t done = nullptr;
 };
+
+// The static device array behind ofp_solve_static: one per device the library
+// runs on, held until the process ends.  Host code never passes `scratch`
+// itself to a kernel or a copy (on the host the name only identifies the
+// symbol): the device address comes from hipGetSymbolAddress, once per device
+// (static_scratch), and only that address is used.
+constexpr size_t kStaticStateBytes = 64ull << 20;
+static __device__ __align__(256) uint32_t scratch[kStaticStateBytes / sizeof(uint32_t)];
 
 namespace {
 
@@ -115,6 +127,24 @@ uint64_t state_bytes(int layout, int n, int k) {
     uint64_t nm, nb;
     if (!counts(n, &nm, &nb)) return 0;
     return (padded_states(nm) + padded_states(nb ? nb : 1)) * (uint64_t)interleave_stride(layout, k) * 4;
+}
+
+// Device address of `scratch` on `device`, which must be current; looked up
+// once per device and kept.
+hipError_t static_scratch(int device, void** out) {
+    static std::mutex mu;
+    static void* addr[64] = {};
+    if (device < 0 || device >= 64) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[device]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        addr[device] = p;
+    }
+    *out = addr[device];
+    return hipSuccess;
 }
 
 void drop_graph(Sweep& s) {
@@ -335,7 +365,7 @@ ofp_status enqueue_cells(ofp_ctx* c, Sweep& s, int p_hi, int p_lo, bool first_re
 // `rows` uncapped rows (cells W-1..1; the blocked input of cell W-1 is zero),
 // via a hipGraph captured once per (state, launch shape) and replayed.
 ofp_status full_rows(ofp_ctx* c, Sweep& s, int rows, int* launches) {
-    if (!c->use_graphs) {
+    if (!c->use_graphs || s.fixed) {
         for (int r = 0; r < rows; ++r) {
             ofp_status st = enqueue_cells(c, s, s.W - 1, 1, false, launches);
             if (st) return st;
@@ -605,6 +635,9 @@ ofp_status ofp_create(const ofp_config* cfg, ofp_ctx** out) {
     }
     if (bad(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking))) return OFP_E_RUNTIME;
     c->st.stream = c->stream;
+    c->fixed.stream = c->stream;
+    c->fixed.fixed = true;
+    if (bad(static_scratch(c->device, &c->scratch))) return OFP_E_RUNTIME;
     if (bad(hipEventCreate(&c->ev0)) || bad(hipEventCreate(&c->ev1))) return OFP_E_RUNTIME;
     std::vector<unsigned char> q32(64 * 1024), q64(64 * 1024);
     size_t b32 = pack_height_table(false, q32.data(), q32.size());
@@ -858,6 +891,45 @@ ofp_status ofp_solve(ofp_ctx* c, int n, const uint32_t* moduli, int k, uint32_t*
         st->wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
     return OFP_OK;
+}
+
+ofp_status ofp_solve_static(ofp_ctx* c, int n, const uint32_t* moduli, int k, uint32_t* residues_out) {
+    if (!c || !moduli || !residues_out) return OFP_E_ARG;
+    uint64_t nm, nb;
+    if (!counts(n, &nm, &nb)) return fail(c, OFP_E_RANGE, "n must be in 1..27");
+    if (k < 1 || k > kMaxKC) return fail(c, OFP_E_ARG, "ofp_solve_static takes 1..16 moduli");
+    for (int j = 0; j < k; ++j)
+        if (moduli[j] < 2) return fail(c, OFP_E_ARG, "modulus must be in [2, 4294967295]");
+    const int Kp = interleave_stride(c->layout, k);
+    const size_t mb = padded_states(nm) * (size_t)Kp * sizeof(uint32_t);
+    const size_t bb = padded_states(nb ? nb : 1) * (size_t)Kp * sizeof(uint32_t);
+    if (mb + bb > kStaticStateBytes)
+        return fail(c, OFP_E_MEMORY, "state of " + std::to_string(mb + bb) + " bytes exceeds the " +
+                                         std::to_string(kStaticStateBytes) + "-byte static array");
+    if (!c->scratch) return fail(c, OFP_E_RUNTIME, "static array not set up");
+    OFP_HIP(c, hipSetDevice(c->device), "hipSetDevice");
+    Sweep& s = c->fixed;
+    s.main = static_cast<uint32_t*>(c->scratch);
+    s.blk = s.main + mb / sizeof(uint32_t);  // mb is a multiple of 256 bytes (64 states x 4 B)
+    s.main_cap = mb;
+    s.blk_cap = bb;
+    s.mods_dev = nullptr;  // the start word takes the moduli by value
+    s.mods.assign(moduli, moduli + k);
+    s.n = n;
+    s.W = n + 1;
+    s.K = k;
+    s.Kp = Kp;
+    s.layout = c->layout;
+    s.nmain = nm;
+    s.nblk = nb;
+    ofp_status st = init_state(c, s);  // clears both arrays: they hold the last call's state
+    if (st) return st;
+    int launches = 0;
+    st = run_rows(c, s, s.W, &launches);
+    if (st) return st;
+    st = read_main(c, s, 0, residues_out, k);
+    s.main = s.blk = nullptr;
+    return st;
 }
 
 ofp_status ofp_set_max_batch(ofp_ctx* c, int max_moduli) {

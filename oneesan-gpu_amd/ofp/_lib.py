@@ -30,7 +30,7 @@ EXPORTS = (
     "ofp_job_create", "ofp_job_enqueue", "ofp_job_info_get", "ofp_job_result", "ofp_job_destroy",
     "ofp_shard_plan", "ofp_shard_sweep", "ofp_set_height_caps", "ofp_height_cap", "ofp_set_xcd_pieces",
     "ofp_set_sorted_cells", "ofp_time_cells", "ofp_set_index_width", "ofp_set_max_batch",
-    "ofp_moduli_groups",
+    "ofp_moduli_groups", "ofp_solve_static",
 )
 
 
@@ -152,6 +152,7 @@ def load() -> ctypes.CDLL:
                                             ctypes.c_int]),
         "ofp_set_index_width": (ctypes.c_int32, [vp, ctypes.c_int]),
         "ofp_set_max_batch": (ctypes.c_int32, [vp, ctypes.c_int]),
+        "ofp_solve_static": (ctypes.c_int32, [vp, ctypes.c_int, u32p, ctypes.c_int, u32p]),
         "ofp_moduli_groups": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                              ctypes.c_int]),
         "ofp_set_launch": (ctypes.c_int32, [vp, ctypes.c_int, ctypes.c_int]),
@@ -282,6 +283,14 @@ class Context:
         stats = Stats()
         self._check(self._lib.ofp_solve(self._h, n, mods, k, out, ctypes.byref(stats)), "ofp_solve")
         return list(out), stats.as_dict()
+
+    def solve_static(self, n: int, moduli: Sequence[int]) -> list[int]:
+        """solve() with the state in the library's static device array (no device
+        allocation); for small states only (see include/ofp.h)."""
+        k = len(moduli)
+        out = (ctypes.c_uint32 * k)()
+        self._check(self._lib.ofp_solve_static(self._h, n, _u32_array(moduli), k, out), "ofp_solve_static")
+        return list(out)
 
     def apply_cells(self, n: int, moduli: Sequence[int], main, blocked, p_hi: int, p_lo: int,
                     blocked_valid: bool = True) -> None:
