@@ -271,13 +271,19 @@ def measure_traffic(n: int, k: int, timeout: float) -> dict:
 def kernel_rows(cell_times: list[list[tuple[int, float, str]]], kc: int, states: int, k: int,
                 traffic: dict | None) -> dict:
     """Per-kernel statistics over rows of time_cells() output: one entry per cell
-    (one launch when k <= 16; a cell of k > 16 moduli sums its launches)."""
+    launched on its own (one launch when k <= 16; a cell of k > 16 moduli sums its
+    launches) and one per pair pass (cells p, p-1; its seven launches, one per
+    window height change, together).  Traffic per pair pass = the pair kernel's
+    DRAM bytes over one row / the passes in a row."""
     by: dict[str, list[float]] = {}
     cells: dict[str, set] = {}
     for row in cell_times:
         for p, ms, name in row:
+            if name == "pair_second":
+                continue  # the second cell of a pair pass (timed with the first)
             by.setdefault(name, []).append(ms)
             cells.setdefault(name, set()).add(p)
+    passes_per_row = len(by.get("pair_kernel", [])) / max(1, len(cell_times))
     declared = 8.0 * states * k
     out = {}
     for name, v in by.items():
@@ -287,6 +293,11 @@ def kernel_rows(cell_times: list[list[tuple[int, float, str]]], kc: int, states:
                "unit": "GB/s"}
         ent["frac"] = ent["achieved"] / HBM_PEAK_GBS
         t = (traffic or {}).get(f"{name}<{kc}>")
+        if name == "pair_kernel":
+            ent["cells"] = [f"{p},{p - 1}" for p in ent["cells"]]
+            ent["unit"] = "GB/s per pass (two cells, seven launches)"
+            if t and passes_per_row:
+                t = {"read": t["read_total"] / passes_per_row, "write": t["write_total"] / passes_per_row}
         if t:
             tb = t["read"] + t["write"]
             ent.update({"traffic": tb, "traffic_read": t["read"], "traffic_write": t["write"],

@@ -140,14 +140,31 @@ ofp_status ofp_row_sweep(ofp_ctx* ctx, int rows, ofp_stats* stats);
  * (rank_full(R), fullmate_dropN_batch.cu:514). */
 ofp_status ofp_read_main(ofp_ctx* ctx, uint64_t index, uint32_t* out, int capacity);
 
-/* One whole-state row of the prepared state with every cell launched on its
- * own and timed with HIP events (no graph, no height caps): ms_out[i] and
+/* One whole-state row of the prepared state, each cell or pair pass launched
+ * on its own and timed with HIP events (no graph, no height caps): ms_out[i] and
  * kind_out[i] for cell p = W-1-i, i < W-1 (kind 0 = cell_kernel, 1 =
- * cell_sorted_kernel; a cell with several launches, k > 16, sums them).
+ * cell_sorted_kernel, 2 = a pair pass over cells p and p-1, whose time it
+ * carries, 3 = the second cell of a pair pass, time 0; a cell with several
+ * launches, k > 16, sums them).
  * `capacity` slots each; fewer than W-1 is refused.  Advances the state by
  * one row.  Used by the benchmark for per-kernel rooflines; no reference
  * counterpart. */
 ofp_status ofp_time_cells(ofp_ctx* ctx, float* ms_out, int* kind_out, int capacity);
+
+/* Pair passes (default) or one launch per cell for the whole-state rows
+ * of the context's own solves (ofp_solve, ofp_prepare + ofp_row_sweep,
+ * ofp_apply_cells, ofp_time_cells).  A pair pass applies cells p and p-1 in
+ * place, in seven launches (one per window height change), so every word is
+ * read and written once per two cells instead of once per cell.  Rows with
+ * height-capped cells still run one launch per cell.  32-bit ranks (n <= 22)
+ * only; elsewhere the setting is ignored.  Results do not depend on it.
+ * mode: 0 = off, 1 = on (the default), 2..16 (even) = on with at most that
+ * many moduli per launch (default 16).  Walking down a row, cells p and p-1 run
+ * as one pair pass when bit p of cell_mask is set (~0u: everywhere), else cell
+ * p runs alone; 0 = the library's default (pairs except over cells 4..3 with 8
+ * or more moduli per launch, 6..3 with fewer; set by the autotuner).  Reference analogue: the reference's fused two-cell group kernels
+ * (row6crt20.cu:1033-1154), which run in group scratch. */
+ofp_status ofp_set_pair_passes(ofp_ctx* ctx, int mode, uint32_t cell_mask);
 
 /* Index width of the cell kernels: 0 = by n (32-bit ranks for n <= 22,
  * 64-bit above), 64 = 64-bit ranks at every n.  Results do not depend on it;

@@ -21,6 +21,7 @@
 
 #include "../../include/ofp.h"
 #include "ofp_internal.h"
+#include "ofp_pair_impl.h"
 
 using namespace ofp;
 
@@ -40,6 +41,7 @@ struct Sweep {
     hipGraphExec_t row_exec = nullptr;
     hipGraph_t row_graph = nullptr;
     std::string row_key;
+    int row_launches = 0;      // kernel launches in the captured row graph
     bool async_alloc = false;  // buffers come from the stream-ordered allocator (jobs)
     bool fixed = false;        // buffers are slices of the library's static device array (no graphs)
     int row = -1;              // rows swept since the initial state; -1: state of unknown origin
@@ -63,6 +65,10 @@ struct ofp_ctx {
     void* dp64 = nullptr;  // packed uint64 height table
     void* dpc32 = nullptr; // packed capped tables, caps 1..kMaxCap (uint32, uint64)
     void* dpc64 = nullptr;
+    void* ct32 = nullptr;  // pair-pass class tables, every (p, d) (32-bit ranks)
+    int pair_rows = 1;     // 1: full-state rows run as pair passes (two cells per pass, in place)
+    int pair_kc = kMaxKC;  // moduli per pair-pass launch (even, <= 16)
+    uint32_t pair_mask = 0;  // bit p: cells p and p-1 run as one pair pass; 0: the library's default
     int cap_percent = -1;  // height caps: cap a cell when it then sweeps <= this % of the blocked words
                            // (-1: 75 at one modulus, else 85)
     int xcd_shift = 5;     // chunk pieces per XCD: 2^xcd_shift chunks, < 0 = one eighth each
@@ -194,6 +200,12 @@ int grid_blocks(const ofp_ctx* c, uint64_t nblk) {
 }
 
 // Ensure device buffers of s for (n, moduli) in the context's current layout.
+// Whether n in the context's settings runs its full rows as pair passes: pair
+// mode on, state-major layout, 32-bit ranks (n <= 22), at least two cells.
+bool pairs_apply(const ofp_ctx* c, int n) {
+    return c->pair_rows && c->layout == 1 && n + 1 <= 23 && c->index_bits != 64 && n >= 2;
+}
+
 ofp_status ensure_state(ofp_ctx* c, Sweep& s, int n, const uint32_t* moduli, int k) {
     uint64_t nm, nb;
     if (!counts(n, &nm, &nb)) return fail(c, OFP_E_RANGE, "n must be in 1..27");
@@ -362,27 +374,108 @@ ofp_status enqueue_cells(ofp_ctx* c, Sweep& s, int p_hi, int p_lo, bool first_re
     return OFP_OK;
 }
 
+// True when s's full rows run as pair passes.
+bool pair_ready(const ofp_ctx* c, const Sweep& s) { return pairs_apply(c, s.n) && s.layout == 1; }
+
+// Cells p_hi..p_lo of s, walking down from p_hi: where bit p of the context's
+// pair mask is set and p-1 >= p_lo, cells p and p-1 run as one pair pass (in
+// place, one launch per window height change d, the classes that read other
+// classes -- d = -1, 0, 1 -- first), otherwise cell p runs alone with the
+// per-cell kernel.  The first cell reads the blocked input iff first_reads_blk.
+// With cell_ev, events bracket each step: a pair's first cell slot (slot
+// p_hi - p, kind 2) gets its time and the second an empty interval (kind 3).
+// The library's default pairing: every pair down to cell 5 with 8 or more moduli
+// per launch, down to cell 7 with fewer (at n=22, 14 moduli, the pass over cells
+// 4 and 3 took 142 ms against 127 ms as two launches; one modulus: the passes over
+// cells 6..3 took 25 and 30 ms against 24 and 26 ms), and cells 2 and 1 as a pair.
+uint32_t default_pair_mask(int k) { return k >= 8 ? ~((1u << 4) | (1u << 3)) : ~0x78u; }
+
+ofp_status enqueue_pairs(ofp_ctx* c, Sweep& s, int p_hi, int p_lo, bool first_reads_blk, int* launches,
+                         hipEvent_t* cell_ev = nullptr, int* cell_kind = nullptr) {
+    const int W = s.W;
+    const size_t tb = class_table_bytes(false);
+    const uint32_t mask = c->pair_mask ? c->pair_mask : default_pair_mask(std::min(s.K, c->pair_kc));
+    for (int p = p_hi; p >= p_lo;) {
+        const int slot = p_hi - p;
+        const bool reads_blk = p == p_hi ? first_reads_blk : true;
+        if (!(p - 1 >= p_lo && p >= 2 && ((mask >> p) & 1u))) {
+            ofp_status st = enqueue_cells(c, s, p, p, reads_blk, launches, -1, cell_ev ? cell_ev + 2 * slot : nullptr,
+                                          cell_kind ? cell_kind + slot : nullptr);
+            if (st) return st;
+            --p;
+            continue;
+        }
+        if (cell_ev) OFP_HIP(c, hipEventRecord(cell_ev[2 * slot], s.stream), "event record");
+        for (int off = 0; off < s.Kp; off += c->pair_kc) {
+            const int kc = std::min(c->pair_kc, s.Kp - off);
+            static const int kOrder[7] = {-1, 0, 1, -3, -2, 2, 3};
+            for (int di = 0; di < 7; ++di) {
+                const int d = kOrder[di];
+                const uint64_t cnt = pair_class_count(W, p, d);
+                if (!cnt) continue;
+                pair::PairLaunch a{};
+                a.main_in = s.main;
+                a.blk_in = s.blk;
+                a.main_out = s.main;
+                a.blk_out = s.blk;
+                a.dp = c->dp32;
+                a.ct = static_cast<const char*>(c->ct32) + tb * (size_t)(p * 7 + d + 3);
+                a.classes = cnt;
+                a.W = W;
+                a.p = p;
+                a.d = d;
+                a.read_blk = reads_blk ? 1 : 0;
+                a.K = s.Kp;
+                a.off = off;
+                a.kc = kc;
+                // lanes past the k moduli (stride padding) compute with a valid modulus
+                for (int jj = 0; jj < kc; ++jj) a.mod[jj] = s.mods[off + jj < s.K ? off + jj : 0];
+                a.xcd_shift = piece_shift(c->xcd_shift, cnt, c->threads);
+                hipError_t e = launch_pair(a, false, c->threads, grid_blocks(c, cnt), s.stream);
+                if (e != hipSuccess) return hip_fail(c, e, "pair pass launch");
+                if (launches) ++*launches;
+            }
+        }
+        if (cell_ev) {
+            OFP_HIP(c, hipEventRecord(cell_ev[2 * slot + 1], s.stream), "event record");
+            OFP_HIP(c, hipEventRecord(cell_ev[2 * (slot + 1)], s.stream), "event record");
+            OFP_HIP(c, hipEventRecord(cell_ev[2 * (slot + 1) + 1], s.stream), "event record");
+        }
+        if (cell_kind) {
+            cell_kind[slot] = 2;      // the pass's time
+            cell_kind[slot + 1] = 3;  // its second cell: no time of its own
+        }
+        p -= 2;
+    }
+    return OFP_OK;
+}
+
 // `rows` uncapped rows (cells W-1..1; the blocked input of cell W-1 is zero),
 // via a hipGraph captured once per (state, launch shape) and replayed.
 ofp_status full_rows(ofp_ctx* c, Sweep& s, int rows, int* launches) {
+    const bool pairs = pair_ready(c, s);
+    // one whole-state row: pair passes where they apply, else one launch per cell
+    auto row = [&](int* l) {
+        return pairs ? enqueue_pairs(c, s, s.W - 1, 1, false, l) : enqueue_cells(c, s, s.W - 1, 1, false, l);
+    };
     if (!c->use_graphs || s.fixed) {
         for (int r = 0; r < rows; ++r) {
-            ofp_status st = enqueue_cells(c, s, s.W - 1, 1, false, launches);
+            ofp_status st = row(launches);
             if (st) return st;
         }
         return OFP_OK;
     }
-    char key[256];
-    snprintf(key, sizeof key, "%p/%p/%d/%d/%d/%d/%d/%d/%d/%d/%d/%d", (void*)s.main, (void*)s.blk, s.n, s.K,
+    char key[320];
+    snprintf(key, sizeof key, "%p/%p/%d/%d/%d/%d/%d/%d/%d/%d/%d/%d/%d/%d/%u", (void*)s.main, (void*)s.blk, s.n, s.K,
              c->threads, c->blocks_per_cu, s.layout, c->low_cells, c->low_kc, c->variant * 100 + c->sorted_from,
-             c->xcd_shift, c->index_bits);
+             c->xcd_shift, c->index_bits, pairs ? 1 : 0, c->pair_kc, c->pair_mask);
     std::string k = key;
     for (uint32_t m : s.mods) k += "/" + std::to_string(m);
     if (k != s.row_key || !s.row_exec) {
         drop_graph(s);
         OFP_HIP(c, hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal), "begin capture");
-        int dummy = 0;
-        ofp_status st = enqueue_cells(c, s, s.W - 1, 1, false, &dummy);
+        int captured = 0;
+        ofp_status st = row(&captured);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(s.stream, &g);
         if (st) {
@@ -393,16 +486,11 @@ ofp_status full_rows(ofp_ctx* c, Sweep& s, int rows, int* launches) {
         s.row_graph = g;
         OFP_HIP(c, hipGraphInstantiate(&s.row_exec, g, nullptr, nullptr, 0), "graph instantiate");
         s.row_key = k;
-    }
-    int per_row = 0;
-    for (int p = s.W - 1; p >= 1; --p) {
-        int step = (p <= c->low_cells && c->low_kc < kMaxKC) ? c->low_kc : kMaxKC;
-        if (s.layout == 1 && s.K > 1 && (step & 1)) step = std::max(2, step - 1);
-        per_row += (s.K + step - 1) / step;
+        s.row_launches = captured;
     }
     for (int r = 0; r < rows; ++r) {
         OFP_HIP(c, hipGraphLaunch(s.row_exec, s.stream), "graph launch");
-        if (launches) *launches += per_row;
+        if (launches) *launches += s.row_launches;
     }
     return OFP_OK;
 }
@@ -618,6 +706,7 @@ ofp_status ofp_create(const ofp_config* cfg, ofp_ctx** out) {
         if (c->dp64) (void)hipFree(c->dp64);
         if (c->dpc32) (void)hipFree(c->dpc32);
         if (c->dpc64) (void)hipFree(c->dpc64);
+        if (c->ct32) (void)hipFree(c->ct32);
         if (c->ev0) (void)hipEventDestroy(c->ev0);
         if (c->ev1) (void)hipEventDestroy(c->ev1);
         if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -660,6 +749,14 @@ ofp_status ofp_create(const ofp_config* cfg, ofp_ctx** out) {
     if (bad(hipMalloc(&c->dpc32, bc32)) || bad(hipMalloc(&c->dpc64, bc64))) return OFP_E_RUNTIME;
     if (bad(hipMemcpy(c->dpc32, c32.data(), bc32, hipMemcpyHostToDevice)) ||
         bad(hipMemcpy(c->dpc64, c64.data(), bc64, hipMemcpyHostToDevice)))
+        return OFP_E_RUNTIME;
+    std::vector<unsigned char> t32(class_table_bytes(false) * kMaxW * 7);
+    const size_t bt32 = pack_class_tables(false, t32.data(), t32.size());
+    if (!bt32) {
+        (void)bad(hipErrorInvalidValue);
+        return OFP_E_RUNTIME;
+    }
+    if (bad(hipMalloc(&c->ct32, bt32)) || bad(hipMemcpy(c->ct32, t32.data(), bt32, hipMemcpyHostToDevice)))
         return OFP_E_RUNTIME;
     *out = c;
     return OFP_OK;
@@ -819,7 +916,8 @@ ofp_status ofp_time_cells(ofp_ctx* c, float* ms_out, int* kind_out, int capacity
     }
     for (int i = 0; i < cells; ++i) kind_out[i] = 0;
     // row -1: an arbitrary state, every cell over the whole state
-    ofp_status st = enqueue_cells(c, s, s.W - 1, 1, false, nullptr, -1, s.seg_ev.data(), kind_out);
+    ofp_status st = pair_ready(c, s) ? enqueue_pairs(c, s, s.W - 1, 1, false, nullptr, s.seg_ev.data(), kind_out)
+                                     : enqueue_cells(c, s, s.W - 1, 1, false, nullptr, -1, s.seg_ev.data(), kind_out);
     if (st) return st;
     OFP_HIP(c, hipEventSynchronize(s.seg_ev[2 * cells - 1]), "time cells sync");
     for (int i = 0; i < cells; ++i) {
@@ -828,6 +926,18 @@ ofp_status ofp_time_cells(ofp_ctx* c, float* ms_out, int* kind_out, int capacity
         ms_out[i] = ms;
     }
     if (s.row >= 0) s.row += 1;
+    return OFP_OK;
+}
+
+ofp_status ofp_set_pair_passes(ofp_ctx* c, int mode, uint32_t cell_mask) {
+    if (!c) return OFP_E_ARG;
+    if (mode < 0 || mode > kMaxKC || (mode > 1 && (mode & 1)))
+        return fail(c, OFP_E_ARG, "pair passes: 0 (off), 1 (on) or an even moduli count per launch up to 16");
+    const int on = mode > 0 ? 1 : 0;
+    if (on != c->pair_rows) drop_graph(c->st);
+    c->pair_rows = on;
+    c->pair_kc = mode > 1 ? mode : kMaxKC;
+    c->pair_mask = cell_mask;
     return OFP_OK;
 }
 
@@ -952,6 +1062,7 @@ ofp_status ofp_apply_cells(ofp_ctx* c, int n, const uint32_t* moduli, int k, uin
     if (p_hi > W - 1 || p_lo < 1 || p_hi < p_lo) return fail(c, OFP_E_ARG, "need W-1 >= p_hi >= p_lo >= 1");
     Sweep& sw = c->st;
     ofp_status s = ensure_state(c, sw, n, moduli, k);
+    const bool pairs = pairs_apply(c, n) && p_hi - p_lo >= 1;
     if (s) return s;
     // host state-major (index*k + j) <-> device layout
     const int Kp = sw.Kp, lay = sw.layout;
@@ -964,7 +1075,8 @@ ofp_status ofp_apply_cells(ofp_ctx* c, int n, const uint32_t* moduli, int k, uin
     OFP_HIP(c, hipMemcpy(sw.main, tm.data(), tm.size() * 4, hipMemcpyHostToDevice), "upload main");
     OFP_HIP(c, hipMemcpy(sw.blk, tb.data(), tb.size() * 4, hipMemcpyHostToDevice), "upload blocked");
     sw.row = -1;  // arbitrary state: no height caps
-    s = enqueue_cells(c, sw, p_hi, p_lo, blocked_valid != 0, nullptr);
+    s = pairs ? enqueue_pairs(c, sw, p_hi, p_lo, blocked_valid != 0, nullptr)
+              : enqueue_cells(c, sw, p_hi, p_lo, blocked_valid != 0, nullptr);
     if (s) return s;
     OFP_HIP(c, hipStreamSynchronize(sw.stream), "apply sync");
     OFP_HIP(c, hipMemcpy(tm.data(), sw.main, tm.size() * 4, hipMemcpyDeviceToHost), "download main");
@@ -1182,6 +1294,7 @@ void ofp_destroy(ofp_ctx* c) {
     if (c->dp64) (void)hipFree(c->dp64);
     if (c->dpc32) (void)hipFree(c->dpc32);
     if (c->dpc64) (void)hipFree(c->dpc64);
+    if (c->ct32) (void)hipFree(c->ct32);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -88,4 +88,15 @@ hipError_t launch_set_one(uint32_t* main, uint64_t index, int K, int k, int layo
 hipError_t launch_set_one_values(uint32_t* main, uint64_t index, int K, int k, int layout,
                                  const uint32_t* mods_host, hipStream_t s);
 
+// ---- pair pass (ofp_pair.hip; launch struct pair::PairLaunch in ofp_pair_impl.h) ----
+namespace pair { struct PairLaunch; }
+// Packed class tables for every (p, d), p in 0..kMaxW-1, d in -3..3, each
+// class_table_bytes(wide) long at offset (p*7 + d+3) * class_table_bytes(wide).
+size_t pack_class_tables(bool wide, void* out, size_t cap);
+size_t class_table_bytes(bool wide);
+// Classes of (p, d) in a row of width W.
+uint64_t pair_class_count(int W, int p, int d);
+// One pair-pass launch (a.kc in {1, 2, 4, ..., 16} moduli, 32-bit ranks).
+hipError_t launch_pair(const pair::PairLaunch& a, bool wide_index, int threads, int blocks, hipStream_t s);
+
 }  // namespace ofp

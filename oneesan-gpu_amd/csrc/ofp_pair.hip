@@ -1,0 +1,119 @@
+// ofp_pair.hip -- the pair pass: cells p and p-1 of a row in one out-of-place
+// launch per window height change d (body and design in ofp_pair_impl.h).
+//
+// Each thread takes one class (P', d, S'), decoded from its class number with
+// the class completion table of (p, d) and the height table, both in LDS; the
+// classes are dealt to the XCDs in pieces exactly as the cell kernels deal
+// their blocked words (cell::Sched).  Loads come from the old copy of the
+// state and stores go to the new one, so the classes need no ordering among
+// themselves and the launch has no atomics.
+#include <cstring>
+
+#include "ofp_pair_impl.h"
+
+namespace ofp {
+
+namespace {
+
+using namespace cell;
+
+template <int C, typename T>
+__device__ void load_rows(T* dst, const T* src) {
+    for (int t = threadIdx.x; t < kLdsRows * C; t += blockDim.x) dst[t] = src[t];
+}
+
+template <int KC, typename Idx, int D>
+__global__ __launch_bounds__(256) void pair_kernel(pair::PairLaunch a) {
+    constexpr int C = LdsCols<Idx>::value;
+    __shared__ Quad<Idx> tab[kLdsRows][C];
+    __shared__ Pair<Idx> ctab[kLdsRows][C];
+    const Quad<Idx>* g = (const Quad<Idx>*)a.dp;
+    for (int t = threadIdx.x; t < kLdsRows * C; t += blockDim.x) (&tab[0][0])[t] = g[(t / C) * kDpCols + t % C];
+    load_rows<C>(&ctab[0][0], (const Pair<Idx>*)a.ct);
+    __syncthreads();
+    Tab<Idx> T{tab};
+    const pair::Io<KC, Idx> io{a.main_in, a.blk_in, a.main_out, a.blk_out, a.K, a.off};
+    const Sched sc(a.classes, blockDim.x, gridDim.x, blockIdx.x, a.xcd_shift);
+    for (uint64_t t = sc.slot; t < sc.count; t += sc.nx) {
+        const uint64_t c = sc.chunk(t) * blockDim.x + threadIdx.x;
+        if (c < a.classes) {
+            const pair::Class<Idx> k = pair::decode_class<Idx>(T, ctab, a.W, a.p, c);
+            pair::pair_body<KC, Idx, D>(a, T, k, io);
+        }
+    }
+}
+
+template <int KC, typename Idx>
+hipError_t launch_d(const pair::PairLaunch& a, int threads, int blocks, hipStream_t s) {
+    switch (a.d) {
+        case -3: hipLaunchKernelGGL((pair_kernel<KC, Idx, -3>), dim3(blocks), dim3(threads), 0, s, a); break;
+        case -2: hipLaunchKernelGGL((pair_kernel<KC, Idx, -2>), dim3(blocks), dim3(threads), 0, s, a); break;
+        case -1: hipLaunchKernelGGL((pair_kernel<KC, Idx, -1>), dim3(blocks), dim3(threads), 0, s, a); break;
+        case 0: hipLaunchKernelGGL((pair_kernel<KC, Idx, 0>), dim3(blocks), dim3(threads), 0, s, a); break;
+        case 1: hipLaunchKernelGGL((pair_kernel<KC, Idx, 1>), dim3(blocks), dim3(threads), 0, s, a); break;
+        case 2: hipLaunchKernelGGL((pair_kernel<KC, Idx, 2>), dim3(blocks), dim3(threads), 0, s, a); break;
+        case 3: hipLaunchKernelGGL((pair_kernel<KC, Idx, 3>), dim3(blocks), dim3(threads), 0, s, a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+size_t pack_class_tables(bool wide, void* out, size_t cap) {
+    uint64_t d0[kDpRows][kDpCols];
+    build_height_table(d0);
+    const size_t one = (wide ? sizeof(Pair<uint64_t>) : sizeof(Pair<uint32_t>)) * kLdsRows *
+                       (wide ? LdsCols<uint64_t>::value : LdsCols<uint32_t>::value);
+    const size_t bytes = one * kMaxW * 7;
+    if (cap < bytes) return 0;
+    char* o = static_cast<char*>(out);
+    for (int p = 0; p < kMaxW; ++p)
+        for (int d = -3; d <= 3; ++d) {
+            char* dst = o + one * (size_t)(p * 7 + d + 3);
+            if (wide) {
+                static Pair<uint64_t> t[kDpRows][LdsCols<uint64_t>::value];
+                pair::pack_class_table<uint64_t, LdsCols<uint64_t>::value>(d0, p, d, t);
+                std::memcpy(dst, t, one);
+            } else {
+                static Pair<uint32_t> t[kDpRows][LdsCols<uint32_t>::value];
+                pair::pack_class_table<uint32_t, LdsCols<uint32_t>::value>(d0, p, d, t);
+                std::memcpy(dst, t, one);
+            }
+        }
+    return bytes;
+}
+
+size_t class_table_bytes(bool wide) {
+    return (wide ? sizeof(Pair<uint64_t>) * LdsCols<uint64_t>::value : sizeof(Pair<uint32_t>) * LdsCols<uint32_t>::value) *
+           kLdsRows;
+}
+
+uint64_t pair_class_count(int W, int p, int d) {
+    static uint64_t d0[kDpRows][kDpCols];
+    static bool ready = false;
+    if (!ready) {
+        build_height_table(d0);
+        ready = true;
+    }
+    return pair::class_count(d0, W, p, d);
+}
+
+hipError_t launch_pair(const pair::PairLaunch& a, bool wide, int threads, int blocks, hipStream_t s) {
+    if (a.p < 2 || a.p > a.W - 1 || a.d < -3 || a.d > 3 || a.kc < 1 || a.off + a.kc > a.K) return hipErrorInvalidValue;
+    if (wide) return hipErrorInvalidValue;  // 64-bit ranks are not built for the pair pass (n <= 22 only)
+    switch (a.kc) {
+        case 1: return launch_d<1, uint32_t>(a, threads, blocks, s);
+        case 2: return launch_d<2, uint32_t>(a, threads, blocks, s);
+        case 4: return launch_d<4, uint32_t>(a, threads, blocks, s);
+        case 6: return launch_d<6, uint32_t>(a, threads, blocks, s);
+        case 8: return launch_d<8, uint32_t>(a, threads, blocks, s);
+        case 10: return launch_d<10, uint32_t>(a, threads, blocks, s);
+        case 12: return launch_d<12, uint32_t>(a, threads, blocks, s);
+        case 14: return launch_d<14, uint32_t>(a, threads, blocks, s);
+        case 16: return launch_d<16, uint32_t>(a, threads, blocks, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace ofp

@@ -30,7 +30,7 @@ EXPORTS = (
     "ofp_job_create", "ofp_job_enqueue", "ofp_job_info_get", "ofp_job_result", "ofp_job_destroy",
     "ofp_shard_plan", "ofp_shard_sweep", "ofp_set_height_caps", "ofp_height_cap", "ofp_set_xcd_pieces",
     "ofp_set_sorted_cells", "ofp_time_cells", "ofp_set_index_width", "ofp_set_max_batch",
-    "ofp_moduli_groups", "ofp_solve_static",
+    "ofp_moduli_groups", "ofp_solve_static", "ofp_set_pair_passes",
 )
 
 
@@ -152,6 +152,7 @@ def load() -> ctypes.CDLL:
                                             ctypes.c_int]),
         "ofp_set_index_width": (ctypes.c_int32, [vp, ctypes.c_int]),
         "ofp_set_max_batch": (ctypes.c_int32, [vp, ctypes.c_int]),
+        "ofp_set_pair_passes": (ctypes.c_int32, [vp, ctypes.c_int, ctypes.c_uint32]),
         "ofp_solve_static": (ctypes.c_int32, [vp, ctypes.c_int, u32p, ctypes.c_int, u32p]),
         "ofp_moduli_groups": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                              ctypes.c_int]),
@@ -326,18 +327,28 @@ class Context:
         return list(out)
 
     def time_cells(self, n: int) -> list[tuple[int, float, str]]:
-        """One whole-state row of the prepared n-state, each cell launched and timed on
-        its own: [(p, ms, kernel)] for p = n..1 (advances the state by one row)."""
+        """One whole-state row of the prepared n-state, each cell or pair pass launched
+        and timed on its own: [(p, ms, kernel)] for p = n..1, kernel 'pair_kernel' for
+        the first cell of a pair pass (the pass's time) and 'pair_second' (0 ms) for
+        its second (advances the state by one row)."""
         cells = n
         ms = (ctypes.c_float * cells)()
         kind = (ctypes.c_int * cells)()
         self._check(self._lib.ofp_time_cells(self._h, ms, kind, cells), "ofp_time_cells")
-        names = ("cell_kernel", "cell_sorted_kernel")
+        names = ("cell_kernel", "cell_sorted_kernel", "pair_kernel", "pair_second")
         return [(cells - i, float(ms[i]), names[kind[i]]) for i in range(cells)]
 
     def set_index_width(self, bits: int) -> None:
         """0 = by n (32-bit ranks up to n=22), 64 = 64-bit ranks at every n."""
         self._check(self._lib.ofp_set_index_width(self._h, bits), "ofp_set_index_width")
+
+    def set_pair_passes(self, mode: bool | int, cell_mask: int = 0) -> None:
+        """Whole-state rows as pair passes (two cells per in-place pass) instead of
+        one launch per cell: False/0 off, True/1 on, an even 2..16 = on with at most
+        that many moduli per launch; cells p and p-1 pair up where bit p of
+        cell_mask is set (0: the library's default)."""
+        m = int(mode) if not isinstance(mode, bool) else (1 if mode else 0)
+        self._check(self._lib.ofp_set_pair_passes(self._h, m, cell_mask & 0xFFFFFFFF), "ofp_set_pair_passes")
 
     def set_max_batch(self, max_moduli: int) -> None:
         """At most max_moduli moduli per sweep of solve() (0 = as many as fit HBM)."""

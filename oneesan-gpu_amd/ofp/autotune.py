@@ -85,7 +85,38 @@ def apply(ctx: "_lib.Context", n: int, k: int) -> dict | None:
         ctx.set_height_caps(int(ent.get("cap_percent", _lib.DEFAULT_CAP_PERCENT)))
         ctx.set_xcd_pieces(int(ent.get("xcd_shift", 5)))
         ctx.set_sorted_cells(int(ent.get("sorted_from", -1)))
+        ctx.set_pair_passes(int(ent.get("pair_mode", 1)), int(ent.get("pair_mask", 0)))
     return ent
+
+
+def pair_stage(ctx: "_lib.Context", n: int, moduli: list[int], log=None) -> tuple[int, float]:
+    """Which cells pair up: one row with one launch per cell and one with a pair
+    pass over every pair (p, p-1), p = W-1, W-3, ..., both timed per launch on
+    whole-state rows; a pair is kept where its pass beats its two cells.  Returns
+    (cell mask, the row's time in ms with that choice); mask 0: no pair pays."""
+    ctx.set_height_caps(0)
+    ctx.set_pair_passes(0)
+    ctx.prepare(n, moduli)
+    ctx.time_cells(n)
+    single = {p: ms for p, ms, _ in ctx.time_cells(n)}
+    ctx.set_pair_passes(1, 0xFFFFFFFF)
+    ctx.time_cells(n)
+    paired = {p: ms for p, ms, name in ctx.time_cells(n) if name == "pair_kernel"}
+    mask, row = 0, 0.0
+    W = n + 1
+    p = W - 1
+    while p >= 1:
+        if p in paired and p - 1 >= 1 and paired[p] < single[p] + single[p - 1]:
+            mask |= 1 << p
+            row += paired[p]
+            p -= 2
+        else:
+            row += single[p]
+            p -= 1
+    if log:
+        log(f"autotune n={n} k={len(moduli)} pairs: mask {mask:#x}, row {row:.2f} ms "
+            f"(one launch per cell {sum(single.values()):.2f} ms)")
+    return mask, row
 
 
 def tune(ctx: "_lib.Context", n: int, moduli: list[int], rows: int = 1,
@@ -99,6 +130,7 @@ def tune(ctx: "_lib.Context", n: int, moduli: list[int], rows: int = 1,
     ctx.set_low_cells(0, 16)
     ctx.set_xcd_pieces(5)
     ctx.set_sorted_cells(0)
+    ctx.set_pair_passes(0)  # the launch-shape stages time the per-cell kernels
     for lay in layouts:
         ctx.set_layout(lay)
         ctx.prepare(n, moduli)
@@ -172,6 +204,13 @@ def tune(ctx: "_lib.Context", n: int, moduli: list[int], rows: int = 1,
             if sf == 0 or row_ms < best["row_ms"]:  # sf = 0 re-times the baseline on this state
                 best = ent
     ctx.set_sorted_cells(best["sorted_from"])
+    best = dict(best, pair_mode=0, pair_mask=0)
+    if best["layout"] == 1 and n <= 22:
+        mask, row = pair_stage(ctx, n, moduli, log)
+        table.append(dict(best, pair_mode=1 if mask else 0, pair_mask=mask, row_ms=row))
+        if mask:
+            best = dict(best, pair_mode=1, pair_mask=mask, row_ms=row)
+    ctx.set_pair_passes(best["pair_mode"], best["pair_mask"])
     best = dict(best, cap_percent=_lib.DEFAULT_CAP_PERCENT)
     rows_capped = min(CAPPED_ROWS, n - 1)
     if best["layout"] == 1 and rows_capped >= 1:
@@ -199,6 +238,7 @@ def tune(ctx: "_lib.Context", n: int, moduli: list[int], rows: int = 1,
                                       "cap_percent": best["cap_percent"],
                                       "xcd_shift": best["xcd_shift"],
                                       "sorted_from": best["sorted_from"],
+                                      "pair_mode": best["pair_mode"], "pair_mask": best["pair_mask"],
                                       "row_ms": best["row_ms"], "tuned_at": time.time()}
         save_cache(data, path)
     return result

@@ -58,9 +58,10 @@ def _kc(name: str) -> int | None:
 
 
 def kernel_family(name: str) -> str | None:
-    """'cell_sorted_kernel<14>' etc. for a cell kernel's full name (index width and
-    other template arguments dropped), None for any other kernel."""
-    m = re.search(r"(cell_(?:capped_|sorted_|shard_)?kernel)<(\d+)", name)
+    """'cell_sorted_kernel<14>', 'pair_kernel<14>' etc. for a cell or pair kernel's
+    full name (index width, window change and other template arguments dropped),
+    None for any other kernel."""
+    m = re.search(r"((?:cell_(?:capped_|sorted_|shard_)?|pair_)kernel)<(\d+)", name)
     return f"{m.group(1)}<{m.group(2)}>" if m else None
 
 
@@ -196,7 +197,9 @@ DRAM_COUNTERS = ("TCC_EA0_RDREQ_DRAM_32B_sum", "TCC_EA0_WRREQ_WRITE_DRAM_32B_sum
 
 
 def dram_bytes(path: str) -> dict[str, dict]:
-    """{family: {"read": B, "write": B, "dispatches": d}}, means per dispatch."""
+    """{family: {"read": B, "write": B, "dispatches": d, "read_total": B,
+    "write_total": B}}: means per dispatch and totals (a pair pass is seven
+    dispatches, one per window height change)."""
     per: dict[int, dict] = {}
     for r in csv.DictReader(open(path)):
         e = per.setdefault(int(r["Dispatch_Id"]), {"kernel": r["Kernel_Name"]})
@@ -212,6 +215,7 @@ def dram_bytes(path: str) -> dict[str, dict]:
         o["write"] += 32.0 * e[DRAM_COUNTERS[1]]
         o["dispatches"] += 1
     for o in out.values():
+        o["read_total"], o["write_total"] = o["read"], o["write"]
         o["read"] /= o["dispatches"]
         o["write"] /= o["dispatches"]
     return out

@@ -60,6 +60,7 @@ def test_tiny_tune_keeps_residues(gpu_ctx, tmp_path):
     ent = autotune.load_cache(path)["n=12,k=3"]
     assert ent["threads"] == 256 and ent["blocks_per_cu"] in (8, 32)
     assert res["best"]["cap_percent"] in autotune.CAP_PERCENTS
+    assert res["best"]["pair_mode"] in (0, 1) and "pair_mask" in ent
     got, _ = gpu_ctx.solve(12, mods)
     assert got == want
     # restore the library defaults for later tests
@@ -69,3 +70,4 @@ def test_tiny_tune_keeps_residues(gpu_ctx, tmp_path):
     gpu_ctx.set_xcd_pieces(5)
     gpu_ctx.set_sorted_cells(-1)
     gpu_ctx.set_height_caps(True)
+    gpu_ctx.set_pair_passes(1)

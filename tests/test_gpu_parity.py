@@ -54,6 +54,7 @@ def layout_ctx(request):
     layout, variant = request.param
     ctx = ofp.Context(layout=layout)
     ctx.set_variant(variant)
+    ctx.set_pair_passes(0)  # these cases test the one-launch-per-cell kernels
     yield ctx
     ctx.close()
 
@@ -178,6 +179,7 @@ def test_sorted_cells_same_residues(gpu_ctx, n):
     moduli (32-bit ranks) and n=23 with one (64-bit ranks, the reference's residue)."""
     mods = [4294967291, 4294966997, 2147483647, 65537] if n == 16 else [4294967291]
     try:
+        gpu_ctx.set_pair_passes(0)  # the sorted kernel runs on whole-state cells launched one by one
         gpu_ctx.set_sorted_cells(0)
         if n == 23:
             want = [2762394459]
@@ -190,6 +192,7 @@ def test_sorted_cells_same_residues(gpu_ctx, n):
             assert got == want, (n, sf)
     finally:
         gpu_ctx.set_sorted_cells(-1)  # the library's default
+        gpu_ctx.set_pair_passes(1)
 
 
 def test_xcd_pieces_same_residues(gpu_ctx):

@@ -89,8 +89,10 @@ def test_dram_bytes_per_family(tmp_path, capsys):
                         "Counter_Value": wr})
     got = rs.dram_bytes(str(p))
     assert set(got) == {"cell_sorted_kernel<14>", "cell_kernel<14>"}
-    assert got["cell_sorted_kernel<14>"] == {"read": 32.0 * 2000, "write": 32.0 * 600, "dispatches": 2}
-    assert got["cell_kernel<14>"] == {"read": 320.0, "write": 640.0, "dispatches": 1}
+    assert got["cell_sorted_kernel<14>"] == {"read": 32.0 * 2000, "write": 32.0 * 600, "dispatches": 2,
+                                             "read_total": 32.0 * 4000, "write_total": 32.0 * 1200}
+    assert got["cell_kernel<14>"] == {"read": 320.0, "write": 640.0, "dispatches": 1,
+                                      "read_total": 320.0, "write_total": 640.0}
     rs.dram(str(p))
     assert "cell_sorted_kernel<14>" in capsys.readouterr().out
 
@@ -100,3 +102,5 @@ def test_kernel_family_names():
     assert rs.kernel_family(CELL) == "cell_kernel<14>"
     assert rs.kernel_family(CAPPED) == "cell_capped_kernel<14>"
     assert rs.kernel_family("void ofp::fill_remote_kernel()") is None
+    assert rs.kernel_family("void ofp::(anonymous namespace)::pair_kernel<14, unsigned int, -1>"
+                            "(ofp::pair::PairLaunch)") == "pair_kernel<14>"
