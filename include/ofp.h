@@ -161,8 +161,11 @@ ofp_status ofp_time_cells(ofp_ctx* ctx, float* ms_out, int* kind_out, int capaci
  * mode: 0 = off, 1 = on (the default), 2..16 (even) = on with at most that
  * many moduli per launch (default 16).  Walking down a row, cells p and p-1 run
  * as one pair pass when bit p of cell_mask is set (~0u: everywhere), else cell
- * p runs alone; 0 = the library's default (pairs except over cells 4..3 with 8
- * or more moduli per launch, 6..3 with fewer; set by the autotuner).  Reference analogue: the reference's fused two-cell group kernels
+ * p runs alone; 0 = the library's default (pairs except over cells 4 and 3;
+ * the autotuner sets it per size).  Bits 8..9
+ * of mode select a kernel variant for 14 or 16 moduli per launch (tuning: bit 8
+ * = closure sums gathered before the class's words are loaded, bit 9 = a
+ * register budget for three workgroups per CU); results do not depend on it.  Reference analogue: the reference's fused two-cell group kernels
  * (row6crt20.cu:1033-1154), which run in group scratch. */
 ofp_status ofp_set_pair_passes(ofp_ctx* ctx, int mode, uint32_t cell_mask);
 

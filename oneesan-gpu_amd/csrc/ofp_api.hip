@@ -69,6 +69,7 @@ struct ofp_ctx {
     int pair_rows = 1;     // 1: full-state rows run as pair passes (two cells per pass, in place)
     int pair_kc = kMaxKC;  // moduli per pair-pass launch (even, <= 16)
     uint32_t pair_mask = 0;  // bit p: cells p and p-1 run as one pair pass; 0: the library's default
+    int pair_variant = 0;    // pair-kernel variant (PairLaunch::variant)
     int cap_percent = -1;  // height caps: cap a cell when it then sweeps <= this % of the blocked words
                            // (-1: 75 at one modulus, else 85)
     int xcd_shift = 5;     // chunk pieces per XCD: 2^xcd_shift chunks, < 0 = one eighth each
@@ -384,11 +385,11 @@ bool pair_ready(const ofp_ctx* c, const Sweep& s) { return pairs_apply(c, s.n) &
 // per-cell kernel.  The first cell reads the blocked input iff first_reads_blk.
 // With cell_ev, events bracket each step: a pair's first cell slot (slot
 // p_hi - p, kind 2) gets its time and the second an empty interval (kind 3).
-// The library's default pairing: every pair down to cell 5 with 8 or more moduli
-// per launch, down to cell 7 with fewer (at n=22, 14 moduli, the pass over cells
-// 4 and 3 took 142 ms against 127 ms as two launches; one modulus: the passes over
-// cells 6..3 took 25 and 30 ms against 24 and 26 ms), and cells 2 and 1 as a pair.
-uint32_t default_pair_mask(int k) { return k >= 8 ? ~((1u << 4) | (1u << 3)) : ~0x78u; }
+// The library's default pairing: every pair except the one over cells 4 and 3,
+// which runs as two launches (n=22: 125 against 60 + 66 ms at 14 moduli, 27.4
+// against 12.8 + 13.3 ms at one modulus; the other passes are 1.2-1.5x faster
+// than their two cells).
+uint32_t default_pair_mask(int) { return ~((1u << 4) | (1u << 3)); }
 
 ofp_status enqueue_pairs(ofp_ctx* c, Sweep& s, int p_hi, int p_lo, bool first_reads_blk, int* launches,
                          hipEvent_t* cell_ev = nullptr, int* cell_kind = nullptr) {
@@ -411,6 +412,9 @@ ofp_status enqueue_pairs(ofp_ctx* c, Sweep& s, int p_hi, int p_lo, bool first_re
             static const int kOrder[7] = {-1, 0, 1, -3, -2, 2, 3};
             for (int di = 0; di < 7; ++di) {
                 const int d = kOrder[di];
+                // d = -3 / +3: the class is one word, RRR or LLL, which neither cell
+                // changes (no target at (p, p-1) nor at (p-1, p-2)): nothing to do
+                if (d == -3 || d == 3) continue;
                 const uint64_t cnt = pair_class_count(W, p, d);
                 if (!cnt) continue;
                 pair::PairLaunch a{};
@@ -431,6 +435,7 @@ ofp_status enqueue_pairs(ofp_ctx* c, Sweep& s, int p_hi, int p_lo, bool first_re
                 // lanes past the k moduli (stride padding) compute with a valid modulus
                 for (int jj = 0; jj < kc; ++jj) a.mod[jj] = s.mods[off + jj < s.K ? off + jj : 0];
                 a.xcd_shift = piece_shift(c->xcd_shift, cnt, c->threads);
+                a.variant = c->pair_variant;
                 hipError_t e = launch_pair(a, false, c->threads, grid_blocks(c, cnt), s.stream);
                 if (e != hipSuccess) return hip_fail(c, e, "pair pass launch");
                 if (launches) ++*launches;
@@ -468,7 +473,7 @@ ofp_status full_rows(ofp_ctx* c, Sweep& s, int rows, int* launches) {
     char key[320];
     snprintf(key, sizeof key, "%p/%p/%d/%d/%d/%d/%d/%d/%d/%d/%d/%d/%d/%d/%u", (void*)s.main, (void*)s.blk, s.n, s.K,
              c->threads, c->blocks_per_cu, s.layout, c->low_cells, c->low_kc, c->variant * 100 + c->sorted_from,
-             c->xcd_shift, c->index_bits, pairs ? 1 : 0, c->pair_kc, c->pair_mask);
+             c->xcd_shift, c->index_bits, pairs ? 1 + c->pair_variant : 0, c->pair_kc, c->pair_mask);
     std::string k = key;
     for (uint32_t m : s.mods) k += "/" + std::to_string(m);
     if (k != s.row_key || !s.row_exec) {
@@ -931,8 +936,13 @@ ofp_status ofp_time_cells(ofp_ctx* c, float* ms_out, int* kind_out, int capacity
 
 ofp_status ofp_set_pair_passes(ofp_ctx* c, int mode, uint32_t cell_mask) {
     if (!c) return OFP_E_ARG;
+    // mode bits 8..9 select the pair-kernel variant (tuning)
+    const int variant = (mode >> 8) & 3;
+    mode &= 0xff;
     if (mode < 0 || mode > kMaxKC || (mode > 1 && (mode & 1)))
         return fail(c, OFP_E_ARG, "pair passes: 0 (off), 1 (on) or an even moduli count per launch up to 16");
+    if (variant != c->pair_variant) drop_graph(c->st);
+    c->pair_variant = variant;
     const int on = mode > 0 ? 1 : 0;
     if (on != c->pair_rows) drop_graph(c->st);
     c->pair_rows = on;

@@ -22,8 +22,10 @@ __device__ void load_rows(T* dst, const T* src) {
     for (int t = threadIdx.x; t < kLdsRows * C; t += blockDim.x) dst[t] = src[t];
 }
 
-template <int KC, typename Idx, int D>
-__global__ __launch_bounds__(256) void pair_kernel(pair::PairLaunch a) {
+// MB: workgroups of 256 per CU the register allocation must allow (2: at most
+// 256 VGPRs, no spills at 14 moduli; 3: at most 168, with spills).
+template <int KC, typename Idx, int D, int GF, int MB>
+__global__ __launch_bounds__(256, MB) void pair_kernel(pair::PairLaunch a) {
     constexpr int C = LdsCols<Idx>::value;
     __shared__ Quad<Idx> tab[kLdsRows][C];
     __shared__ Pair<Idx> ctab[kLdsRows][C];
@@ -38,24 +40,40 @@ __global__ __launch_bounds__(256) void pair_kernel(pair::PairLaunch a) {
         const uint64_t c = sc.chunk(t) * blockDim.x + threadIdx.x;
         if (c < a.classes) {
             const pair::Class<Idx> k = pair::decode_class<Idx>(T, ctab, a.W, a.p, c);
-            pair::pair_body<KC, Idx, D>(a, T, k, io);
+            pair::pair_body<KC, Idx, D, GF>(a, T, k, io);
         }
     }
 }
 
-template <int KC, typename Idx>
+template <int KC, typename Idx, int GF, int MB>
 hipError_t launch_d(const pair::PairLaunch& a, int threads, int blocks, hipStream_t s) {
     switch (a.d) {
-        case -3: hipLaunchKernelGGL((pair_kernel<KC, Idx, -3>), dim3(blocks), dim3(threads), 0, s, a); break;
-        case -2: hipLaunchKernelGGL((pair_kernel<KC, Idx, -2>), dim3(blocks), dim3(threads), 0, s, a); break;
-        case -1: hipLaunchKernelGGL((pair_kernel<KC, Idx, -1>), dim3(blocks), dim3(threads), 0, s, a); break;
-        case 0: hipLaunchKernelGGL((pair_kernel<KC, Idx, 0>), dim3(blocks), dim3(threads), 0, s, a); break;
-        case 1: hipLaunchKernelGGL((pair_kernel<KC, Idx, 1>), dim3(blocks), dim3(threads), 0, s, a); break;
-        case 2: hipLaunchKernelGGL((pair_kernel<KC, Idx, 2>), dim3(blocks), dim3(threads), 0, s, a); break;
-        case 3: hipLaunchKernelGGL((pair_kernel<KC, Idx, 3>), dim3(blocks), dim3(threads), 0, s, a); break;
+        case -3: hipLaunchKernelGGL((pair_kernel<KC, Idx, -3, GF, MB>), dim3(blocks), dim3(threads), 0, s, a); break;
+        case -2: hipLaunchKernelGGL((pair_kernel<KC, Idx, -2, GF, MB>), dim3(blocks), dim3(threads), 0, s, a); break;
+        case -1: hipLaunchKernelGGL((pair_kernel<KC, Idx, -1, GF, MB>), dim3(blocks), dim3(threads), 0, s, a); break;
+        case 0: hipLaunchKernelGGL((pair_kernel<KC, Idx, 0, GF, MB>), dim3(blocks), dim3(threads), 0, s, a); break;
+        case 1: hipLaunchKernelGGL((pair_kernel<KC, Idx, 1, GF, MB>), dim3(blocks), dim3(threads), 0, s, a); break;
+        case 2: hipLaunchKernelGGL((pair_kernel<KC, Idx, 2, GF, MB>), dim3(blocks), dim3(threads), 0, s, a); break;
+        case 3: hipLaunchKernelGGL((pair_kernel<KC, Idx, 3, GF, MB>), dim3(blocks), dim3(threads), 0, s, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+// Pair-kernel variant (PairLaunch::variant): bit 0 = gathers first, bit 1 = the
+// three-workgroups-per-CU register budget; only the 14- and 16-moduli kernels
+// carry the variants (the others run variant 0).
+template <int KC, typename Idx>
+hipError_t launch_gf(const pair::PairLaunch& a, int threads, int blocks, hipStream_t s) {
+    if constexpr (KC >= 14) {
+        switch (a.variant & 3) {
+            case 1: return launch_d<KC, Idx, 1, 2>(a, threads, blocks, s);
+            case 2: return launch_d<KC, Idx, 0, 3>(a, threads, blocks, s);
+            case 3: return launch_d<KC, Idx, 1, 3>(a, threads, blocks, s);
+            default: break;
+        }
+    }
+    return launch_d<KC, Idx, 0, 2>(a, threads, blocks, s);
 }
 
 }  // namespace
@@ -103,15 +121,15 @@ hipError_t launch_pair(const pair::PairLaunch& a, bool wide, int threads, int bl
     if (a.p < 2 || a.p > a.W - 1 || a.d < -3 || a.d > 3 || a.kc < 1 || a.off + a.kc > a.K) return hipErrorInvalidValue;
     if (wide) return hipErrorInvalidValue;  // 64-bit ranks are not built for the pair pass (n <= 22 only)
     switch (a.kc) {
-        case 1: return launch_d<1, uint32_t>(a, threads, blocks, s);
-        case 2: return launch_d<2, uint32_t>(a, threads, blocks, s);
-        case 4: return launch_d<4, uint32_t>(a, threads, blocks, s);
-        case 6: return launch_d<6, uint32_t>(a, threads, blocks, s);
-        case 8: return launch_d<8, uint32_t>(a, threads, blocks, s);
-        case 10: return launch_d<10, uint32_t>(a, threads, blocks, s);
-        case 12: return launch_d<12, uint32_t>(a, threads, blocks, s);
-        case 14: return launch_d<14, uint32_t>(a, threads, blocks, s);
-        case 16: return launch_d<16, uint32_t>(a, threads, blocks, s);
+        case 1: return launch_gf<1, uint32_t>(a, threads, blocks, s);
+        case 2: return launch_gf<2, uint32_t>(a, threads, blocks, s);
+        case 4: return launch_gf<4, uint32_t>(a, threads, blocks, s);
+        case 6: return launch_gf<6, uint32_t>(a, threads, blocks, s);
+        case 8: return launch_gf<8, uint32_t>(a, threads, blocks, s);
+        case 10: return launch_gf<10, uint32_t>(a, threads, blocks, s);
+        case 12: return launch_gf<12, uint32_t>(a, threads, blocks, s);
+        case 14: return launch_gf<14, uint32_t>(a, threads, blocks, s);
+        case 16: return launch_gf<16, uint32_t>(a, threads, blocks, s);
         default: return hipErrorInvalidValue;
     }
 }

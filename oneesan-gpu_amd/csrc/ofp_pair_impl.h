@@ -142,6 +142,7 @@ struct PairLaunch {
     int read_blk;      // blocked input of cell p is valid (false at the first cell of a row)
     int K, off, kc;    // interleave stride, first modulus, moduli in this launch
     int xcd_shift;
+    int variant;        // kernel variant: bit 0 = gathers first (pair_body's GF), bit 1 = tighter registers
     uint32_t mod[kMaxKC];
 };
 
@@ -322,8 +323,11 @@ OFP_HD void closures_after(const Tab<Idx>& T, int W, int p, const Class<Idx>& k,
     }
 }
 
-// The pass over one class of window change D.
-template <int KC, typename Idx, int D>
+// The pass over one class of window change D.  GF = 0: the class's words are
+// loaded first and the closure scans run while they are in flight; GF = 1: the
+// closure sums (the parts read from memory) are gathered first and the class's
+// words loaded after them, so that fewer values are live during the scans.
+template <int KC, typename Idx, int D, int GF = 0>
 OFP_HD void pair_body(const PairLaunch& a, const Tab<Idx>& T, const Class<Idx>& k, const Io<KC, Idx>& io) {
     using S = Slots<D>;
     const int W = a.W, p = a.p, h = k.h;
@@ -331,10 +335,10 @@ OFP_HD void pair_body(const PairLaunch& a, const Tab<Idx>& T, const Class<Idx>& 
 #pragma unroll
     for (int j = 0; j < KC; ++j) mod[j] = a.mod[j];
 
+    constexpr int yN = D == 0 ? 0 : (D == -1 ? 1 : (D == 1 ? 2 : -1));  // dh(yN) = D
     // indices and presence of the class's words
     uint64_t im[S::NM > 0 ? S::NM : 1];
     bool pm[S::NM > 0 ? S::NM : 1];
-    Vals<KC> m[S::NM > 0 ? S::NM : 1];
 #pragma unroll
     for (int i = 0; i < S::NM; ++i) {
         const int code = S::tri(i), x = code / 9, y = (code / 3) % 3, z = code % 3;
@@ -342,70 +346,136 @@ OFP_HD void pair_body(const PairLaunch& a, const Tab<Idx>& T, const Class<Idx>& 
         pm[i] = h1 >= 0 && h2 >= 0;
         im[i] = (uint64_t)(Idx)(k.mainbase + contrib(T, p, x, h) + contrib(T, p - 1, y, h1 < 0 ? 0 : h1) +
                                 contrib(T, p - 2, z, h2 < 0 ? 0 : h2) + k.sr);
-        if (pm[i]) io.lm(im[i], m[i]);
-        else vzero<KC>(m[i]);
     }
     uint64_t ib[S::NB > 0 ? S::NB : 1];
     bool pb[S::NB > 0 ? S::NB : 1];
-    Vals<KC> bl[S::NB > 0 ? S::NB : 1];
 #pragma unroll
     for (int i = 0; i < S::NB; ++i) {
         const int code = S::duo(i), x = code / 3, y = code % 3;
         const int h1 = h + dh(x);
         pb[i] = h1 >= 0;
         ib[i] = (uint64_t)(Idx)(k.blkbase + contrib(T, p - 1, x, h) + contrib(T, p - 2, y, h1 < 0 ? 0 : h1) + k.sr);
+    }
+    // closure sums read from memory: cell p's x=N word (sources before cell p)
+    // and cell p-1's x'=N word (sources in other classes, rebuilt as after cell p)
+    Vals<KC> clo, clo2;
+    vzero<KC>(clo);
+    vzero<KC>(clo2);
+    auto gather_p = [&]() {
+        if constexpr (yN >= 0) {
+            if (pb[S::b(0, yN)]) {
+                cell::Gather<KC, cell::kAoS> g;
+                g.init();
+                const cell::Direct<KC, cell::kAoS, Idx> old{const_cast<uint32_t*>(io.mi),
+                                                            const_cast<uint32_t*>(io.bi), io.K, io.off};
+                const Idx srp = contrib(T, p - 2, yN, h) + k.sr;  // rank of S = (y, S') entered at h
+                cell::gather_closures<KC, Idx, cell::kAoS>(T, W, p, h, k.mainbase, srp, k.word, old, g, mod);
+                g.finish(old, mod);
+                clo = g.acc;
+            }
+        }
+    };
+    auto gather_p1 = [&]() {
+        if constexpr (yN >= 0) {
+            if (pb[S::b(yN, 0)]) closures_after<KC, Idx>(T, W, p, k, yN, a.read_blk != 0, io, clo2, mod);
+        }
+    };
+    if constexpr (GF == 1) {
+        gather_p();
+        gather_p1();
+    }
+    Vals<KC> m[S::NM > 0 ? S::NM : 1];
+#pragma unroll
+    for (int i = 0; i < S::NM; ++i) {
+        if (pm[i]) io.lm(im[i], m[i]);
+        else vzero<KC>(m[i]);
+    }
+    Vals<KC> bl[S::NB > 0 ? S::NB : 1];
+#pragma unroll
+    for (int i = 0; i < S::NB; ++i) {
         if (pb[i] && a.read_blk) io.lb(ib[i], bl[i]);
         else vzero<KC>(bl[i]);
     }
 
+    // Cell-by-cell, the orbits whose blocked word is x=R or x=L first and the x=N
+    // orbit (whose blocked output is the closure sum) last, so that the closure
+    // scans run with as few values live as possible; bl[i] holds, in turn, the
+    // blocked word before cell p, after cell p and after cell p-1.  A word is
+    // stored as soon as it is final.
+    //
+    // A word [P', a, b, c, S'] keeps its value when neither (a, b) is a target of
+    // cell p nor (b, c) one of cell p-1 (targets: NN, LR, NR, NL; at cell 1 also
+    // RN): RRN, RRR, RRL, LLN, LLL, RLN, RLL.  Those are not stored.
+
     // ---- cell p: orbits of the blocked words (x, y) (x at p-1, y at p-2) ----
-    // closure sources of the x=N word (y = the symbol with dh(y) = D), before cell p
-    Vals<KC> clo;
-    vzero<KC>(clo);
-    constexpr int yN = D == 0 ? 0 : (D == -1 ? 1 : (D == 1 ? 2 : -1));
-    if constexpr (yN >= 0) {
-        if (pb[S::b(0, yN)]) {
-            cell::Gather<KC, cell::kAoS> g;
-            g.init();
-            const cell::Direct<KC, cell::kAoS, Idx> old{const_cast<uint32_t*>(io.mi), const_cast<uint32_t*>(io.bi),
-                                                        io.K, io.off};
-            const Idx srp = contrib(T, p - 2, yN, h) + k.sr;  // rank of S = (y, S') entered at h
-            cell::gather_closures<KC, Idx, cell::kAoS>(T, W, p, h, k.mainbase, srp, k.word, old, g, mod);
-            g.finish(old, mod);
-            clo = g.acc;
-        }
-    }
-    Vals<KC> mid[S::NB > 0 ? S::NB : 1];  // blocked words after cell p
 #pragma unroll
     for (int i = 0; i < S::NB; ++i) {
         const int code = S::duo(i), x = code / 3, y = code % 3;
-        if (x == 0) {
-            const int iNN = S::m(0, 0, y), iLR = S::m(2, 1, y);
-            const Vals<KC> nn = m[iNN];
-            vadd<KC>(m[iNN], bl[i], mod);
-            vadd<KC>(m[iLR], nn, mod);
-            mid[i] = clo;
-        } else if (x == 1) {
+        if (x == 1) {
             const int iNR = S::m(0, 1, y), iRN = S::m(1, 0, y);
-            mid[i] = m[iNR];
+            const Vals<KC> nr = m[iNR];
             vadd<KC>(m[iNR], m[iRN], mod);
             vadd<KC>(m[iNR], bl[i], mod);
-        } else {
+            bl[i] = nr;
+        } else if (x == 2) {
             const int iNL = S::m(0, 2, y), iLN = S::m(2, 0, y);
-            mid[i] = m[iNL];
+            const Vals<KC> nl = m[iNL];
             vadd<KC>(m[iNL], m[iLN], mod);
             vadd<KC>(m[iNL], bl[i], mod);
+            bl[i] = nl;
         }
+    }
+    if constexpr (yN >= 0) {
+        constexpr int i = S::b(0, yN);
+        constexpr int iNN = S::m(0, 0, yN), iLR = S::m(2, 1, yN);
+        vadd<KC>(m[iLR], m[iNN], mod);
+        vadd<KC>(m[iNN], bl[i], mod);
+        if constexpr (GF == 0) gather_p();
+        bl[i] = clo;  // blocked output: the closure sum
     }
 
     // ---- cell p-1: orbits of the blocked words (u, x') (u at p, x' at p-2) ----
-    constexpr int uN = yN;  // the x'=N word has dh(u) = D
-    Vals<KC> clo2;
-    vzero<KC>(clo2);
-    if constexpr (uN >= 0) {
-        if (pb[S::b(uN, 0)]) {
+    auto store_main = [&](int i) {
+        if (pm[i]) io.sm(im[i], m[i]);
+    };
+    auto store_blk = [&](int i) {
+        if (p - 1 >= 2 && pb[i]) io.sb(ib[i], bl[i]);
+    };
+#pragma unroll
+    for (int i = 0; i < S::NB; ++i) {
+        const int code = S::duo(i), u = code / 3, x = code % 3;
+        if (x == 1) {
+            const int iNR = S::m(u, 0, 1), iRN = S::m(u, 1, 0);
+            const Vals<KC> nr = m[iNR];
+            vadd<KC>(m[iNR], m[iRN], mod);
+            vadd<KC>(m[iNR], bl[i], mod);
+            bl[i] = nr;
+            if (p - 1 == 1) vadd<KC>(m[iRN], nr, mod);
+            store_main(iNR);
+            // RN is a target of cell p when u = N... it is [P', u, R, N]: (u, R) at
+            // cell p is NR (u = N) or LR (u = L), changed; RRN keeps (cell 1 aside)
+            if (u != 1 || p - 1 == 1) store_main(iRN);
+            store_blk(i);
+        } else if (x == 2) {
+            const int iNL = S::m(u, 0, 2), iLN = S::m(u, 2, 0);
+            const Vals<KC> nl = m[iNL];
+            vadd<KC>(m[iNL], m[iLN], mod);
+            vadd<KC>(m[iNL], bl[i], mod);
+            bl[i] = nl;
+            store_main(iNL);
+            // LN = [P', u, L, N]: (u, L) at cell p is NL (u = N, changed), RL or LL (keep)
+            if (u == 0) store_main(iLN);
+            store_blk(i);
+        }
+    }
+    if constexpr (yN >= 0) {
+        constexpr int uN = yN;  // the x'=N word has dh(u) = D
+        constexpr int i = S::b(uN, 0);
+        constexpr int iNN = S::m(uN, 0, 0), iLR = S::m(uN, 2, 1);
+        if (pb[i]) {
             // in-class sources, after cell p: RL = [P', u, R, L, S'] (cell p-1 >= 2),
             // and with u = R the source with u itself turned into L: [P', L, R, R, S']
+            // (neither is a target of cell p-1)
             if (p - 1 >= 2) {
                 constexpr int iRL = S::m(uN, 1, 2);
                 if (iRL >= 0 && pm[iRL]) vadd<KC>(clo2, m[iRL], mod);
@@ -414,51 +484,24 @@ OFP_HD void pair_body(const PairLaunch& a, const Tab<Idx>& T, const Class<Idx>& 
                 constexpr int iLRR = S::m(2, 1, 1);
                 if (pm[iLRR]) vadd<KC>(clo2, m[iLRR], mod);
             }
-            closures_after<KC, Idx>(T, W, p, k, uN, a.read_blk != 0, io, clo2, mod);
+            if constexpr (GF == 0) gather_p1();
         }
+        vadd<KC>(m[iLR], m[iNN], mod);
+        vadd<KC>(m[iNN], bl[i], mod);
+        if (p - 1 == 1) vadd<KC>(m[iNN], clo2, mod);
+        bl[i] = clo2;
+        store_main(iNN);
+        store_main(iLR);
+        store_blk(i);
     }
-#pragma unroll
-    for (int i = 0; i < S::NB; ++i) {
-        const int code = S::duo(i), u = code / 3, x = code % 3;
-        if (x == 0) {
-            const int iNN = S::m(u, 0, 0), iLR = S::m(u, 2, 1);
-            const Vals<KC> nn = m[iNN];
-            vadd<KC>(m[iNN], mid[i], mod);
-            if (p - 1 == 1) vadd<KC>(m[iNN], clo2, mod);
-            vadd<KC>(m[iLR], nn, mod);
-            bl[i] = clo2;
-        } else if (x == 1) {
-            const int iNR = S::m(u, 0, 1), iRN = S::m(u, 1, 0);
-            const Vals<KC> nr = m[iNR];
-            bl[i] = nr;
-            vadd<KC>(m[iNR], m[iRN], mod);
-            vadd<KC>(m[iNR], mid[i], mod);
-            if (p - 1 == 1) vadd<KC>(m[iRN], nr, mod);
-        } else {
-            const int iNL = S::m(u, 0, 2), iLN = S::m(u, 2, 0);
-            bl[i] = m[iNL];
-            vadd<KC>(m[iNL], m[iLN], mod);
-            vadd<KC>(m[iNL], mid[i], mod);
-        }
-    }
-
-    // ---- write the class back: every word the two cells change ----
-    // A word [P', a, b, c, S'] keeps its value when neither (a, b) is a target of
-    // cell p nor (b, c) one of cell p-1 (targets: NN, LR, NR, NL; at cell 1 also
-    // RN): RRN, RRR, RRL, LLN, LLL, RLN, RLL.  Their stores are skipped.
+    // words outside every cell-(p-1) orbit ((b, c) = RR, LL or RL) that cell p
+    // changed: (a, b) = NR, NL or LR, i.e. NRR, NRL, NLL, LRR, LRL
 #pragma unroll
     for (int i = 0; i < S::NM; ++i) {
         const int code = S::tri(i);
-        const bool keeps = code == 1 * 9 + 1 * 3 + 0 || code == 1 * 9 + 1 * 3 + 1 || code == 1 * 9 + 1 * 3 + 2 ||
-                           code == 2 * 9 + 2 * 3 + 0 || code == 2 * 9 + 2 * 3 + 2 || code == 1 * 9 + 2 * 3 + 0 ||
-                           code == 1 * 9 + 2 * 3 + 2;
-        const bool rrn = code == 1 * 9 + 1 * 3 + 0;
-        if (pm[i] && (!keeps || (rrn && p - 1 == 1))) io.sm(im[i], m[i]);
-    }
-    if (p - 1 >= 2) {
-#pragma unroll
-        for (int i = 0; i < S::NB; ++i)
-            if (pb[i]) io.sb(ib[i], bl[i]);
+        if (code == 0 * 9 + 1 * 3 + 1 || code == 0 * 9 + 1 * 3 + 2 || code == 0 * 9 + 2 * 3 + 2 ||
+            code == 2 * 9 + 1 * 3 + 1 || code == 2 * 9 + 1 * 3 + 2)
+            store_main(i);
     }
 }
 

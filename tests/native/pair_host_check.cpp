@@ -30,7 +30,7 @@ static void build_d() {
             D0[w][h] = D0[w - 1][h] + (h > 0 ? D0[w - 1][h - 1] : 0) + (h + 1 < kDpCols ? D0[w - 1][h + 1] : 0);
 }
 
-template <int KC, typename Idx, int D>
+template <int KC, typename Idx, int D, int GF>
 static void run_d(const pair::PairLaunch& a, const cell::Tab<Idx>& T, const pair::Io<KC, Idx>& io, bool rev) {
     constexpr int C = cell::LdsCols<Idx>::value;
     static cell::Pair<Idx> ct[kDpRows][C];
@@ -39,11 +39,11 @@ static void run_d(const pair::PairLaunch& a, const cell::Tab<Idx>& T, const pair
     for (uint64_t t = 0; t < n; ++t) {
         const uint64_t c = rev ? n - 1 - t : t;
         const pair::Class<Idx> k = pair::decode_class<Idx>(T, ct, a.W, a.p, c);
-        pair::pair_body<KC, Idx, D>(a, T, k, io);
+        pair::pair_body<KC, Idx, D, GF>(a, T, k, io);
     }
 }
 
-template <int KC, typename Idx>
+template <int KC, typename Idx, int GF>
 static int check(int n, int p, bool read_blk, const uint32_t* mods, unsigned seed, bool rev) {
     const int W = n + 1;
     const uint64_t nm = D0[W][1], nb = D0[W - 1][1];
@@ -72,13 +72,13 @@ static int check(int n, int p, bool read_blk, const uint32_t* mods, unsigned see
     std::vector<uint32_t>& mo = mi;
     std::vector<uint32_t>& bo = bi;
     pair::Io<KC, Idx> io{mi.data(), bi.data(), mi.data(), bi.data(), KC, 0};
-    run_d<KC, Idx, -1>(a, T, io, rev);
-    run_d<KC, Idx, 0>(a, T, io, rev);
-    run_d<KC, Idx, 1>(a, T, io, rev);
-    run_d<KC, Idx, -3>(a, T, io, rev);
-    run_d<KC, Idx, -2>(a, T, io, rev);
-    run_d<KC, Idx, 2>(a, T, io, rev);
-    run_d<KC, Idx, 3>(a, T, io, rev);
+    run_d<KC, Idx, -1, GF>(a, T, io, rev);
+    run_d<KC, Idx, 0, GF>(a, T, io, rev);
+    run_d<KC, Idx, 1, GF>(a, T, io, rev);
+    run_d<KC, Idx, -3, GF>(a, T, io, rev);
+    run_d<KC, Idx, -2, GF>(a, T, io, rev);
+    run_d<KC, Idx, 2, GF>(a, T, io, rev);
+    run_d<KC, Idx, 3, GF>(a, T, io, rev);
     for (int j = 0; j < KC; ++j) {
         std::vector<uint32_t> em(nm), eb(nb ? nb : 1);
         for (uint64_t i = 0; i < nm; ++i) em[i] = m0[i * KC + j];
@@ -98,8 +98,8 @@ static int check(int n, int p, bool read_blk, const uint32_t* mods, unsigned see
                     return 1;
                 }
     }
-    printf("ok n=%d p=%d blk=%d idx=%s k=%d order=%s\n", n, p, read_blk, sizeof(Idx) == 8 ? "u64" : "u32", KC,
-           rev ? "reverse" : "forward");
+    printf("ok n=%d p=%d blk=%d idx=%s k=%d order=%s gathers_first=%d\n", n, p, read_blk,
+           sizeof(Idx) == 8 ? "u64" : "u32", KC, rev ? "reverse" : "forward", GF);
     return 0;
 }
 
@@ -112,8 +112,9 @@ int main(int argc, char** argv) {
         for (int p = W - 1; p >= 2; --p)
             for (int rb = 0; rb < 2; ++rb)
                 for (int rev = 0; rev < 2; ++rev) {
-                    if (check<3, uint32_t>(n, p, rb, mods, 1000u * n + 10u * p + rb, rev)) return 1;
-                    if (n % 3 == 0 && check<3, uint64_t>(n, p, rb, mods, 7000u * n + 10u * p + rb, rev)) return 1;
+                    if (check<3, uint32_t, 0>(n, p, rb, mods, 1000u * n + 10u * p + rb, rev)) return 1;
+                    if (check<3, uint32_t, 1>(n, p, rb, mods, 3000u * n + 10u * p + rb, rev)) return 1;
+                    if (n % 3 == 0 && check<3, uint64_t, 0>(n, p, rb, mods, 7000u * n + 10u * p + rb, rev)) return 1;
                 }
     }
     printf("ALL OK\n");
