@@ -290,9 +290,55 @@ typedef struct ofp_shard_info {
 /* Window of a shard (no device work; ctx may be NULL). */
 ofp_status ofp_shard_plan(ofp_ctx* ctx, const ofp_shard_spec* spec, ofp_shard_info* info);
 /* main_out (main_count*k) / blocked_out (blocked_count*k): optional host copies
- * of the window after the sweep, state-major. */
+ * of the window after the sweep, state-major.  Shard and group sweeps keep
+ * their window buffer in the context from call to call (streaming windows of
+ * one size re-touches no pages and frees nothing); ofp_destroy frees it, and
+ * so does a full-state sweep (ofp_solve, ofp_prepare, ...) before sizing its
+ * own state. */
 ofp_status ofp_shard_sweep(ofp_ctx* ctx, const ofp_shard_spec* spec, ofp_shard_info* info,
                            uint32_t* main_out, uint32_t* blocked_out);
+
+/* ---- group sweep: a window of the high cells of a row ----
+ * The high cells p = W-1..low+1 of a row rewrite positions >= low only and
+ * keep the occupancy (N or not) of the low positions 0..low-1, so the words
+ * with one occupancy mask of those positions (a group) are closed under them.
+ * A window is the groups first_group..first_group+groups-1 (masks in
+ * increasing order).  Each group is stored contiguously with its words in
+ * global rank order (32-bit ranks local to the group: every group of the
+ * window must hold fewer than 2^32 words); the window's main array is the
+ * groups' main words one group after the other, its blocked array likewise.
+ * The cells run in place on the window from a row's start (the first cell reads
+ * no blocked input); the last cell's blocked output is the window's blocked
+ * array.  Reference analogue: one GPU's owner-local HIGH window of the 8-GPU
+ * bucket layout (docs/b300-bucket-snake.md:7-20; docs/b300-hbm32.md:251-270:
+ * W = 28, LOW14 / HIGH13, 16,384 groups). */
+typedef struct ofp_group_spec {
+    int n;
+    int k;                     /* 1..16 moduli, interleaved */
+    const uint32_t* moduli;
+    int low;                   /* low positions 0..low-1, 1..min(W-2, 24); 0 = W/2 */
+    uint32_t first_group, groups;
+    uint64_t seed;             /* seeded inputs when main_io is NULL */
+} ofp_group_spec;
+typedef struct ofp_group_info {
+    uint64_t main_count, blocked_count;  /* words of the window */
+    uint64_t largest_group;              /* main + blocked words of its largest group */
+    uint64_t state_bytes;                /* device bytes of the window */
+    int low, cells, launches;            /* cells = W-1-low */
+    double kernel_s;                     /* all cells, HIP events */
+    float cell_ms[OFP_MAX_N];            /* cell_ms[i]: cell W-1-i over every group */
+} ofp_group_info;
+/* Window totals into info, and per-group counts into main_counts /
+ * blocked_counts (groups entries each; either may be NULL).  No device work;
+ * ctx may be NULL. */
+ofp_status ofp_group_plan(ofp_ctx* ctx, const ofp_group_spec* spec, ofp_group_info* info,
+                          uint64_t* main_counts, uint64_t* blocked_counts);
+/* main_io: NULL for seeded inputs (uniform in [0, p)), else main_count*k
+ * residues (state-major, window order) that are replaced by the result.
+ * blocked_out: NULL or blocked_count*k, the window's blocked words after cell
+ * low+1. */
+ofp_status ofp_group_sweep(ofp_ctx* ctx, const ofp_group_spec* spec, ofp_group_info* info,
+                           uint32_t* main_io, uint32_t* blocked_out);
 
 /* Counts only, no device needed: main / blocked word counts for n. */
 ofp_status ofp_state_counts(int n, uint64_t* main_states, uint64_t* blocked_states);

@@ -77,6 +77,8 @@ struct ofp_ctx {
                            // -1: from cell 6 for launches of 8 or more moduli, else none)
     int index_bits = 0;    // 0: 32-bit ranks for W <= 23, 64-bit above; 64: 64-bit ranks always
     int max_batch = 0;     // moduli per sweep of ofp_solve (0: as many as fit HBM)
+    char* win = nullptr;   // window buffer of shard and group sweeps, reused from call to call
+    size_t win_bytes = 0;
     void* scratch = nullptr;  // device address of the static state array on this device
     Sweep fixed;           // ofp_solve_static's state, inside that array
     std::string err;
@@ -200,6 +202,49 @@ int grid_blocks(const ofp_ctx* c, uint64_t nblk) {
     return (int)std::max<uint64_t>(b, 8);
 }
 
+// The window buffer of shard and group sweeps: kept from call to call, so that
+// streaming windows of one size through HBM neither frees a touched buffer nor
+// touches fresh pages per window (each costs about a second per 100 GB).
+// Grown on demand; a full-state sweep gives it up (release_window) before it
+// sizes its own state.
+ofp_status window_buffer(ofp_ctx* c, size_t bytes, char** out) {
+    bytes = (bytes + 255) / 256 * 256;
+    if (c->win && c->win_bytes >= bytes) {
+        *out = c->win;
+        return OFP_OK;
+    }
+    if (c->win) {
+        OFP_HIP(c, hipFree(c->win), "free window buffer");
+        c->win = nullptr;
+        c->win_bytes = 0;
+    }
+    size_t fr = 0, tot = 0;
+    OFP_HIP(c, hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
+    if (bytes + c->reserve > fr) return fail(c, OFP_E_MEMORY, "window does not fit HBM");
+    OFP_HIP(c, hipMalloc(&c->win, bytes), "window buffer");
+    c->win_bytes = bytes;
+    *out = c->win;
+    return OFP_OK;
+}
+
+void release_window(ofp_ctx* c) {
+    if (c->win) (void)hipFree(c->win);
+    c->win = nullptr;
+    c->win_bytes = 0;
+}
+
+// Carve consecutive 256-byte aligned pieces out of a window buffer.
+struct Carve {
+    char* base;
+    size_t used = 0;
+    template <typename T>
+    T* take(size_t bytes) {
+        T* p = reinterpret_cast<T*>(base + used);
+        used += (std::max<size_t>(bytes, 1) + 255) / 256 * 256;
+        return p;
+    }
+};
+
 // Ensure device buffers of s for (n, moduli) in the context's current layout.
 // Whether n in the context's settings runs its full rows as pair passes: pair
 // mode on, state-major layout, 32-bit ranks (n <= 22), at least two cells.
@@ -223,6 +268,7 @@ ofp_status ensure_state(ofp_ctx* c, Sweep& s, int n, const uint32_t* moduli, int
         sweep_free(s, s.blk);
         s.main = s.blk = nullptr;
         s.main_cap = s.blk_cap = 0;
+        release_window(c);
         size_t fr = 0, tot = 0;
         OFP_HIP(c, hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
         if (mb + bb + c->reserve > fr)
@@ -973,7 +1019,7 @@ ofp_status ofp_solve(ofp_ctx* c, int n, const uint32_t* moduli, int k, uint32_t*
     size_t fr = 0, tot = 0;
     OFP_HIP(c, hipSetDevice(c->device), "hipSetDevice");
     OFP_HIP(c, hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
-    const uint64_t avail = fr + c->st.main_cap + c->st.blk_cap;
+    const uint64_t avail = fr + c->st.main_cap + c->st.blk_cap + c->win_bytes;
     int g = c->max_batch > 0 ? std::min(k, c->max_batch) : k;
     while (g > 1 && state_bytes(c->layout, n, g) + c->reserve > avail) --g;
     if (state_bytes(c->layout, n, g) + c->reserve > avail)
@@ -1139,26 +1185,21 @@ ofp_status ofp_shard_sweep(ofp_ctx* c, const ofp_shard_spec* sp, ofp_shard_info*
     const int W = sp->n + 1, k = sp->k, Kp = interleave_stride(1, k);
     const uint64_t M = info->main_count, B = info->blocked_count;
     OFP_HIP(c, hipSetDevice(c->device), "hipSetDevice");
-    size_t fr = 0, tot = 0;
-    OFP_HIP(c, hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
-    if (info->state_bytes + c->reserve > fr) return fail(c, OFP_E_MEMORY, "shard window does not fit HBM");
-    uint32_t *dm = nullptr, *db = nullptr, *dmods = nullptr;
-    unsigned long long* dstray = nullptr;
-    auto release = [&]() {
-        if (dm) (void)hipFree(dm);
-        if (db) (void)hipFree(db);
-        if (dmods) (void)hipFree(dmods);
-        if (dstray) (void)hipFree(dstray);
-    };
+    const size_t mbytes = std::max<uint64_t>(M, 1) * Kp * 4, bbytes = std::max<uint64_t>(B, 1) * Kp * 4;
+    char* wbuf = nullptr;
+    st = window_buffer(c, (mbytes + 255) / 256 * 256 + (bbytes + 255) / 256 * 256 + 512, &wbuf);
+    if (st) return st;
+    Carve cv{wbuf};
+    uint32_t* dm = cv.take<uint32_t>(mbytes);
+    uint32_t* db = cv.take<uint32_t>(bbytes);
+    uint32_t* dmods = cv.take<uint32_t>(16 * 4);
+    unsigned long long* dstray = cv.take<unsigned long long>(sizeof(unsigned long long));
+    auto release = [&]() {};
 #define OFP_HIP_R(expr, what)                                        \
     do {                                                             \
         hipError_t e_ = (expr);                                      \
         if (e_ != hipSuccess) { release(); return hip_fail(c, e_, what); } \
     } while (0)
-    OFP_HIP_R(hipMalloc(&dm, std::max<uint64_t>(M, 1) * Kp * 4), "shard main window");
-    OFP_HIP_R(hipMalloc(&db, std::max<uint64_t>(B, 1) * Kp * 4), "shard blocked window");
-    OFP_HIP_R(hipMalloc(&dmods, 16 * 4), "shard moduli");
-    OFP_HIP_R(hipMalloc(&dstray, sizeof(unsigned long long)), "shard counter");
     OFP_HIP_R(hipMemcpy(dmods, sp->moduli, k * 4, hipMemcpyHostToDevice), "copy moduli");
     OFP_HIP_R(hipMemset(dstray, 0, sizeof(unsigned long long)), "zero counter");
     hipError_t e = launch_fill_remote(dm, info->first_main, M, db, info->first_blocked, B, Kp, k, dmods,
@@ -1213,6 +1254,175 @@ ofp_status ofp_shard_sweep(ofp_ctx* c, const ofp_shard_spec* sp, ofp_shard_info*
         }
         if (blk_out && B) {
             OFP_HIP_R(hipMemcpy(t.data(), db, B * Kp * 4, hipMemcpyDeviceToHost), "download blocked");
+            for (uint64_t i = 0; i < B; ++i)
+                for (int j = 0; j < k; ++j) blk_out[i * k + j] = t[i * Kp + j];
+        }
+    }
+#undef OFP_HIP_R
+    release();
+    return OFP_OK;
+}
+
+// ---- group sweeps: the high cells of a row on low-occupancy groups ----
+
+namespace {
+
+ofp_status group_check(ofp_ctx* c, const ofp_group_spec* sp, int* low) {
+    if (!sp || !sp->moduli) return fail(c, OFP_E_ARG, "group spec and moduli are required");
+    if (sp->n < 2 || sp->n > OFP_MAX_N) return fail(c, OFP_E_RANGE, "group sweeps take n in 2..27");
+    const int W = sp->n + 1;
+    const int L = sp->low ? sp->low : W / 2;
+    if (L < 1 || L > W - 2 || L > 24) return fail(c, OFP_E_ARG, "low must be in 1..min(W-2, 24)");
+    if (sp->k < 1 || sp->k > kMaxKC) return fail(c, OFP_E_ARG, "group sweeps take 1..16 moduli");
+    const uint64_t all = 1ull << L;
+    if (sp->groups < 1 || sp->first_group >= all || sp->groups > all - sp->first_group)
+        return fail(c, OFP_E_ARG, "groups must lie in 0..2^low-1");
+    *low = L;
+    return OFP_OK;
+}
+
+constexpr size_t kGroupTableBytes = sizeof(cell::Quad<uint32_t>) * kDpRows * kDpCols;
+
+}  // namespace
+
+ofp_status ofp_group_plan(ofp_ctx* c, const ofp_group_spec* sp, ofp_group_info* info, uint64_t* main_counts,
+                          uint64_t* blocked_counts) {
+    if (!info) return fail(c, OFP_E_ARG, "info is required");
+    int L = 0;
+    ofp_status st = group_check(c, sp, &L);
+    if (st) return st;
+    const int W = sp->n + 1;
+    uint64_t M = 0, B = 0, big = 0;
+    for (uint32_t g = 0; g < sp->groups; ++g) {
+        uint64_t m, b;
+        group_counts(W, L, sp->first_group + g, &m, &b);
+        if (main_counts) main_counts[g] = m;
+        if (blocked_counts) blocked_counts[g] = b;
+        M += m;
+        B += b;
+        big = std::max(big, m + b);
+    }
+    *info = ofp_group_info{};
+    info->main_count = M;
+    info->blocked_count = B;
+    info->largest_group = big;
+    const int Kp = interleave_stride(1, sp->k);
+    info->state_bytes = (M + B) * (uint64_t)Kp * 4 + (uint64_t)sp->groups * kGroupTableBytes;
+    info->low = L;
+    info->cells = W - 1 - L;
+    return OFP_OK;
+}
+
+ofp_status ofp_group_sweep(ofp_ctx* c, const ofp_group_spec* sp, ofp_group_info* info, uint32_t* main_io,
+                           uint32_t* blk_out) {
+    if (!c || !sp || !info) return fail(c, OFP_E_ARG, "ctx, spec and info are required");
+    const uint32_t G = sp->groups;
+    std::vector<uint64_t> mc(G), bc(G);
+    ofp_status st = ofp_group_plan(c, sp, info, mc.data(), bc.data());
+    if (st) return st;
+    for (int j = 0; j < sp->k; ++j)
+        if (sp->moduli[j] < 2) return fail(c, OFP_E_ARG, "modulus must be in [2, 4294967295]");
+    for (uint32_t g = 0; g < G; ++g)
+        if ((mc[g] >> 32) || (bc[g] >> 32))
+            return fail(c, OFP_E_ARG, "a group of the window holds 2^32 or more words: use more low positions");
+    const int W = sp->n + 1, k = sp->k, Kp = interleave_stride(1, k), L = info->low;
+    const uint64_t M = info->main_count, B = info->blocked_count;
+    std::vector<uint64_t> moff(G + 1, 0), boff(G + 1, 0);
+    for (uint32_t g = 0; g < G; ++g) {
+        moff[g + 1] = moff[g] + mc[g];
+        boff[g + 1] = boff[g] + bc[g];
+    }
+    OFP_HIP(c, hipSetDevice(c->device), "hipSetDevice");
+    const size_t mbytes = std::max<uint64_t>(M, 1) * Kp * 4, bbytes = std::max<uint64_t>(B, 1) * Kp * 4;
+    const size_t tbytes = (size_t)G * kGroupTableBytes;
+    char* wbuf = nullptr;
+    st = window_buffer(c, (mbytes + 255) / 256 * 256 + (bbytes + 255) / 256 * 256 + tbytes + 256, &wbuf);
+    if (st) return st;
+    Carve cv{wbuf};
+    uint32_t* dm = cv.take<uint32_t>(mbytes);
+    uint32_t* db = cv.take<uint32_t>(bbytes);
+    char* dtab = cv.take<char>(tbytes);
+    uint32_t* dmods = cv.take<uint32_t>(kMaxKC * 4);
+    std::vector<hipEvent_t> ev(info->cells + 1, nullptr);
+    auto release = [&]() {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+    };
+#define OFP_HIP_R(expr, what)                                              \
+    do {                                                                   \
+        hipError_t e_ = (expr);                                            \
+        if (e_ != hipSuccess) { release(); return hip_fail(c, e_, what); } \
+    } while (0)
+    for (hipEvent_t& e : ev) OFP_HIP_R(hipEventCreate(&e), "group event");
+    {
+        std::vector<char> tabs((size_t)G * kGroupTableBytes);
+        for (uint32_t g = 0; g < G; ++g)
+            if (pack_group_table(L, sp->first_group + g, tabs.data() + (size_t)g * kGroupTableBytes,
+                                 kGroupTableBytes) != kGroupTableBytes) {
+                release();
+                return fail(c, OFP_E_RUNTIME, "group table packing");
+            }
+        OFP_HIP_R(hipMemcpy(dtab, tabs.data(), tabs.size(), hipMemcpyHostToDevice), "upload group tables");
+    }
+    OFP_HIP_R(hipMemcpy(dmods, sp->moduli, k * 4, hipMemcpyHostToDevice), "copy moduli");
+    if (main_io) {
+        std::vector<uint32_t> t(std::max<uint64_t>(M, 1) * Kp, 0);
+        for (uint64_t i = 0; i < M; ++i)
+            for (int j = 0; j < k; ++j) t[i * Kp + j] = main_io[i * k + j];
+        OFP_HIP_R(hipMemcpy(dm, t.data(), t.size() * 4, hipMemcpyHostToDevice), "upload main window");
+    } else {
+        // one hash domain per window start; blocked words need no input (the
+        // first cell reads none)
+        OFP_HIP_R(launch_fill_remote(dm, (uint64_t)sp->first_group << 36, M, db, 0, 0, Kp, k, dmods, sp->seed,
+                                     c->stream),
+                  "fill group window");
+    }
+    const int sf = c->sorted_from >= 0 ? c->sorted_from : (k >= 8 ? 6 : 0);
+    int launches = 0;
+    OFP_HIP_R(hipEventRecord(ev[0], c->stream), "event");
+    for (int p = W - 1, i = 0; p >= L + 1; --p, ++i) {
+        for (uint32_t g = 0; g < G; ++g) {
+            if (!bc[g]) continue;  // no blocked word: the cell changes nothing in the group
+            CellLaunch a{};
+            a.main = dm + moff[g] * Kp;
+            a.blk = db + boff[g] * Kp;
+            a.dp = dtab + (size_t)g * kGroupTableBytes;
+            a.nblk = bc[g];
+            a.first = 0;
+            a.layout = 1;
+            a.variant = (sf > 0 && p >= sf) ? 4 : c->variant;
+            a.xcd_shift = piece_shift(c->xcd_shift, bc[g], c->threads);
+            a.W = W;
+            a.p = p;
+            a.read_blk = (p == W - 1) ? 0 : 1;
+            a.K = Kp;
+            a.off = 0;
+            a.kc = k;
+            for (int j = 0; j < k; ++j) a.mod[j] = sp->moduli[j];
+            OFP_HIP_R(launch_cell(a, false, c->threads, grid_blocks(c, bc[g]), c->stream), "group cell launch");
+            ++launches;
+        }
+        OFP_HIP_R(hipEventRecord(ev[i + 1], c->stream), "event");
+    }
+    OFP_HIP_R(hipEventSynchronize(ev[info->cells]), "group sync");
+    double total = 0;
+    for (int i = 0; i < info->cells; ++i) {
+        float ms = 0;
+        OFP_HIP_R(hipEventElapsedTime(&ms, ev[i], ev[i + 1]), "event time");
+        if (i < OFP_MAX_N) info->cell_ms[i] = ms;
+        total += ms;
+    }
+    info->kernel_s = total * 1e-3;
+    info->launches = launches;
+    if (main_io || blk_out) {
+        std::vector<uint32_t> t(std::max<uint64_t>(std::max(M, B), 1) * Kp);
+        if (main_io && M) {
+            OFP_HIP_R(hipMemcpy(t.data(), dm, M * Kp * 4, hipMemcpyDeviceToHost), "download main window");
+            for (uint64_t i = 0; i < M; ++i)
+                for (int j = 0; j < k; ++j) main_io[i * k + j] = t[i * Kp + j];
+        }
+        if (blk_out && B) {
+            OFP_HIP_R(hipMemcpy(t.data(), db, B * Kp * 4, hipMemcpyDeviceToHost), "download blocked window");
             for (uint64_t i = 0; i < B; ++i)
                 for (int j = 0; j < k; ++j) blk_out[i * k + j] = t[i * Kp + j];
         }
@@ -1305,6 +1515,7 @@ void ofp_destroy(ofp_ctx* c) {
     if (c->dpc32) (void)hipFree(c->dpc32);
     if (c->dpc64) (void)hipFree(c->dpc64);
     if (c->ct32) (void)hipFree(c->ct32);
+    if (c->win) (void)hipFree(c->win);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);

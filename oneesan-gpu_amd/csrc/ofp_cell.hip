@@ -353,6 +353,24 @@ size_t pack_height_table(bool wide, void* out, size_t cap) {
     return wide ? pack_into<uint64_t>(d, out, cap) : pack_into<uint32_t>(d, out, cap);
 }
 
+size_t pack_group_table(int low, uint32_t mask, void* out, size_t cap) {
+    const size_t bytes = sizeof(Quad<uint32_t>) * kDpRows * kDpCols;
+    if (cap < bytes) return 0;
+    uint64_t f[kDpRows][kDpCols];
+    build_group_table(low, mask, f);
+    std::vector<Quad<uint32_t>> q(kDpRows * kDpCols);
+    pack_group_quads<uint32_t>(low, mask, f, (Quad<uint32_t>(*)[kDpCols])q.data());
+    std::memcpy(out, q.data(), bytes);
+    return bytes;
+}
+
+void group_counts(int W, int low, uint32_t mask, uint64_t* main_words, uint64_t* blocked_words) {
+    uint64_t f[kDpRows][kDpCols];
+    build_group_table(low, mask, f);
+    *main_words = f[W][1];
+    *blocked_words = f[W - 1][1];
+}
+
 void build_capped_table(int cap, uint64_t (*d)[kDpCols]) {
     for (int w = 0; w < kDpRows; ++w)
         for (int h = 0; h < kDpCols; ++h) d[w][h] = 0;

@@ -637,5 +637,77 @@ inline void pack_quads(const uint64_t (*d)[kDpCols], Quad<Idx> (*q)[kDpCols]) {
         }
 }
 
+// ---- occupancy groups ----
+// The words whose low positions 0..low-1 have a fixed occupancy (bit i of mask
+// set: position i holds R or L; clear: N) form a group that every cell p > low
+// maps onto itself.  Such a cell rewrites positions p and p-1 (both >= low) and
+// may turn a closure partner's R into L or back, which never changes whether a
+// position is occupied (reference: the HIGH window of the bucket layout,
+// docs/b300-bucket-snake.md:7-20).  Blocked words (width W-1) of these cells
+// keep positions 0..p-2 where the main words have them, so the same mask
+// selects their group.
+//
+// A group's words ranked in the global order (lexicographic, N < R < L from
+// the top) have ranks that are sums of the same kind of table entries as the
+// global ranks, taken from the group's completion table:
+//   F[0][h] = [h == 0]
+//   F[q][h] = F[q-1][h-1] + F[q-1][h+1]           position q-1 < low, occupied
+//           = F[q-1][h]                           position q-1 < low, empty
+//           = F[q-1][h] + F[q-1][h-1] + F[q-1][h+1]   position q-1 >= low
+// with the entries of a low position counting only its allowed symbols:
+//   occupied: zn = 0 (no N to pass), zc = F[pos][h-1] (the R completions)
+//   empty:    zn = zc = F[pos][h] (every rank stays below zn: N is decoded)
+// Decode walks, orbit ranks and closure scans of a cell p > low only read
+// entries of positions >= low at symbols that change and entries of low
+// positions at symbols of the group, so the cell body runs unchanged on
+// group-local ranks.  Counts: main words F[W][1], blocked words F[W-1][1].
+inline void build_group_table(int low, uint32_t mask, uint64_t (*f)[kDpCols]) {
+    for (int w = 0; w < kDpRows; ++w)
+        for (int h = 0; h < kDpCols; ++h) f[w][h] = 0;
+    f[0][0] = 1;
+    for (int q = 1; q < kDpRows; ++q) {
+        const int pos = q - 1;
+        const bool lowpos = pos < low, occ = lowpos && ((mask >> pos) & 1u);
+        for (int h = 0; h < kDpCols; ++h) {
+            const uint64_t dn = h > 0 ? f[q - 1][h - 1] : 0, up = h + 1 < kDpCols ? f[q - 1][h + 1] : 0;
+            f[q][h] = occ ? dn + up : (lowpos ? f[q - 1][h] : f[q - 1][h] + dn + up);
+        }
+    }
+}
+
+// Rank contributions (zn, zc) of position pos at height h in a group table.
+inline void group_entry(int low, uint32_t mask, const uint64_t (*f)[kDpCols], int pos, int h, uint64_t& zn,
+                        uint64_t& zc) {
+    if (pos >= kDpRows) {
+        zn = zc = 0;
+        return;
+    }
+    const uint64_t here = f[pos][h], below = h > 0 ? f[pos][h - 1] : 0;
+    if (pos < low) {
+        if ((mask >> pos) & 1u) {
+            zn = 0;
+            zc = below;
+        } else {
+            zn = zc = here;
+        }
+    } else {
+        zn = here;
+        zc = here + below;
+    }
+}
+
+// Packed table of a group (the Quad layout of pack_quads: mn, mc are the
+// entries of the position one up).
+template <typename Idx>
+inline void pack_group_quads(int low, uint32_t mask, const uint64_t (*f)[kDpCols], Quad<Idx> (*q)[kDpCols]) {
+    for (int pos = 0; pos < kDpRows; ++pos)
+        for (int h = 0; h < kDpCols; ++h) {
+            uint64_t zn, zc, mn, mc;
+            group_entry(low, mask, f, pos, h, zn, zc);
+            group_entry(low, mask, f, pos + 1, h, mn, mc);
+            q[pos][h] = Quad<Idx>{(Idx)zn, (Idx)zc, (Idx)mn, (Idx)mc};
+        }
+}
+
 }  // namespace cell
 }  // namespace ofp

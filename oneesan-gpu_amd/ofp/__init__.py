@@ -6,5 +6,6 @@ C ABI in include/ofp.h); this package binds it and carries the host logic
 around it.
 """
 from ._lib import (  # noqa: F401
-    EXPORTS, OFP_MAX_N, Context, Job, OfpError, Plan, height_cap, load, moduli_groups, state_counts, version,
+    EXPORTS, OFP_MAX_N, Context, Job, OfpError, Plan, group_plan, height_cap, load, moduli_groups, state_counts,
+    version,
 )

@@ -28,7 +28,7 @@ EXPORTS = (
     "ofp_row_sweep", "ofp_read_main", "ofp_set_launch", "ofp_set_variant", "ofp_set_low_cells", "ofp_set_layout", "ofp_state_counts",
     "ofp_last_error", "ofp_version", "ofp_destroy",
     "ofp_job_create", "ofp_job_enqueue", "ofp_job_info_get", "ofp_job_result", "ofp_job_destroy",
-    "ofp_shard_plan", "ofp_shard_sweep", "ofp_set_height_caps", "ofp_height_cap", "ofp_set_xcd_pieces",
+    "ofp_shard_plan", "ofp_shard_sweep", "ofp_group_plan", "ofp_group_sweep", "ofp_set_height_caps", "ofp_height_cap", "ofp_set_xcd_pieces",
     "ofp_set_sorted_cells", "ofp_time_cells", "ofp_set_index_width", "ofp_set_max_batch",
     "ofp_moduli_groups", "ofp_solve_static", "ofp_set_pair_passes",
 )
@@ -123,6 +123,37 @@ class ShardInfo(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class GroupSpec(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int),
+        ("k", ctypes.c_int),
+        ("moduli", ctypes.POINTER(ctypes.c_uint32)),
+        ("low", ctypes.c_int),
+        ("first_group", ctypes.c_uint32),
+        ("groups", ctypes.c_uint32),
+        ("seed", ctypes.c_uint64),
+    ]
+
+
+class GroupInfo(ctypes.Structure):
+    _fields_ = [
+        ("main_count", ctypes.c_uint64),
+        ("blocked_count", ctypes.c_uint64),
+        ("largest_group", ctypes.c_uint64),
+        ("state_bytes", ctypes.c_uint64),
+        ("low", ctypes.c_int),
+        ("cells", ctypes.c_int),
+        ("launches", ctypes.c_int),
+        ("kernel_s", ctypes.c_double),
+        ("cell_ms", ctypes.c_float * OFP_MAX_N),
+    ]
+
+    def as_dict(self) -> dict:
+        d = {name: getattr(self, name) for name, _ in self._fields_ if name != "cell_ms"}
+        d["cell_ms"] = [float(self.cell_ms[i]) for i in range(min(self.cells, OFP_MAX_N))]
+        return d
+
+
 _lib = None
 
 
@@ -175,6 +206,8 @@ def load() -> ctypes.CDLL:
         "ofp_job_destroy": (None, [vp]),
         "ofp_shard_plan": (ctypes.c_int32, [vp, ctypes.POINTER(ShardSpec), ctypes.POINTER(ShardInfo)]),
         "ofp_shard_sweep": (ctypes.c_int32, [vp, ctypes.POINTER(ShardSpec), ctypes.POINTER(ShardInfo), u32p, u32p]),
+        "ofp_group_plan": (ctypes.c_int32, [vp, ctypes.POINTER(GroupSpec), ctypes.POINTER(GroupInfo), u64p, u64p]),
+        "ofp_group_sweep": (ctypes.c_int32, [vp, ctypes.POINTER(GroupSpec), ctypes.POINTER(GroupInfo), u32p, u32p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -389,6 +422,32 @@ class Context:
         self._check(st, "ofp_shard_sweep")
         return info.as_dict(), mo, bo
 
+    def group_sweep(self, n: int, moduli: Sequence[int], first_group: int, groups: int, low: int = 0,
+                    seed: int = 0, main_in=None, blocked_out: bool = False):
+        """High cells W-1..low+1 of a row over the occupancy groups first_group..
+        first_group+groups-1.  main_in: None for seeded inputs, else a
+        (main_count, k) uint32 array in window order.  Returns (info, main,
+        blocked): main is the result when main_in was given, blocked the
+        window's blocked words when blocked_out is set (else None)."""
+        import numpy as np
+        k = len(moduli)
+        mods = _u32_array(moduli)
+        spec = GroupSpec(n, k, ctypes.cast(mods, ctypes.POINTER(ctypes.c_uint32)), low, first_group, groups, seed)
+        info = GroupInfo()
+        self._check(self._lib.ofp_group_plan(self._h, ctypes.byref(spec), ctypes.byref(info), None, None),
+                    "ofp_group_plan")
+        u32p = ctypes.POINTER(ctypes.c_uint32)
+        mo = bo = None
+        if main_in is not None:
+            mo = np.ascontiguousarray(np.asarray(main_in, dtype=np.uint32).reshape(info.main_count, k)).copy()
+        if blocked_out:
+            bo = np.zeros((info.blocked_count, k), np.uint32)
+        st = self._lib.ofp_group_sweep(self._h, ctypes.byref(spec), ctypes.byref(info),
+                                       mo.ctypes.data_as(u32p) if mo is not None else None,
+                                       bo.ctypes.data_as(u32p) if bo is not None else None)
+        self._check(st, "ofp_group_sweep")
+        return info.as_dict(), mo, bo
+
     def set_variant(self, variant: int) -> None:
         """Cell-kernel variant (state-major layout): 0 = early orbit loads, 1 = late,
         2 = one branch-free orbit path, 3 = as 2 with early stores, 4 = as 2 with the
@@ -468,6 +527,24 @@ def shard_plan(n: int, k: int, slices: int, slice_: int, low_cells: int = 0) -> 
     if st != OFP_OK:
         raise OfpError(st, f"shard_plan(n={n}, k={k}, slices={slices}, slice={slice_})")
     return info.as_dict()
+
+
+def group_plan(n: int, k: int, first_group: int, groups: int, low: int = 0):
+    """Window totals and per-group (main, blocked) counts of occupancy groups
+    (host computation, no GPU needed).  Returns (info, main_counts, blocked_counts)."""
+    import numpy as np
+    lib = load()
+    mods = _u32_array([4294967291] * k)
+    spec = GroupSpec(n, k, ctypes.cast(mods, ctypes.POINTER(ctypes.c_uint32)), low, first_group, groups, 0)
+    info = GroupInfo()
+    mc = np.zeros(max(groups, 1), np.uint64)
+    bc = np.zeros(max(groups, 1), np.uint64)
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+    st = lib.ofp_group_plan(None, ctypes.byref(spec), ctypes.byref(info), mc.ctypes.data_as(u64p),
+                            bc.ctypes.data_as(u64p))
+    if st != OFP_OK:
+        raise OfpError(st, f"group_plan(n={n}, k={k}, first_group={first_group}, groups={groups}, low={low})")
+    return info.as_dict(), mc[:groups], bc[:groups]
 
 
 def version() -> str:

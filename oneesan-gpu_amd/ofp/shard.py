@@ -1,20 +1,30 @@
 """One GPU's share of an n=27 row at real shape (measurement).
 
-    python -m ofp.shard [--n 27] [--k 4] [--slices 64] [--slice 21]
+    python -m ofp.shard [--n 27] [--k 4] [--slices 64] [--slice 21]   one low-cells window
+    python -m ofp.shard --share 0 [--shares 8]                        a whole row of one share
 
 The n=27 frontier (520.7e9 states, 2 TB per modulus) does not fit one GPU; an
 8-GPU run gives each GPU 1/8 of it (242 GiB per modulus, ~1 TB at 4 moduli),
-which does not fit either.  This measures the owner-local part of a row at the
-real shape: the low cells p = 14..1 (W = 28) over one resident window of
-1/`slices` of the state (whole prefix classes, ~130 GB at slices=64, k=4),
-with the real n=27 index math (64-bit ranks), 4 moduli interleaved, window
-residues seeded uniform in [0, p), and main words of other shards (read only by
-RR closures that flip a position above the low cells) read as seeded synthetic
-residues.  The 1/8 share is 8 such windows; its time is extrapolated as 8x.
+which does not fit either, so a share is streamed through HBM in windows.  A
+row has two halves with two placements of the state:
 
-Left out (stated in the report): the high cells p = 27..15, which need the
-transposed (LOW-occupancy-major) placement, and the per-row all-to-all that
-moves the state between the two placements.
+  * low cells p = 14..1 (W = 28) over whole prefix classes (words with the same
+    content above the low cells): a window is 1/`slices` of the state, with
+    64-bit global ranks; main words of other shards (read only by RR closures
+    that flip a position above the low cells) read as seeded residues.
+  * high cells p = 27..15 over LOW14 occupancy groups (words with the same
+    occupancy of positions 0..13; the high cells keep it, so a group is closed):
+    a window is a run of consecutive groups, each stored alone with 32-bit
+    ranks local to the group.
+
+`measure` times one low-cells window; `measure_share` streams every window of
+one share through both halves (share `share` of `shares`: 1/8 of the groups by
+word count, and slices share*64/8 .. of the prefix classes), seeding each
+window in HBM, and reports the kernel time of the share's whole row.
+
+Left out (stated in the report): the per-row all-to-all that moves the state
+between the two placements, and the host or peer traffic that would stage the
+non-resident windows (each window here is seeded in place).
 
 Roofline bytes per launch: 8 B x k x (main + blocked words of the window).
 """
@@ -22,6 +32,7 @@ from __future__ import annotations
 
 import json
 import sys
+import time
 
 from . import _lib
 from .path_bound import PRIMES
@@ -58,6 +69,92 @@ def measure(ctx: _lib.Context, n: int = 27, k: int = 4, slices: int = 64, slice_
     }
 
 
+def share_windows(n: int, k: int, shares: int, share: int, window_bytes: float, low: int = 0):
+    """The high-half windows of one share: (first_group, groups) runs of
+    consecutive occupancy groups whose device bytes stay <= window_bytes (a
+    single larger group gets a window of its own).  The share holds the groups
+    from the first whose words start at or after share/shares of the state."""
+    import numpy as np
+    W = n + 1
+    low = low or W // 2
+    info, mc, bc = _lib.group_plan(n, k, 0, 1 << low, low)
+    words = mc.astype(np.float64) + bc.astype(np.float64)
+    cum = np.concatenate([[0.0], np.cumsum(words)])
+    total = cum[-1]
+    starts = [int(np.searchsorted(cum, total * s / shares, side="left")) for s in range(shares)] + [1 << low]
+    g0, g1 = starts[share], starts[share + 1]
+    per_word = 4 * (k + (k & 1) if k > 1 else 1)
+    table = 15360  # one packed 32-bit group table
+    windows, start, acc = [], g0, 0.0
+    for g in range(g0, g1):
+        b = words[g] * per_word + table
+        if g > start and acc + b > window_bytes:
+            windows.append((start, g - start))
+            start, acc = g, 0.0
+        acc += b
+    if g1 > start:
+        windows.append((start, g1 - start))
+    return low, windows
+
+
+def measure_share(ctx: _lib.Context, n: int = 27, k: int = 4, shares: int = 8, share: int = 0,
+                  slices: int = 64, window_bytes: float = 136e9, seed: int = 20261202, log=None) -> dict:
+    """Stream every window of one share through both halves of a row."""
+    if slices % shares:
+        raise ValueError("slices must be a multiple of shares")
+    mods = PRIMES[:k]
+    W = n + 1
+    low, windows = share_windows(n, k, shares, share, window_bytes)
+    t0 = time.perf_counter()
+    hi = {"windows": len(windows), "groups": 0, "words": 0, "kernel_s": 0.0, "launches": 0, "max_window_bytes": 0,
+          "cell_ms": [0.0] * (W - 1 - low)}
+    for g0, g in windows:
+        info, _, _ = ctx.group_sweep(n, mods, g0, g, low, seed=seed)
+        hi["groups"] += g
+        hi["words"] += info["main_count"] + info["blocked_count"]
+        hi["kernel_s"] += info["kernel_s"]
+        hi["launches"] += info["launches"]
+        hi["max_window_bytes"] = max(hi["max_window_bytes"], info["state_bytes"])
+        hi["cell_ms"] = [a + b for a, b in zip(hi["cell_ms"], info["cell_ms"])]
+        if log:
+            log(f"share {share}: high window groups {g0}..{g0 + g - 1}: {info['state_bytes'] / 1e9:.1f} GB, "
+                f"{info['kernel_s']:.3f} s")
+    hi["cells"] = W - 1 - low
+    per = slices // shares
+    lo = {"windows": per, "words": 0, "kernel_s": 0.0, "launches": 0, "max_window_bytes": 0, "stray_stores": 0}
+    for s in range(share * per, (share + 1) * per):
+        info, _, _ = ctx.shard_sweep(n, mods, slices, s, seed=seed)
+        lo["words"] += info["main_count"] + info["blocked_count"]
+        lo["kernel_s"] += info["kernel_s"]
+        lo["launches"] += info["launches"]
+        lo["stray_stores"] += info["stray_stores"]
+        lo["max_window_bytes"] = max(lo["max_window_bytes"], info["state_bytes"])
+        lo["cells"] = info["low_cells"]
+        if log:
+            log(f"share {share}: low window slice {s}/{slices}: {info['state_bytes'] / 1e9:.1f} GB, "
+                f"{info['kernel_s']:.3f} s")
+    wall = time.perf_counter() - t0
+    for half in (hi, lo):
+        declared = 8.0 * k * half["words"] * half["cells"]
+        half["declared_bytes"] = declared
+        half["transitions_per_s"] = half["cells"] * half["words"] * k / half["kernel_s"]
+        half["roofline"] = {"bound": "hbm", "achieved": declared / half["kernel_s"] / 1e9, "peak": HBM_PEAK_GBS,
+                            "unit": "GB/s", "frac": declared / half["kernel_s"] / 1e9 / HBM_PEAK_GBS}
+    nm, nb = _lib.state_counts(n)
+    return {
+        "workload": f"n={n} row of one GPU's share ({share} of {shares}), {k} moduli interleaved, seeded inputs: "
+                    f"high cells {W - 1}..{low + 1} over LOW{low} occupancy groups in {len(windows)} windows, "
+                    f"low cells {lo['cells']}..1 over whole prefix classes in {per} windows",
+        "n": n, "k": k, "shares": shares, "share": share,
+        "share_fraction_high": hi["words"] / (nm + nb), "share_fraction_low": lo["words"] / (nm + nb),
+        "row_kernel_s": hi["kernel_s"] + lo["kernel_s"],
+        "row_wall_s": wall,
+        "high": hi, "low": lo,
+        "left_out": "the per-row all-to-all between the two placements, and the host/peer staging of the "
+                    "non-resident windows (each window is seeded in HBM); wall time includes the seeding",
+    }
+
+
 def main(argv=None) -> int:
     import argparse
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
@@ -65,11 +162,19 @@ def main(argv=None) -> int:
     ap.add_argument("--k", type=int, default=4)
     ap.add_argument("--slices", type=int, default=64)
     ap.add_argument("--slice", type=int, default=None)
+    ap.add_argument("--share", type=int, default=None, help="stream a whole row of this share")
+    ap.add_argument("--shares", type=int, default=8)
     a = ap.parse_args(argv)
     with _lib.Context() as ctx:
-        rep = measure(ctx, a.n, a.k, a.slices, a.slice)
+        if a.share is not None:
+            rep = measure_share(ctx, a.n, a.k, a.shares, a.share, a.slices,
+                                log=lambda m: print(m, file=sys.stderr, flush=True))
+            stray = rep["low"]["stray_stores"]
+        else:
+            rep = measure(ctx, a.n, a.k, a.slices, a.slice)
+            stray = rep["stray_stores"]
     print(json.dumps(rep), flush=True)
-    return 0 if rep["stray_stores"] == 0 else 1
+    return 0 if stray == 0 else 1
 
 
 if __name__ == "__main__":

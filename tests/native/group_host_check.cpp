@@ -1,0 +1,303 @@
+// Host check of the occupancy-group placement (oneesan-gpu_amd/csrc/
+// ofp_cell_impl.h: build_group_table, pack_group_quads): the cell body, run on
+// the CPU with each group's packed table over the group's words alone (ranks
+// local to the group, arrays holding only the group), for the high cells
+// p = W-1..low+1 of a row, against the CPU oracle's cells on the whole state
+// (oracle/gridfp_oracle.c or_apply_cells) on seeded random vectors.
+//
+// Also checked: the group counts F[W][1], F[W-1][1] against enumeration, and
+// that no access of the body leaves the group (every load and store index is
+// range-checked).
+//
+// `group_host_check big n low mask...` checks single groups at full size (n up
+// to 27, where the whole state is out of reach): the group's words are
+// enumerated depth-first in lexicographic order (their global ranks, from the
+// oracle, must come out increasing, so the enumeration index is the rank in
+// the group), and the oracle's forward transition (or_include /
+// or_blocked_exclude, the push form of one cell) is applied to the group's
+// words alone, every target looked up in the group; the body's result must
+// equal it.  Built and run by tests/test_group.py; prints one line per case
+// and "ALL OK" at the end, exits non-zero on the first mismatch.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <random>
+#include <unordered_map>
+#include <vector>
+
+#include "../../oneesan-gpu_amd/csrc/ofp_cell_impl.h"
+
+extern "C" {
+int or_include(uint64_t m, int width, int p, uint64_t* out, int* blocked);
+uint64_t or_blocked_exclude(uint64_t b, int p);
+uint64_t or_rank(uint64_t m, int width);
+int or_apply_cells(int n, uint32_t mod, uint32_t* main, uint32_t* blk, int p_hi, int p_lo, int blk_valid,
+                   int use_pull);
+uint64_t or_main_count(int n);
+uint64_t or_blocked_count(int n);
+uint64_t or_unrank(uint64_t r, int width);
+}
+
+using namespace ofp;
+
+static bool g_oob = false;
+
+// State-major arrays of one group with stride K; every index is range-checked.
+template <int KC>
+struct Checked {
+    uint32_t* main;
+    uint32_t* blk;
+    uint64_t nm, nb;
+    int K, off;
+    void lm(uint64_t i, cell::Vals<KC>& v) const {
+        if (i >= nm) { g_oob = true; return; }
+        cell::load_vals<KC, cell::kAoS>(main, i, K, off, v);
+    }
+    void sm(uint64_t i, const cell::Vals<KC>& v) const {
+        if (i >= nm) { g_oob = true; return; }
+        cell::store_vals<KC, cell::kAoS>(main, i, K, off, v);
+    }
+    void lb(uint64_t b, cell::Vals<KC>& v) const {
+        if (b >= nb) { g_oob = true; return; }
+        cell::load_vals<KC, cell::kAoS>(blk, b, K, off, v);
+    }
+    void sb(uint64_t b, const cell::Vals<KC>& v) const {
+        if (b >= nb) { g_oob = true; return; }
+        cell::store_vals<KC, cell::kAoS>(blk, b, K, off, v);
+    }
+};
+
+static uint32_t occupancy(uint64_t word, int low) {
+    uint32_t m = 0;
+    for (int pos = 0; pos < low; ++pos)
+        if ((word >> (2 * pos)) & 3u) m |= 1u << pos;
+    return m;
+}
+
+template <int V>
+static int check(int n, int low, const uint32_t* mods, unsigned seed) {
+    constexpr int KC = 3, K = 4;
+    const int W = n + 1;
+    const uint64_t nm = or_main_count(n), nb = or_blocked_count(n);
+    // group of every word and its rank inside the group (global order)
+    std::vector<uint32_t> gm(nm), gb(nb);
+    std::vector<uint64_t> lm(nm), lb(nb);
+    std::vector<uint64_t> cm(1u << low, 0), cb(1u << low, 0);
+    for (uint64_t i = 0; i < nm; ++i) {
+        gm[i] = occupancy(or_unrank(i, W), low);
+        lm[i] = cm[gm[i]]++;
+    }
+    for (uint64_t i = 0; i < nb; ++i) {
+        gb[i] = occupancy(or_unrank(i, W - 1), low);
+        lb[i] = cb[gb[i]]++;
+    }
+    std::mt19937_64 rng(seed);
+    std::vector<uint32_t> mi(nm * KC);
+    for (uint64_t i = 0; i < nm; ++i)
+        for (int j = 0; j < KC; ++j) mi[i * KC + j] = (uint32_t)(rng() % mods[j]);
+    std::vector<uint32_t> mo(nm * KC, 0), bo((nb ? nb : 1) * KC, 0);
+    // per group: gather, run the cells with the group's table, scatter back
+    static uint64_t f[kDpRows][kDpCols];
+    static cell::Quad<uint32_t> q[kDpRows][kDpCols];
+    int groups = 0;
+    for (uint32_t g = 0; g < (1u << low); ++g) {
+        cell::build_group_table(low, g, f);
+        if (f[W][1] != cm[g] || f[W - 1][1] != cb[g]) {
+            printf("COUNT n=%d low=%d group=%u main %llu/%llu blocked %llu/%llu\n", n, low, g,
+                   (unsigned long long)f[W][1], (unsigned long long)cm[g], (unsigned long long)f[W - 1][1],
+                   (unsigned long long)cb[g]);
+            return 1;
+        }
+        if (!cm[g]) continue;
+        ++groups;
+        cell::pack_group_quads<uint32_t>(low, g, f, q);
+        cell::Tab<uint32_t> T{q};
+        std::vector<uint32_t> am(cm[g] * K, 0), ab((cb[g] ? cb[g] : 1) * K, 0);
+        for (uint64_t i = 0; i < nm; ++i)
+            if (gm[i] == g)
+                for (int j = 0; j < KC; ++j) am[lm[i] * K + j] = mi[i * KC + j];
+        Checked<KC> mem{am.data(), ab.data(), cm[g], cb[g], K, 0};
+        CellLaunch a{};
+        a.W = W;
+        a.K = K;
+        a.off = 0;
+        a.kc = KC;
+        a.layout = 1;
+        for (int j = 0; j < KC; ++j) a.mod[j] = mods[j];
+        for (int p = W - 1; p >= low + 1; --p) {
+            a.p = p;
+            a.read_blk = p == W - 1 ? 0 : 1;
+            for (uint64_t b = 0; b < cb[g]; ++b) cell::cell_one<KC, uint32_t, cell::kAoS, V>(a, T, b, mem);
+            if (g_oob) {
+                printf("OUT OF GROUP n=%d low=%d group=%u p=%d\n", n, low, g, p);
+                return 1;
+            }
+        }
+        for (uint64_t i = 0; i < nm; ++i)
+            if (gm[i] == g)
+                for (int j = 0; j < KC; ++j) mo[i * KC + j] = am[lm[i] * K + j];
+        for (uint64_t i = 0; i < nb; ++i)
+            if (gb[i] == g)
+                for (int j = 0; j < KC; ++j) bo[i * KC + j] = ab[lb[i] * K + j];
+    }
+    for (int j = 0; j < KC; ++j) {
+        std::vector<uint32_t> em(nm), eb(nb ? nb : 1, 0);
+        for (uint64_t i = 0; i < nm; ++i) em[i] = mi[i * KC + j];
+        if (or_apply_cells(n, mods[j], em.data(), eb.data(), W - 1, low + 1, 0, 1) != 0) return 2;
+        for (uint64_t i = 0; i < nm; ++i)
+            if (mo[i * KC + j] != em[i]) {
+                printf("MISMATCH main n=%d low=%d V=%d j=%d i=%llu got=%u want=%u\n", n, low, V, j,
+                       (unsigned long long)i, mo[i * KC + j], em[i]);
+                return 1;
+            }
+        for (uint64_t i = 0; i < nb; ++i)
+            if (bo[i * KC + j] != eb[i]) {
+                printf("MISMATCH blocked n=%d low=%d V=%d j=%d i=%llu got=%u want=%u\n", n, low, V, j,
+                       (unsigned long long)i, bo[i * KC + j], eb[i]);
+                return 1;
+            }
+    }
+    printf("ok n=%d low=%d V=%d groups=%d cells=%d..%d\n", n, low, V, groups, W - 1, low + 1);
+    return 0;
+}
+
+// Words of `width` positions in the group, in lexicographic order (N < R < L
+// from the top, entered at height 1, ending at 0).
+static void group_words(int width, int low, uint32_t mask, const uint64_t (*f)[kDpCols], std::vector<uint64_t>& out) {
+    std::function<void(int, int, uint64_t)> rec = [&](int pos, int h, uint64_t word) {
+        if (pos < 0) {
+            if (h == 0) out.push_back(word);
+            return;
+        }
+        for (int v = 0; v < 3; ++v) {
+            if (pos < low && (v != 0) != (((mask >> pos) & 1u) != 0)) continue;
+            const int nh = h + (v == 2) - (v == 1);
+            if (nh < 0 || nh >= kDpCols || f[pos][nh] == 0) continue;
+            rec(pos - 1, nh, word | ((uint64_t)v << (2 * pos)));
+        }
+    };
+    rec(width - 1, 1, 0);
+}
+
+static int check_big(int n, int low, uint32_t g, const uint32_t* mods, unsigned seed) {
+    constexpr int KC = 3, K = 4;
+    const int W = n + 1;
+    static uint64_t f[kDpRows][kDpCols];
+    static cell::Quad<uint32_t> q[kDpRows][kDpCols];
+    cell::build_group_table(low, g, f);
+    std::vector<uint64_t> wm, wb;
+    group_words(W, low, g, f, wm);
+    group_words(W - 1, low, g, f, wb);
+    if (wm.size() != f[W][1] || wb.size() != f[W - 1][1]) {
+        printf("COUNT big n=%d low=%d group=%u\n", n, low, g);
+        return 1;
+    }
+    std::unordered_map<uint64_t, uint64_t> im, ib;
+    for (uint64_t i = 0; i < wm.size(); ++i) {
+        if (i && or_rank(wm[i], W) <= or_rank(wm[i - 1], W)) {
+            printf("ORDER big main n=%d group=%u i=%llu\n", n, g, (unsigned long long)i);
+            return 1;
+        }
+        im[wm[i]] = i;
+    }
+    for (uint64_t i = 0; i < wb.size(); ++i) {
+        if (i && or_rank(wb[i], W - 1) <= or_rank(wb[i - 1], W - 1)) {
+            printf("ORDER big blocked n=%d group=%u i=%llu\n", n, g, (unsigned long long)i);
+            return 1;
+        }
+        ib[wb[i]] = i;
+    }
+    const uint64_t nm = wm.size(), nb = wb.size();
+    std::mt19937_64 rng(seed);
+    std::vector<uint32_t> mi(nm * K, 0), bi((nb ? nb : 1) * K, 0);
+    for (uint64_t i = 0; i < nm; ++i)
+        for (int j = 0; j < KC; ++j) mi[i * K + j] = (uint32_t)(rng() % mods[j]);
+    std::vector<uint32_t> em = mi, eb = bi;  // the oracle's run
+    cell::pack_group_quads<uint32_t>(low, g, f, q);
+    cell::Tab<uint32_t> T{q};
+    Checked<KC> mem{mi.data(), bi.data(), nm, nb, K, 0};
+    CellLaunch a{};
+    a.W = W;
+    a.K = K;
+    a.kc = KC;
+    a.layout = 1;
+    for (int j = 0; j < KC; ++j) a.mod[j] = mods[j];
+    for (int p = W - 1; p >= low + 1; --p) {
+        a.p = p;
+        a.read_blk = p == W - 1 ? 0 : 1;
+        for (uint64_t b = 0; b < nb; ++b) cell::cell_one<KC, uint32_t, cell::kAoS, 2>(a, T, b, mem);
+        if (g_oob) {
+            printf("OUT OF GROUP big n=%d low=%d group=%u p=%d\n", n, low, g, p);
+            return 1;
+        }
+        // oracle: push form of the cell over the group's words
+        std::vector<uint32_t> mo = em, bo(eb.size(), 0);
+        for (uint64_t i = 0; i < nm; ++i) {
+            uint64_t t;
+            int blocked;
+            if (!or_include(wm[i], W, p, &t, &blocked)) continue;
+            auto& map = blocked ? ib : im;
+            auto it = map.find(t);
+            if (it == map.end()) {
+                printf("TRANSITION LEAVES GROUP n=%d group=%u p=%d\n", n, g, p);
+                return 1;
+            }
+            uint32_t* dst = blocked ? &bo[it->second * K] : &mo[it->second * K];
+            for (int j = 0; j < KC; ++j) dst[j] = cell::addm(dst[j], em[i * K + j], mods[j]);
+        }
+        if (p != W - 1)
+            for (uint64_t i = 0; i < nb; ++i) {
+                auto it = im.find(or_blocked_exclude(wb[i], p));
+                if (it == im.end()) {
+                    printf("EXCLUDE LEAVES GROUP n=%d group=%u p=%d\n", n, g, p);
+                    return 1;
+                }
+                for (int j = 0; j < KC; ++j)
+                    mo[it->second * K + j] = cell::addm(mo[it->second * K + j], eb[i * K + j], mods[j]);
+            }
+        em.swap(mo);
+        eb.swap(bo);
+        for (uint64_t i = 0; i < nm; ++i)
+            for (int j = 0; j < KC; ++j)
+                if (mi[i * K + j] != em[i * K + j]) {
+                    printf("MISMATCH big main n=%d group=%u p=%d i=%llu j=%d\n", n, g, p, (unsigned long long)i, j);
+                    return 1;
+                }
+        if (p >= 2)
+            for (uint64_t i = 0; i < nb; ++i)
+                for (int j = 0; j < KC; ++j)
+                    if (bi[i * K + j] != eb[i * K + j]) {
+                        printf("MISMATCH big blocked n=%d group=%u p=%d i=%llu j=%d\n", n, g, p,
+                               (unsigned long long)i, j);
+                        return 1;
+                    }
+    }
+    printf("ok big n=%d low=%d group=%u main=%llu blocked=%llu cells=%d..%d\n", n, low, g,
+           (unsigned long long)nm, (unsigned long long)nb, W - 1, low + 1);
+    return 0;
+}
+
+int main(int argc, char** argv) {
+    const uint32_t bmods[3] = {4294967291u, 2u, 4294966997u};
+    if (argc > 3 && !strcmp(argv[1], "big")) {
+        const int n = atoi(argv[2]), low = atoi(argv[3]);
+        for (int i = 4; i < argc; ++i)
+            if (check_big(n, low, (uint32_t)strtoul(argv[i], nullptr, 10), bmods, 77u + i)) return 1;
+        printf("ALL OK\n");
+        return 0;
+    }
+    const int nmax = argc > 1 ? atoi(argv[1]) : 10;
+    const uint32_t mods[3] = {4294967291u, 2u, 4294966997u};
+    for (int n = 2; n <= nmax; ++n) {
+        const int W = n + 1;
+        for (int low = 1; low <= W - 2; ++low) {
+            if (check<0>(n, low, mods, 100u * n + low)) return 1;
+            if (check<2>(n, low, mods, 200u * n + low)) return 1;
+            if (n % 2 == 0 && (check<1>(n, low, mods, 300u * n + low) || check<3>(n, low, mods, 400u * n + low)))
+                return 1;
+        }
+    }
+    printf("ALL OK\n");
+    return 0;
+}

@@ -1,0 +1,97 @@
+"""Occupancy groups of the high cells (CPU only): group counts through the C
+ABI against enumeration, the partition of the state, argument checks, and the
+per-group cell body run on the host against the oracle's full-state cells."""
+from __future__ import annotations
+
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import ofp
+from conftest import ROOT
+from group_masks import occupancy_masks
+
+HIPCC = "/opt/rocm/bin/hipcc"
+SRC = ROOT / "tests" / "native" / "group_host_check.cpp"
+
+
+@pytest.mark.parametrize("n,low", [(5, 2), (9, 4), (13, 6), (13, 1), (20, 10), (27, 14), (27, 20)])
+def test_groups_partition_the_state(n, low):
+    nm, nb = ofp.state_counts(n)
+    info, mc, bc = ofp.group_plan(n, 4, 0, 1 << low, low)
+    assert int(mc.sum()) == nm and int(bc.sum()) == nb
+    assert info["main_count"] == nm and info["blocked_count"] == nb
+    assert info["largest_group"] == int((mc + bc).max())
+    assert info["cells"] == n - low and info["low"] == low
+    # windows of consecutive masks add up
+    half = 1 << (low - 1)
+    a, _, _ = ofp.group_plan(n, 4, 0, half, low)
+    b, _, _ = ofp.group_plan(n, 4, half, half, low)
+    assert a["main_count"] + b["main_count"] == nm and a["blocked_count"] + b["blocked_count"] == nb
+
+
+def test_n27_low14_groups_fit_32bit_ranks():
+    """The W=28 LOW14 split (16,384 groups): every group ranks in 32 bits."""
+    info, mc, bc = ofp.group_plan(27, 4, 0, 1 << 14, 14)
+    assert int(mc.max()) == 961466716 and int(mc.max()) < 2**32 and int(bc.max()) < 2**32
+    assert int((mc > 0).sum()) == 1 << 14
+    assert info["state_bytes"] == (info["main_count"] + info["blocked_count"]) * 16 + (1 << 14) * 15360
+
+
+@pytest.mark.parametrize("n,low", [(9, 4), (11, 3), (12, 7)])
+def test_group_counts_match_enumeration(oracle, n, low):
+    nm, nb = ofp.state_counts(n)
+    _, mc, bc = ofp.group_plan(n, 1, 0, 1 << low, low)
+    lib = oracle.load()
+    mm = occupancy_masks(nm, n + 1, low)
+    mb = occupancy_masks(nb, n, low)
+    assert np.array_equal(np.bincount(mm, minlength=1 << low), mc.astype(np.int64))
+    assert np.array_equal(np.bincount(mb, minlength=1 << low), bc.astype(np.int64))
+    # the numpy unrank agrees with the oracle's on a sample
+    for i in range(0, nm, max(1, nm // 97)):
+        w = lib.or_unrank(i, n + 1)
+        assert mm[i] == sum(1 << p for p in range(low) if (w >> (2 * p)) & 3)
+
+
+@pytest.mark.parametrize("kw,msg", [({"low": -1}, "low"), ({"low": 9}, "low"), ({"low": 26, "n": 27}, "low"),
+                                    ({"first": 16, "groups": 1}, "groups"), ({"first": 8, "groups": 9}, "groups")])
+def test_group_plan_refuses(kw, msg):
+    n, low = kw.get("n", 9), kw.get("low", 4)
+    with pytest.raises(ofp.OfpError) as e:
+        ofp.group_plan(n, 1, kw.get("first", 0), kw.get("groups", 1), low)
+    assert e.value.status == -1
+
+
+@pytest.fixture(scope="module")
+def checker(tmp_path_factory, oracle):
+    if not shutil.which(HIPCC) and not (ROOT / HIPCC).exists():
+        pytest.skip("hipcc not available")
+    out = tmp_path_factory.mktemp("group") / "group_host_check"
+    lib = ROOT / "oracle"
+    cmd = [HIPCC, "-O1", "-std=c++17", "-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+           str(SRC), "-o", str(out), f"-L{lib}", "-loracle", f"-Wl,-rpath,{lib}"]
+    subprocess.run(cmd, check=True, capture_output=True, timeout=600)
+    return out
+
+
+def test_group_body_equals_oracle_cells(checker):
+    """Every group of every low split at n = 2..8, each swept alone on
+    group-local ranks, reproduces the oracle's full-state high cells."""
+    res = subprocess.run([str(checker), "8"], capture_output=True, text=True, timeout=600)
+    assert res.returncode == 0, res.stdout[-2000:] + res.stderr[-2000:]
+    assert res.stdout.strip().endswith("ALL OK")
+    assert res.stdout.count("ok ") >= 60
+
+
+def test_group_body_at_full_size(checker):
+    """Single groups of the n = 27, LOW14 split at their real size (and of n = 13):
+    the depth-first enumeration of the group comes out in global rank order, and
+    the body on group-local ranks equals the oracle's transitions applied to the
+    group's words alone, cell by cell, 27..15."""
+    for args in (["13", "6", "5", "17", "63"], ["27", "14", "3", "100"]):
+        res = subprocess.run([str(checker), "big", *args], capture_output=True, text=True, timeout=600)
+        assert res.returncode == 0, res.stdout[-2000:] + res.stderr[-2000:]
+        assert res.stdout.strip().endswith("ALL OK")
+        assert res.stdout.count("ok big") == len(args) - 2
