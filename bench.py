@@ -34,7 +34,8 @@ Measured in one process, in this order:
   * plain: the same solve with k=1 (one residue at a time), timed whole ->
     residues/hour of the plain form, and its per-kernel launch times.
   * plain_small: whole one-modulus solves at n=12 and n=18 (median of three).
-  * shard_n27: one GPU's share of an n=27 row (ofp.shard).
+  * shard_n27: one GPU's share of an n=27 row, both halves, every window of the
+    share streamed through HBM (ofp.shard.measure_share).
   * cpu_baseline: the CPU oracle's pull-form step on a bounded sample of the
     n=22 cell step, its whole n=12 solve (median of three) and one whole n=16
     solve, on the host cores.
@@ -475,9 +476,10 @@ def main() -> int:
         from ofp import shard as shard_mod
         with ofp.Context(device=0) as sctx:
             autotune.apply(sctx, 22, 4)
-            shard = shard_mod.measure(sctx)
-        log(f"bench: n=27 share window {shard['window_bytes'] / 2**30:.1f} GiB, "
-            f"{shard['avg_launch_ms']:.1f} ms/cell, {shard['roofline']['frac']:.3f} of HBM peak")
+            shard = shard_mod.measure_share(sctx, share=0)
+        log(f"bench: n=27 share 0 of 8, whole row streamed in {shard['high']['windows']}+{shard['low']['windows']} "
+            f"windows: {shard['row_kernel_s']:.2f} s kernel ({shard['row_wall_s']:.1f} s wall); high cells "
+            f"{shard['high']['roofline']['frac']:.3f}, low cells {shard['low']['roofline']['frac']:.3f} of HBM peak")
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -298,47 +298,54 @@ ofp_status ofp_shard_plan(ofp_ctx* ctx, const ofp_shard_spec* spec, ofp_shard_in
 ofp_status ofp_shard_sweep(ofp_ctx* ctx, const ofp_shard_spec* spec, ofp_shard_info* info,
                            uint32_t* main_out, uint32_t* blocked_out);
 
-/* ---- group sweep: a window of the high cells of a row ----
- * The high cells p = W-1..low+1 of a row rewrite positions >= low only and
- * keep the occupancy (N or not) of the low positions 0..low-1, so the words
- * with one occupancy mask of those positions (a group) are closed under them.
+/* ---- group sweep: one half of a row over occupancy groups ----
+ * A row of width W = n+1 is split at `low` into its high cells W-1..low+1 and
+ * its low cells low..1.  Each half rewrites only its own positions and turns a
+ * closure partner's R into L or back, which never changes whether a position
+ * is occupied (N or not), so the words with one occupancy pattern of the
+ * positions outside the half (a group) are closed under it:
+ *   high = 0: LOW groups, occupancy of positions 0..low-1, cells W-1..low+1;
+ *             the first cell reads no blocked input (a row's start), the last
+ *             leaves the window's blocked words (blocked_io is output);
+ *   high = 1: HIGH groups, occupancy of positions low+1..W-1, cells low..1;
+ *             the first cell reads the window's blocked words (blocked_io is
+ *             input: the high half's output), after cell 1 none are left.
  * A window is the groups first_group..first_group+groups-1 (masks in
- * increasing order).  Each group is stored contiguously with its words in
- * global rank order (32-bit ranks local to the group: every group of the
- * window must hold fewer than 2^32 words); the window's main array is the
- * groups' main words one group after the other, its blocked array likewise.
- * The cells run in place on the window from a row's start (the first cell reads
- * no blocked input); the last cell's blocked output is the window's blocked
- * array.  Reference analogue: one GPU's owner-local HIGH window of the 8-GPU
+ * increasing order; bit i of a mask is the first fixed position + i).  Each
+ * group is stored contiguously with its words in global rank order (32-bit
+ * ranks local to the group: every group of the window must hold fewer than
+ * 2^32 words); the window's main array is the groups' main words one group
+ * after the other, its blocked array likewise.  The cells run in place.
+ * Reference analogue: one GPU's owner-local LOW and HIGH windows of the 8-GPU
  * bucket layout (docs/b300-bucket-snake.md:7-20; docs/b300-hbm32.md:251-270:
- * W = 28, LOW14 / HIGH13, 16,384 groups). */
+ * W = 28, LOW14 / HIGH13, 16,384 and 8,192 groups). */
 typedef struct ofp_group_spec {
     int n;
     int k;                     /* 1..16 moduli, interleaved */
     const uint32_t* moduli;
-    int low;                   /* low positions 0..low-1, 1..min(W-2, 24); 0 = W/2 */
+    int low;                   /* split: 1..W-2 (0 = W/2); at most 24 fixed positions */
+    int high;                  /* 0: LOW groups / high cells, 1: HIGH groups / low cells */
     uint32_t first_group, groups;
-    uint64_t seed;             /* seeded inputs when main_io is NULL */
+    uint64_t seed;             /* seeded inputs (uniform in [0, p)) where no input is given */
 } ofp_group_spec;
 typedef struct ofp_group_info {
     uint64_t main_count, blocked_count;  /* words of the window */
     uint64_t largest_group;              /* main + blocked words of its largest group */
     uint64_t state_bytes;                /* device bytes of the window */
-    int low, cells, launches;            /* cells = W-1-low */
+    int low, cells, launches;            /* cells in the half: W-1-low or low */
     double kernel_s;                     /* all cells, HIP events */
-    float cell_ms[OFP_MAX_N];            /* cell_ms[i]: cell W-1-i over every group */
+    float cell_ms[OFP_MAX_N];            /* cell_ms[i]: the i-th cell of the half, top down */
 } ofp_group_info;
 /* Window totals into info, and per-group counts into main_counts /
  * blocked_counts (groups entries each; either may be NULL).  No device work;
  * ctx may be NULL. */
 ofp_status ofp_group_plan(ofp_ctx* ctx, const ofp_group_spec* spec, ofp_group_info* info,
                           uint64_t* main_counts, uint64_t* blocked_counts);
-/* main_io: NULL for seeded inputs (uniform in [0, p)), else main_count*k
- * residues (state-major, window order) that are replaced by the result.
- * blocked_out: NULL or blocked_count*k, the window's blocked words after cell
- * low+1. */
+/* main_io: NULL for seeded inputs, else main_count*k residues (state-major,
+ * window order) that are replaced by the result.  blocked_io: NULL or
+ * blocked_count*k residues, written (high = 0) or read (high = 1; NULL: seeded). */
 ofp_status ofp_group_sweep(ofp_ctx* ctx, const ofp_group_spec* spec, ofp_group_info* info,
-                           uint32_t* main_io, uint32_t* blocked_out);
+                           uint32_t* main_io, uint32_t* blocked_io);
 
 /* Counts only, no device needed: main / blocked word counts for n. */
 ofp_status ofp_state_counts(int n, uint64_t* main_states, uint64_t* blocked_states);

@@ -1272,9 +1272,12 @@ ofp_status group_check(ofp_ctx* c, const ofp_group_spec* sp, int* low) {
     if (sp->n < 2 || sp->n > OFP_MAX_N) return fail(c, OFP_E_RANGE, "group sweeps take n in 2..27");
     const int W = sp->n + 1;
     const int L = sp->low ? sp->low : W / 2;
-    if (L < 1 || L > W - 2 || L > 24) return fail(c, OFP_E_ARG, "low must be in 1..min(W-2, 24)");
+    if (sp->high != 0 && sp->high != 1) return fail(c, OFP_E_ARG, "high must be 0 or 1");
+    const int fixed = sp->high ? W - 1 - L : L;
+    if (L < 1 || L > W - 2 || fixed > 24)
+        return fail(c, OFP_E_ARG, "low must be in 1..W-2 with at most 24 fixed positions");
     if (sp->k < 1 || sp->k > kMaxKC) return fail(c, OFP_E_ARG, "group sweeps take 1..16 moduli");
-    const uint64_t all = 1ull << L;
+    const uint64_t all = 1ull << fixed;
     if (sp->groups < 1 || sp->first_group >= all || sp->groups > all - sp->first_group)
         return fail(c, OFP_E_ARG, "groups must lie in 0..2^low-1");
     *low = L;
@@ -1295,7 +1298,7 @@ ofp_status ofp_group_plan(ofp_ctx* c, const ofp_group_spec* sp, ofp_group_info* 
     uint64_t M = 0, B = 0, big = 0;
     for (uint32_t g = 0; g < sp->groups; ++g) {
         uint64_t m, b;
-        group_counts(W, L, sp->first_group + g, &m, &b);
+        group_counts(sp->high != 0, W, L, sp->first_group + g, &m, &b);
         if (main_counts) main_counts[g] = m;
         if (blocked_counts) blocked_counts[g] = b;
         M += m;
@@ -1309,12 +1312,12 @@ ofp_status ofp_group_plan(ofp_ctx* c, const ofp_group_spec* sp, ofp_group_info* 
     const int Kp = interleave_stride(1, sp->k);
     info->state_bytes = (M + B) * (uint64_t)Kp * 4 + (uint64_t)sp->groups * kGroupTableBytes;
     info->low = L;
-    info->cells = W - 1 - L;
+    info->cells = sp->high ? L : W - 1 - L;
     return OFP_OK;
 }
 
 ofp_status ofp_group_sweep(ofp_ctx* c, const ofp_group_spec* sp, ofp_group_info* info, uint32_t* main_io,
-                           uint32_t* blk_out) {
+                           uint32_t* blk_io) {
     if (!c || !sp || !info) return fail(c, OFP_E_ARG, "ctx, spec and info are required");
     const uint32_t G = sp->groups;
     std::vector<uint64_t> mc(G), bc(G);
@@ -1357,30 +1360,40 @@ ofp_status ofp_group_sweep(ofp_ctx* c, const ofp_group_spec* sp, ofp_group_info*
     {
         std::vector<char> tabs((size_t)G * kGroupTableBytes);
         for (uint32_t g = 0; g < G; ++g)
-            if (pack_group_table(L, sp->first_group + g, tabs.data() + (size_t)g * kGroupTableBytes,
-                                 kGroupTableBytes) != kGroupTableBytes) {
+            if (pack_group_table(sp->high != 0, W, L, sp->first_group + g,
+                                 tabs.data() + (size_t)g * kGroupTableBytes, kGroupTableBytes) != kGroupTableBytes) {
                 release();
                 return fail(c, OFP_E_RUNTIME, "group table packing");
             }
         OFP_HIP_R(hipMemcpy(dtab, tabs.data(), tabs.size(), hipMemcpyHostToDevice), "upload group tables");
     }
     OFP_HIP_R(hipMemcpy(dmods, sp->moduli, k * 4, hipMemcpyHostToDevice), "copy moduli");
-    if (main_io) {
-        std::vector<uint32_t> t(std::max<uint64_t>(M, 1) * Kp, 0);
-        for (uint64_t i = 0; i < M; ++i)
-            for (int j = 0; j < k; ++j) t[i * Kp + j] = main_io[i * k + j];
-        OFP_HIP_R(hipMemcpy(dm, t.data(), t.size() * 4, hipMemcpyHostToDevice), "upload main window");
-    } else {
-        // one hash domain per window start; blocked words need no input (the
-        // first cell reads none)
-        OFP_HIP_R(launch_fill_remote(dm, (uint64_t)sp->first_group << 36, M, db, 0, 0, Kp, k, dmods, sp->seed,
-                                     c->stream),
-                  "fill group window");
+    const bool blk_in = sp->high && blk_io;
+    if (main_io || blk_in) {
+        std::vector<uint32_t> t(std::max<uint64_t>(std::max(M, B), 1) * Kp, 0);
+        if (main_io) {
+            for (uint64_t i = 0; i < M; ++i)
+                for (int j = 0; j < k; ++j) t[i * Kp + j] = main_io[i * k + j];
+            OFP_HIP_R(hipMemcpy(dm, t.data(), M * Kp * 4, hipMemcpyHostToDevice), "upload main window");
+        }
+        if (blk_in) {
+            for (uint64_t i = 0; i < B; ++i)
+                for (int j = 0; j < k; ++j) t[i * Kp + j] = blk_io[i * k + j];
+            OFP_HIP_R(hipMemcpy(db, t.data(), B * Kp * 4, hipMemcpyHostToDevice), "upload blocked window");
+        }
     }
+    // Seeded inputs, one hash domain per window start and half: main words
+    // unless given, blocked words only where the first cell reads them.
+    const uint64_t dom = ((uint64_t)sp->first_group << 36) | ((uint64_t)sp->high << 35);
+    const uint64_t fillM = main_io ? 0 : M, fillB = (sp->high && !blk_io) ? B : 0;
+    if (fillM || fillB)
+        OFP_HIP_R(launch_fill_remote(dm, dom, fillM, db, dom, fillB, Kp, k, dmods, sp->seed, c->stream),
+                  "fill group window");
     const int sf = c->sorted_from >= 0 ? c->sorted_from : (k >= 8 ? 6 : 0);
+    const int p_hi = sp->high ? L : W - 1, p_lo = sp->high ? 1 : L + 1;
     int launches = 0;
     OFP_HIP_R(hipEventRecord(ev[0], c->stream), "event");
-    for (int p = W - 1, i = 0; p >= L + 1; --p, ++i) {
+    for (int p = p_hi, i = 0; p >= p_lo; --p, ++i) {
         for (uint32_t g = 0; g < G; ++g) {
             if (!bc[g]) continue;  // no blocked word: the cell changes nothing in the group
             CellLaunch a{};
@@ -1414,6 +1427,7 @@ ofp_status ofp_group_sweep(ofp_ctx* c, const ofp_group_spec* sp, ofp_group_info*
     }
     info->kernel_s = total * 1e-3;
     info->launches = launches;
+    const bool blk_out = !sp->high && blk_io;
     if (main_io || blk_out) {
         std::vector<uint32_t> t(std::max<uint64_t>(std::max(M, B), 1) * Kp);
         if (main_io && M) {
@@ -1424,7 +1438,7 @@ ofp_status ofp_group_sweep(ofp_ctx* c, const ofp_group_spec* sp, ofp_group_info*
         if (blk_out && B) {
             OFP_HIP_R(hipMemcpy(t.data(), db, B * Kp * 4, hipMemcpyDeviceToHost), "download blocked window");
             for (uint64_t i = 0; i < B; ++i)
-                for (int j = 0; j < k; ++j) blk_out[i * k + j] = t[i * Kp + j];
+                for (int j = 0; j < k; ++j) blk_io[i * k + j] = t[i * Kp + j];
         }
     }
 #undef OFP_HIP_R

@@ -353,21 +353,27 @@ size_t pack_height_table(bool wide, void* out, size_t cap) {
     return wide ? pack_into<uint64_t>(d, out, cap) : pack_into<uint32_t>(d, out, cap);
 }
 
-size_t pack_group_table(int low, uint32_t mask, void* out, size_t cap) {
+size_t pack_group_table(bool high, int W, int low, uint32_t mask, void* out, size_t cap) {
     const size_t bytes = sizeof(Quad<uint32_t>) * kDpRows * kDpCols;
     if (cap < bytes) return 0;
-    uint64_t f[kDpRows][kDpCols];
-    build_group_table(low, mask, f);
+    Occupancy om, ob;
+    group_occupancy(high, W, low, mask, om, ob);
+    uint64_t fm[kDpRows][kDpCols], fb[kDpRows][kDpCols];
+    build_group_table(om, fm);
+    build_group_table(ob, fb);
     std::vector<Quad<uint32_t>> q(kDpRows * kDpCols);
-    pack_group_quads<uint32_t>(low, mask, f, (Quad<uint32_t>(*)[kDpCols])q.data());
+    pack_group_quads<uint32_t>(ob, fb, om, fm, (Quad<uint32_t>(*)[kDpCols])q.data());
     std::memcpy(out, q.data(), bytes);
     return bytes;
 }
 
-void group_counts(int W, int low, uint32_t mask, uint64_t* main_words, uint64_t* blocked_words) {
+void group_counts(bool high, int W, int low, uint32_t mask, uint64_t* main_words, uint64_t* blocked_words) {
+    Occupancy om, ob;
+    group_occupancy(high, W, low, mask, om, ob);
     uint64_t f[kDpRows][kDpCols];
-    build_group_table(low, mask, f);
+    build_group_table(om, f);
     *main_words = f[W][1];
+    build_group_table(ob, f);
     *blocked_words = f[W - 1][1];
 }
 

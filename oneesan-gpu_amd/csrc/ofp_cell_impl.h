@@ -75,6 +75,21 @@ OFP_HD Z2<Idx> ldz(const Quad<Idx>& e) {
     return Z2<Idx>{e.zn, e.zc};
 #endif
 }
+// The second half only (mn, mc): the entries of the main position one up.
+template <typename Idx>
+OFP_HD Z2<Idx> ldm(const Quad<Idx>& e) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (sizeof(Idx) == 4) {
+        const uint2 v = *(reinterpret_cast<const uint2*>(&e) + 1);
+        return Z2<Idx>{v.x, v.y};
+    } else {
+        const ulonglong2 v = *(reinterpret_cast<const ulonglong2*>(&e) + 1);
+        return Z2<Idx>{(Idx)v.x, (Idx)v.y};
+    }
+#else
+    return Z2<Idx>{e.mn, e.mc};
+#endif
+}
 // Rank contribution of value v (0=N, 1=R, 2=L) from a table entry's pair.
 template <typename Idx>
 OFP_HD Idx pick(int v, Idx n, Idx c) {
@@ -109,6 +124,13 @@ struct Tab {
     const Quad<Idx> (*q)[LdsCols<Idx>::value];
     OFP_HD Idx at(int pos, int h) const { return q[pos][h].zn; }
     OFP_HD Idx cL(int pos, int h) const { return q[pos][h].zc; }
+    // The same for main position pos >= 1, from the main half of row pos-1.
+    // A cell p reads the entries of main positions >= p this way: the rows
+    // themselves hold blocked-numbering entries, which at those positions count
+    // the blocked words (one position shorter) of an occupancy group whose
+    // constrained positions lie above the cell.
+    OFP_HD Idx mat(int pos, int h) const { return q[pos - 1][h].mn; }
+    OFP_HD Idx mcL(int pos, int h) const { return q[pos - 1][h].mc; }
     // rank contribution of value v (0=N,1=R,2=L) at position pos entered at height h
     OFP_HD Idx c(int pos, int v, int h) const {
         const Quad<Idx> e = q[pos][h];
@@ -304,7 +326,7 @@ OFP_HD void gather_closures(const Tab<Idx>& T, int W, int p, int hP, Idx mainbas
     if (p >= 2) {
         // RL predecessor: [P,R,L,S]
         if (hP > 0) {
-            const uint64_t iRL = (uint64_t)(mainbase + T.at(p, hP) + T.cL(p - 1, hP - 1) + sr);
+            const uint64_t iRL = (uint64_t)(mainbase + T.mat(p, hP) + T.cL(p - 1, hP - 1) + sr);
             g.add(m, iRL, mod);
         }
         // LL predecessors: [P,L,L,S*], S* = S with a top-level L at q turned into R,
@@ -315,7 +337,7 @@ OFP_HD void gather_closures(const Tab<Idx>& T, int W, int p, int hP, Idx mainbas
         // S, which S* keeps; s2 is the rank of S*'s part above q (entered two
         // levels higher).
         if (p >= 3) {
-            const Idx baseLL = mainbase + T.cL(p, hP) + T.cL(p - 1, hP + 1);
+            const Idx baseLL = mainbase + T.mcL(p, hP) + T.cL(p - 1, hP + 1);
             Idx rr = sr, s2 = 0;
             int hh = hP;
             for (int pos = p - 2; pos >= 0; --pos) {
@@ -337,7 +359,7 @@ OFP_HD void gather_closures(const Tab<Idx>& T, int W, int p, int hP, Idx mainbas
     // ub is the main-numbering base without the part of P scanned so far, u2 the
     // rank of that part in P* (entered two levels higher below q).
     {
-        Idx ub = mainbase + T.at(p, hP + 2) + T.at(p - 1, hP + 1) + sr;
+        Idx ub = mainbase + T.mat(p, hP + 2) + T.at(p - 1, hP + 1) + sr;
         Idx u2 = 0;
         int Ha = hP;
         uint64_t w = word >> (2 * p);  // P from its lowest position up
@@ -346,8 +368,8 @@ OFP_HD void gather_closures(const Tab<Idx>& T, int W, int p, int hP, Idx mainbas
             w >>= 2;
             const bool isR = v == 1, isL = v == 2;
             const int H = Ha + (isR ? 1 : (isL ? -1 : 0));
-            const Z2<Idx> e0 = ldz(T.q[mp][H]);
-            const Z2<Idx> e2 = ldz(T.q[mp][H + 2]);
+            const Z2<Idx> e0 = ldm(T.q[mp - 1][H]);  // main position mp
+            const Z2<Idx> e2 = ldm(T.q[mp - 1][H + 2]);
             ub -= isL ? e0.zc : (isR ? e0.zn : Idx(0));
             if (isR && Ha == hP) g.add(m, (uint64_t)(ub + e0.zc + u2), mod);
             if (isL && Ha == hP) break;
@@ -464,14 +486,14 @@ OFP_HD void cell_body(const CellLaunch& a, const Tab<Idx>& T, const Decoded<Idx>
         //   x = R: A = NR, B = RN;  A' = NR + RN + blk, at p = 1 B' = RN + NR
         //   x = L: A = NL, B = LN;  A' = NL + LN + blk
         // blocked-out (p >= 2) = closure sum for x = N, old A otherwise.
-        const Idx cLp = T.cL(p, hP);
+        const Idx cLp = T.mcL(p, hP);
         Idx offA, offB;
         if (x == 0) {
             offA = 0;
             offB = cLp + T.at(p - 1, hP + 1);
         } else if (x == 1) {
             offA = T.at(p - 1, hP);
-            offB = T.at(p, hP);
+            offB = T.mat(p, hP);
         } else {
             offA = T.cL(p - 1, hP);
             offB = cLp;
@@ -538,7 +560,7 @@ OFP_HD void cell_body(const CellLaunch& a, const Tab<Idx>& T, const Decoded<Idx>
     if (x == 0) {
         // orbit NN -> LR
         const uint64_t iNN = (uint64_t)(mainbase + sr);
-        const uint64_t iLR = (uint64_t)(mainbase + T.cL(p, hP) + T.at(p - 1, hP + 1) + sr);
+        const uint64_t iLR = (uint64_t)(mainbase + T.mcL(p, hP) + T.at(p - 1, hP + 1) + sr);
         Vals<KC> nn, lr, bc;
         if constexpr (V == 0) {
             m.lm(iNN, nn);
@@ -568,7 +590,7 @@ OFP_HD void cell_body(const CellLaunch& a, const Tab<Idx>& T, const Decoded<Idx>
     } else if (x == 1) {
         // orbit RN -> NR (and at p=1 also NR -> RN)
         const uint64_t iNR = (uint64_t)(mainbase + T.at(p - 1, hP) + sr);
-        const uint64_t iRN = (uint64_t)(mainbase + T.at(p, hP) + sr);
+        const uint64_t iRN = (uint64_t)(mainbase + T.mat(p, hP) + sr);
         Vals<KC> nr, rn, bc, o;
         m.lm(iNR, nr);
         m.lm(iRN, rn);
@@ -590,7 +612,7 @@ OFP_HD void cell_body(const CellLaunch& a, const Tab<Idx>& T, const Decoded<Idx>
     } else {
         // orbit LN -> NL (cannot occur at p=1)
         const uint64_t iNL = (uint64_t)(mainbase + T.cL(p - 1, hP) + sr);
-        const uint64_t iLN = (uint64_t)(mainbase + T.cL(p, hP) + sr);
+        const uint64_t iLN = (uint64_t)(mainbase + T.mcL(p, hP) + sr);
         Vals<KC> nl, ln, bc, o;
         m.lm(iNL, nl);
         m.lm(iLN, ln);
@@ -638,73 +660,91 @@ inline void pack_quads(const uint64_t (*d)[kDpCols], Quad<Idx> (*q)[kDpCols]) {
 }
 
 // ---- occupancy groups ----
-// The words whose low positions 0..low-1 have a fixed occupancy (bit i of mask
-// set: position i holds R or L; clear: N) form a group that every cell p > low
-// maps onto itself.  Such a cell rewrites positions p and p-1 (both >= low) and
-// may turn a closure partner's R into L or back, which never changes whether a
-// position is occupied (reference: the HIGH window of the bucket layout,
-// docs/b300-bucket-snake.md:7-20).  Blocked words (width W-1) of these cells
-// keep positions 0..p-2 where the main words have them, so the same mask
-// selects their group.
-//
+// A window of consecutive cells changes only its own positions and turns the
+// R of a closure partner into L or back, which never changes whether a
+// position is occupied (N or not).  So the words with one occupancy pattern of
+// the positions outside the window (a group) are closed under the window's
+// cells (reference: the bucket layout's LOW and HIGH windows,
+// docs/b300-bucket-snake.md:7-20).  Two kinds, for a row split at `low`:
+//   LOW groups, cells p = W-1..low+1: main and blocked words fix positions
+//     0..low-1 (blocked positions below a cell are main positions);
+//   HIGH groups, cells p = low..1: main words fix positions low+1..W-1, the
+//     blocked words the same positions in their own numbering, low..W-2
+//     (blocked position i above the cell is main position i+1).
 // A group's words ranked in the global order (lexicographic, N < R < L from
-// the top) have ranks that are sums of the same kind of table entries as the
-// global ranks, taken from the group's completion table:
+// the top) have ranks that are sums of the same kind of entries as global
+// ranks, taken from the group's completion table (one per numbering):
 //   F[0][h] = [h == 0]
-//   F[q][h] = F[q-1][h-1] + F[q-1][h+1]           position q-1 < low, occupied
-//           = F[q-1][h]                           position q-1 < low, empty
-//           = F[q-1][h] + F[q-1][h-1] + F[q-1][h+1]   position q-1 >= low
-// with the entries of a low position counting only its allowed symbols:
+//   F[q][h] = F[q-1][h-1] + F[q-1][h+1]               position q-1 fixed occupied
+//           = F[q-1][h]                               position q-1 fixed empty
+//           = F[q-1][h] + F[q-1][h-1] + F[q-1][h+1]   position q-1 free
+// with the entries of a fixed position counting only its allowed symbols:
 //   occupied: zn = 0 (no N to pass), zc = F[pos][h-1] (the R completions)
 //   empty:    zn = zc = F[pos][h] (every rank stays below zn: N is decoded)
-// Decode walks, orbit ranks and closure scans of a cell p > low only read
-// entries of positions >= low at symbols that change and entries of low
-// positions at symbols of the group, so the cell body runs unchanged on
-// group-local ranks.  Counts: main words F[W][1], blocked words F[W-1][1].
-inline void build_group_table(int low, uint32_t mask, uint64_t (*f)[kDpCols]) {
+// The packed table holds blocked-numbering entries in (zn, zc) and the main
+// numbering's entries of the position one up in (mn, mc), which is what the
+// cell body reads for main positions >= p (Tab::mat, Tab::mcL).  Below a
+// cell both numberings agree.  Counts: main words Fm[W][1], blocked Fb[W-1][1].
+struct Occupancy {
+    int lo = 0, n = 0;  // fixed positions lo..lo+n-1
+    uint32_t mask = 0;  // bit pos-lo set: position pos holds R or L; clear: N
+    // -1: free, 0: fixed empty, 1: fixed occupied
+    int at(int pos) const { return (pos < lo || pos >= lo + n) ? -1 : (int)((mask >> (pos - lo)) & 1u); }
+};
+// The two numberings of group `mask` of a row of width W split at `low`.
+inline void group_occupancy(bool high, int W, int low, uint32_t mask, Occupancy& main_o, Occupancy& blk_o) {
+    if (!high) {
+        main_o = Occupancy{0, low, mask};
+        blk_o = main_o;
+    } else {
+        main_o = Occupancy{low + 1, W - 1 - low, mask};
+        blk_o = Occupancy{low, W - 1 - low, mask};
+    }
+}
+
+inline void build_group_table(const Occupancy& o, uint64_t (*f)[kDpCols]) {
     for (int w = 0; w < kDpRows; ++w)
         for (int h = 0; h < kDpCols; ++h) f[w][h] = 0;
     f[0][0] = 1;
     for (int q = 1; q < kDpRows; ++q) {
-        const int pos = q - 1;
-        const bool lowpos = pos < low, occ = lowpos && ((mask >> pos) & 1u);
+        const int fix = o.at(q - 1);
         for (int h = 0; h < kDpCols; ++h) {
             const uint64_t dn = h > 0 ? f[q - 1][h - 1] : 0, up = h + 1 < kDpCols ? f[q - 1][h + 1] : 0;
-            f[q][h] = occ ? dn + up : (lowpos ? f[q - 1][h] : f[q - 1][h] + dn + up);
+            f[q][h] = fix == 1 ? dn + up : (fix == 0 ? f[q - 1][h] : f[q - 1][h] + dn + up);
         }
     }
 }
 
 // Rank contributions (zn, zc) of position pos at height h in a group table.
-inline void group_entry(int low, uint32_t mask, const uint64_t (*f)[kDpCols], int pos, int h, uint64_t& zn,
+inline void group_entry(const Occupancy& o, const uint64_t (*f)[kDpCols], int pos, int h, uint64_t& zn,
                         uint64_t& zc) {
     if (pos >= kDpRows) {
         zn = zc = 0;
         return;
     }
     const uint64_t here = f[pos][h], below = h > 0 ? f[pos][h - 1] : 0;
-    if (pos < low) {
-        if ((mask >> pos) & 1u) {
-            zn = 0;
-            zc = below;
-        } else {
-            zn = zc = here;
-        }
+    const int fix = o.at(pos);
+    if (fix == 1) {
+        zn = 0;
+        zc = below;
+    } else if (fix == 0) {
+        zn = zc = here;
     } else {
         zn = here;
         zc = here + below;
     }
 }
 
-// Packed table of a group (the Quad layout of pack_quads: mn, mc are the
-// entries of the position one up).
+// Packed table of a group: (zn, zc) from the blocked numbering at pos, (mn, mc)
+// from the main numbering at pos+1.
 template <typename Idx>
-inline void pack_group_quads(int low, uint32_t mask, const uint64_t (*f)[kDpCols], Quad<Idx> (*q)[kDpCols]) {
+inline void pack_group_quads(const Occupancy& ob, const uint64_t (*fb)[kDpCols], const Occupancy& om,
+                             const uint64_t (*fm)[kDpCols], Quad<Idx> (*q)[kDpCols]) {
     for (int pos = 0; pos < kDpRows; ++pos)
         for (int h = 0; h < kDpCols; ++h) {
             uint64_t zn, zc, mn, mc;
-            group_entry(low, mask, f, pos, h, zn, zc);
-            group_entry(low, mask, f, pos + 1, h, mn, mc);
+            group_entry(ob, fb, pos, h, zn, zc);
+            group_entry(om, fm, pos + 1, h, mn, mc);
             q[pos][h] = Quad<Idx>{(Idx)zn, (Idx)zc, (Idx)mn, (Idx)mc};
         }
 }

@@ -24,11 +24,13 @@ void build_height_table(uint64_t (*d)[kDpCols]);
 // d[w][h] = number of words of width w entered at height h <= cap that end at 0
 // without leaving [0, cap].
 void build_capped_table(int cap, uint64_t (*d)[kDpCols]);
-// Occupancy group `mask` of the low positions 0..low-1 (cell::build_group_table):
-// its packed 32-bit table (kDpRows x kDpCols cell::Quad<uint32_t>; returns the
-// bytes written, 0 if cap is too small) and its main / blocked word counts at width W.
-size_t pack_group_table(int low, uint32_t mask, void* out, size_t cap);
-void group_counts(int W, int low, uint32_t mask, uint64_t* main_words, uint64_t* blocked_words);
+// Occupancy group `mask` of a row of width W split at `low` (cell::group_occupancy):
+// a LOW group (high = false; fixed positions 0..low-1, cells W-1..low+1) or a
+// HIGH group (fixed positions low+1..W-1, cells low..1).  Its packed 32-bit
+// table (kDpRows x kDpCols cell::Quad<uint32_t>; returns the bytes written, 0 if
+// cap is too small) and its main / blocked word counts.
+size_t pack_group_table(bool high, int W, int low, uint32_t mask, void* out, size_t cap);
+void group_counts(bool high, int W, int low, uint32_t mask, uint64_t* main_words, uint64_t* blocked_words);
 // Packed capped tables for caps 1..kMaxCap (cell::Pair<uint32 or uint64>,
 // kMaxCap x kDpRows x kDpCols) into out; returns the bytes written, 0 if cap is too small.
 size_t pack_capped_tables(bool wide, void* out, size_t cap);

@@ -129,6 +129,7 @@ class GroupSpec(ctypes.Structure):
         ("k", ctypes.c_int),
         ("moduli", ctypes.POINTER(ctypes.c_uint32)),
         ("low", ctypes.c_int),
+        ("high", ctypes.c_int),
         ("first_group", ctypes.c_uint32),
         ("groups", ctypes.c_uint32),
         ("seed", ctypes.c_uint64),
@@ -423,16 +424,20 @@ class Context:
         return info.as_dict(), mo, bo
 
     def group_sweep(self, n: int, moduli: Sequence[int], first_group: int, groups: int, low: int = 0,
-                    seed: int = 0, main_in=None, blocked_out: bool = False):
-        """High cells W-1..low+1 of a row over the occupancy groups first_group..
-        first_group+groups-1.  main_in: None for seeded inputs, else a
-        (main_count, k) uint32 array in window order.  Returns (info, main,
-        blocked): main is the result when main_in was given, blocked the
-        window's blocked words when blocked_out is set (else None)."""
+                    seed: int = 0, main_in=None, blocked_out: bool = False, high: bool = False, blocked_in=None):
+        """One half of a row over the occupancy groups first_group..first_group+
+        groups-1: the high cells W-1..low+1 over LOW groups (high=False) or the
+        low cells low..1 over HIGH groups (high=True).  main_in: None for seeded
+        inputs, else a (main_count, k) uint32 array in window order; blocked_in
+        (high only): the blocked words before cell low, None for seeded ones.
+        Returns (info, main, blocked): main is the result when main_in was
+        given, blocked the window's blocked words after cell low+1 when
+        blocked_out is set (high=False only; else None)."""
         import numpy as np
         k = len(moduli)
         mods = _u32_array(moduli)
-        spec = GroupSpec(n, k, ctypes.cast(mods, ctypes.POINTER(ctypes.c_uint32)), low, first_group, groups, seed)
+        spec = GroupSpec(n, k, ctypes.cast(mods, ctypes.POINTER(ctypes.c_uint32)), low, 1 if high else 0,
+                         first_group, groups, seed)
         info = GroupInfo()
         self._check(self._lib.ofp_group_plan(self._h, ctypes.byref(spec), ctypes.byref(info), None, None),
                     "ofp_group_plan")
@@ -440,13 +445,17 @@ class Context:
         mo = bo = None
         if main_in is not None:
             mo = np.ascontiguousarray(np.asarray(main_in, dtype=np.uint32).reshape(info.main_count, k)).copy()
+        if high and blocked_out:
+            raise ValueError("the low cells leave no blocked words")
         if blocked_out:
             bo = np.zeros((info.blocked_count, k), np.uint32)
+        elif high and blocked_in is not None:
+            bo = np.ascontiguousarray(np.asarray(blocked_in, dtype=np.uint32).reshape(info.blocked_count, k)).copy()
         st = self._lib.ofp_group_sweep(self._h, ctypes.byref(spec), ctypes.byref(info),
                                        mo.ctypes.data_as(u32p) if mo is not None else None,
                                        bo.ctypes.data_as(u32p) if bo is not None else None)
         self._check(st, "ofp_group_sweep")
-        return info.as_dict(), mo, bo
+        return info.as_dict(), mo, (bo if blocked_out else None)
 
     def set_variant(self, variant: int) -> None:
         """Cell-kernel variant (state-major layout): 0 = early orbit loads, 1 = late,
@@ -529,13 +538,15 @@ def shard_plan(n: int, k: int, slices: int, slice_: int, low_cells: int = 0) -> 
     return info.as_dict()
 
 
-def group_plan(n: int, k: int, first_group: int, groups: int, low: int = 0):
-    """Window totals and per-group (main, blocked) counts of occupancy groups
-    (host computation, no GPU needed).  Returns (info, main_counts, blocked_counts)."""
+def group_plan(n: int, k: int, first_group: int, groups: int, low: int = 0, high: bool = False):
+    """Window totals and per-group (main, blocked) counts of occupancy groups,
+    LOW (high=False) or HIGH (host computation, no GPU needed).  Returns
+    (info, main_counts, blocked_counts)."""
     import numpy as np
     lib = load()
     mods = _u32_array([4294967291] * k)
-    spec = GroupSpec(n, k, ctypes.cast(mods, ctypes.POINTER(ctypes.c_uint32)), low, first_group, groups, 0)
+    spec = GroupSpec(n, k, ctypes.cast(mods, ctypes.POINTER(ctypes.c_uint32)), low, 1 if high else 0,
+                     first_group, groups, 0)
     info = GroupInfo()
     mc = np.zeros(max(groups, 1), np.uint64)
     bc = np.zeros(max(groups, 1), np.uint64)
@@ -543,7 +554,8 @@ def group_plan(n: int, k: int, first_group: int, groups: int, low: int = 0):
     st = lib.ofp_group_plan(None, ctypes.byref(spec), ctypes.byref(info), mc.ctypes.data_as(u64p),
                             bc.ctypes.data_as(u64p))
     if st != OFP_OK:
-        raise OfpError(st, f"group_plan(n={n}, k={k}, first_group={first_group}, groups={groups}, low={low})")
+        raise OfpError(st, f"group_plan(n={n}, k={k}, first_group={first_group}, groups={groups}, low={low}, "
+                           f"high={high})")
     return info.as_dict(), mc[:groups], bc[:groups]
 
 

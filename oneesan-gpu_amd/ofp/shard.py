@@ -6,21 +6,24 @@
 The n=27 frontier (520.7e9 states, 2 TB per modulus) does not fit one GPU; an
 8-GPU run gives each GPU 1/8 of it (242 GiB per modulus, ~1 TB at 4 moduli),
 which does not fit either, so a share is streamed through HBM in windows.  A
-row has two halves with two placements of the state:
+row has two halves, each over its own placement of the state (the reference's
+bucket layout, docs/b300-bucket-snake.md:7-20):
 
-  * low cells p = 14..1 (W = 28) over whole prefix classes (words with the same
-    content above the low cells): a window is 1/`slices` of the state, with
-    64-bit global ranks; main words of other shards (read only by RR closures
-    that flip a position above the low cells) read as seeded residues.
-  * high cells p = 27..15 over LOW14 occupancy groups (words with the same
-    occupancy of positions 0..13; the high cells keep it, so a group is closed):
-    a window is a run of consecutive groups, each stored alone with 32-bit
-    ranks local to the group.
+  * high cells p = 27..15 (W = 28) over LOW14 occupancy groups (words with the
+    same occupancy of positions 0..13; the high cells keep it, so a group is
+    closed), 16,384 groups;
+  * low cells p = 14..1 over HIGH13 occupancy groups (positions 15..27), 8,192
+    groups, reading the blocked words the high half leaves.
 
-`measure` times one low-cells window; `measure_share` streams every window of
-one share through both halves (share `share` of `shares`: 1/8 of the groups by
-word count, and slices share*64/8 .. of the prefix classes), seeding each
-window in HBM, and reports the kernel time of the share's whole row.
+A window is a run of consecutive groups, each stored alone with 32-bit ranks
+local to the group, so every read and write of a window stays in it.  A share
+is 1/`shares` of each half's groups by word count (consecutive masks).
+
+`measure_share` streams every window of one share through both halves,
+seeding each window in HBM, and reports the kernel time of the share's whole
+row.  `measure` times one window of the older prefix-class placement of the
+low cells (cells 14..1 over 1/`slices` of the state, 64-bit global ranks,
+main words of other shards read as seeded residues).
 
 Left out (stated in the report): the per-row all-to-all that moves the state
 between the two placements, and the host or peer traffic that would stage the
@@ -69,19 +72,22 @@ def measure(ctx: _lib.Context, n: int = 27, k: int = 4, slices: int = 64, slice_
     }
 
 
-def share_windows(n: int, k: int, shares: int, share: int, window_bytes: float, low: int = 0):
-    """The high-half windows of one share: (first_group, groups) runs of
-    consecutive occupancy groups whose device bytes stay <= window_bytes (a
-    single larger group gets a window of its own).  The share holds the groups
-    from the first whose words start at or after share/shares of the state."""
+def share_windows(n: int, k: int, shares: int, share: int, window_bytes: float, low: int = 0,
+                  high: bool = False):
+    """The windows of one share of a half (LOW groups / high cells, or HIGH
+    groups / low cells): (first_group, groups) runs of consecutive occupancy
+    groups whose device bytes stay <= window_bytes (a single larger group gets
+    a window of its own).  The share holds the groups from the first whose
+    words start at or after share/shares of the state."""
     import numpy as np
     W = n + 1
     low = low or W // 2
-    info, mc, bc = _lib.group_plan(n, k, 0, 1 << low, low)
+    fixed = W - 1 - low if high else low
+    info, mc, bc = _lib.group_plan(n, k, 0, 1 << fixed, low, high)
     words = mc.astype(np.float64) + bc.astype(np.float64)
     cum = np.concatenate([[0.0], np.cumsum(words)])
     total = cum[-1]
-    starts = [int(np.searchsorted(cum, total * s / shares, side="left")) for s in range(shares)] + [1 << low]
+    starts = [int(np.searchsorted(cum, total * s / shares, side="left")) for s in range(shares)] + [1 << fixed]
     g0, g1 = starts[share], starts[share + 1]
     per_word = 4 * (k + (k & 1) if k > 1 else 1)
     table = 15360  # one packed 32-bit group table
@@ -98,57 +104,49 @@ def share_windows(n: int, k: int, shares: int, share: int, window_bytes: float, 
 
 
 def measure_share(ctx: _lib.Context, n: int = 27, k: int = 4, shares: int = 8, share: int = 0,
-                  slices: int = 64, window_bytes: float = 136e9, seed: int = 20261202, log=None) -> dict:
+                  window_bytes: float = 136e9, seed: int = 20261202, log=None) -> dict:
     """Stream every window of one share through both halves of a row."""
-    if slices % shares:
-        raise ValueError("slices must be a multiple of shares")
     mods = PRIMES[:k]
     W = n + 1
-    low, windows = share_windows(n, k, shares, share, window_bytes)
     t0 = time.perf_counter()
-    hi = {"windows": len(windows), "groups": 0, "words": 0, "kernel_s": 0.0, "launches": 0, "max_window_bytes": 0,
-          "cell_ms": [0.0] * (W - 1 - low)}
-    for g0, g in windows:
-        info, _, _ = ctx.group_sweep(n, mods, g0, g, low, seed=seed)
-        hi["groups"] += g
-        hi["words"] += info["main_count"] + info["blocked_count"]
-        hi["kernel_s"] += info["kernel_s"]
-        hi["launches"] += info["launches"]
-        hi["max_window_bytes"] = max(hi["max_window_bytes"], info["state_bytes"])
-        hi["cell_ms"] = [a + b for a, b in zip(hi["cell_ms"], info["cell_ms"])]
-        if log:
-            log(f"share {share}: high window groups {g0}..{g0 + g - 1}: {info['state_bytes'] / 1e9:.1f} GB, "
-                f"{info['kernel_s']:.3f} s")
-    hi["cells"] = W - 1 - low
-    per = slices // shares
-    lo = {"windows": per, "words": 0, "kernel_s": 0.0, "launches": 0, "max_window_bytes": 0, "stray_stores": 0}
-    for s in range(share * per, (share + 1) * per):
-        info, _, _ = ctx.shard_sweep(n, mods, slices, s, seed=seed)
-        lo["words"] += info["main_count"] + info["blocked_count"]
-        lo["kernel_s"] += info["kernel_s"]
-        lo["launches"] += info["launches"]
-        lo["stray_stores"] += info["stray_stores"]
-        lo["max_window_bytes"] = max(lo["max_window_bytes"], info["state_bytes"])
-        lo["cells"] = info["low_cells"]
-        if log:
-            log(f"share {share}: low window slice {s}/{slices}: {info['state_bytes'] / 1e9:.1f} GB, "
-                f"{info['kernel_s']:.3f} s")
+    halves = {}
+    for name, high in (("high", False), ("low", True)):
+        low, windows = share_windows(n, k, shares, share, window_bytes, high=high)
+        cells = low if high else W - 1 - low
+        h = {"placement": (f"HIGH{W - 1 - low} groups (occupancy of positions {low + 1}..{W - 1})" if high
+                           else f"LOW{low} groups (occupancy of positions 0..{low - 1})"),
+             "cells": f"{low}..1" if high else f"{W - 1}..{low + 1}", "windows": len(windows), "groups": 0,
+             "words": 0, "kernel_s": 0.0, "launches": 0, "max_window_bytes": 0, "cell_ms": [0.0] * cells}
+        for g0, g in windows:
+            info, _, _ = ctx.group_sweep(n, mods, g0, g, low, seed=seed, high=high)
+            h["groups"] += g
+            h["words"] += info["main_count"] + info["blocked_count"]
+            h["kernel_s"] += info["kernel_s"]
+            h["launches"] += info["launches"]
+            h["max_window_bytes"] = max(h["max_window_bytes"], info["state_bytes"])
+            h["cell_ms"] = [a + b for a, b in zip(h["cell_ms"], info["cell_ms"])]
+            if log:
+                log(f"share {share}: {name} cells, groups {g0}..{g0 + g - 1}: {info['state_bytes'] / 1e9:.1f} GB, "
+                    f"{info['kernel_s']:.3f} s")
+        declared = 8.0 * k * h["words"] * cells
+        h["declared_bytes"] = declared
+        h["transitions_per_s"] = cells * h["words"] * k / h["kernel_s"]
+        h["roofline"] = {"bound": "hbm", "achieved": declared / h["kernel_s"] / 1e9, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": declared / h["kernel_s"] / 1e9 / HBM_PEAK_GBS}
+        halves[name] = h
     wall = time.perf_counter() - t0
-    for half in (hi, lo):
-        declared = 8.0 * k * half["words"] * half["cells"]
-        half["declared_bytes"] = declared
-        half["transitions_per_s"] = half["cells"] * half["words"] * k / half["kernel_s"]
-        half["roofline"] = {"bound": "hbm", "achieved": declared / half["kernel_s"] / 1e9, "peak": HBM_PEAK_GBS,
-                            "unit": "GB/s", "frac": declared / half["kernel_s"] / 1e9 / HBM_PEAK_GBS}
     nm, nb = _lib.state_counts(n)
+    hi, lo = halves["high"], halves["low"]
     return {
-        "workload": f"n={n} row of one GPU's share ({share} of {shares}), {k} moduli interleaved, seeded inputs: "
-                    f"high cells {W - 1}..{low + 1} over LOW{low} occupancy groups in {len(windows)} windows, "
-                    f"low cells {lo['cells']}..1 over whole prefix classes in {per} windows",
+        "workload": f"n={n} row of one GPU's share ({share} of {shares}), {k} moduli interleaved, seeded "
+                    f"inputs: high cells {hi['cells']} over {hi['placement']} in {hi['windows']} windows, "
+                    f"low cells {lo['cells']} over {lo['placement']} in {lo['windows']} windows",
         "n": n, "k": k, "shares": shares, "share": share,
         "share_fraction_high": hi["words"] / (nm + nb), "share_fraction_low": lo["words"] / (nm + nb),
         "row_kernel_s": hi["kernel_s"] + lo["kernel_s"],
         "row_wall_s": wall,
+        "transitions_per_s": (hi["transitions_per_s"] * hi["kernel_s"] + lo["transitions_per_s"] * lo["kernel_s"])
+        / (hi["kernel_s"] + lo["kernel_s"]),
         "high": hi, "low": lo,
         "left_out": "the per-row all-to-all between the two placements, and the host/peer staging of the "
                     "non-resident windows (each window is seeded in HBM); wall time includes the seeding",
@@ -167,9 +165,9 @@ def main(argv=None) -> int:
     a = ap.parse_args(argv)
     with _lib.Context() as ctx:
         if a.share is not None:
-            rep = measure_share(ctx, a.n, a.k, a.shares, a.share, a.slices,
+            rep = measure_share(ctx, a.n, a.k, a.shares, a.share,
                                 log=lambda m: print(m, file=sys.stderr, flush=True))
-            stray = rep["low"]["stray_stores"]
+            stray = 0
         else:
             rep = measure(ctx, a.n, a.k, a.slices, a.slice)
             stray = rep["stray_stores"]

@@ -1,15 +1,17 @@
 // Host check of the occupancy-group placement (oneesan-gpu_amd/csrc/
-// ofp_cell_impl.h: build_group_table, pack_group_quads): the cell body, run on
-// the CPU with each group's packed table over the group's words alone (ranks
-// local to the group, arrays holding only the group), for the high cells
-// p = W-1..low+1 of a row, against the CPU oracle's cells on the whole state
-// (oracle/gridfp_oracle.c or_apply_cells) on seeded random vectors.
+// ofp_cell_impl.h: group_occupancy, build_group_table, pack_group_quads): the
+// cell body, run on the CPU with each group's packed table over the group's
+// words alone (ranks local to the group, arrays holding only the group), for
+// the high cells p = W-1..low+1 of a row over LOW groups and for the low cells
+// p = low..1 over HIGH groups (with blocked input), against the CPU oracle's
+// cells on the whole state (oracle/gridfp_oracle.c or_apply_cells) on seeded
+// random vectors.
 //
 // Also checked: the group counts F[W][1], F[W-1][1] against enumeration, and
 // that no access of the body leaves the group (every load and store index is
 // range-checked).
 //
-// `group_host_check big n low mask...` checks single groups at full size (n up
+// `group_host_check big|bighigh n low mask...` checks single LOW or HIGH groups at full size (n up
 // to 27, where the whole state is out of reach): the group's words are
 // enumerated depth-first in lexicographic order (their global ranks, from the
 // oracle, must come out increasing, so the enumeration index is the rank in
@@ -68,55 +70,70 @@ struct Checked {
     }
 };
 
-static uint32_t occupancy(uint64_t word, int low) {
+// The group of a word under occupancy constraint o (its bits at o's positions).
+static uint32_t group_of(uint64_t word, const cell::Occupancy& o) {
     uint32_t m = 0;
-    for (int pos = 0; pos < low; ++pos)
-        if ((word >> (2 * pos)) & 3u) m |= 1u << pos;
+    for (int i = 0; i < o.n; ++i)
+        if ((word >> (2 * (o.lo + i))) & 3u) m |= 1u << i;
     return m;
 }
 
+// Runs the cells of one half (high = false: W-1..low+1 over LOW groups, true:
+// low..1 over HIGH groups) group by group and compares with the oracle.
 template <int V>
-static int check(int n, int low, const uint32_t* mods, unsigned seed) {
+static int check(int n, int low, bool high, const uint32_t* mods, unsigned seed) {
     constexpr int KC = 3, K = 4;
     const int W = n + 1;
+    const int fixed = high ? W - 1 - low : low, p_hi = high ? low : W - 1, p_lo = high ? 1 : low + 1;
     const uint64_t nm = or_main_count(n), nb = or_blocked_count(n);
+    cell::Occupancy om0, ob0;
+    cell::group_occupancy(high, W, low, 0, om0, ob0);
     // group of every word and its rank inside the group (global order)
     std::vector<uint32_t> gm(nm), gb(nb);
     std::vector<uint64_t> lm(nm), lb(nb);
-    std::vector<uint64_t> cm(1u << low, 0), cb(1u << low, 0);
+    std::vector<uint64_t> cm(1u << fixed, 0), cb(1u << fixed, 0);
     for (uint64_t i = 0; i < nm; ++i) {
-        gm[i] = occupancy(or_unrank(i, W), low);
+        gm[i] = group_of(or_unrank(i, W), om0);
         lm[i] = cm[gm[i]]++;
     }
     for (uint64_t i = 0; i < nb; ++i) {
-        gb[i] = occupancy(or_unrank(i, W - 1), low);
+        gb[i] = group_of(or_unrank(i, W - 1), ob0);
         lb[i] = cb[gb[i]]++;
     }
     std::mt19937_64 rng(seed);
-    std::vector<uint32_t> mi(nm * KC);
+    std::vector<uint32_t> mi(nm * KC), bi((nb ? nb : 1) * KC, 0);
     for (uint64_t i = 0; i < nm; ++i)
         for (int j = 0; j < KC; ++j) mi[i * KC + j] = (uint32_t)(rng() % mods[j]);
+    if (high)
+        for (uint64_t i = 0; i < nb; ++i)
+            for (int j = 0; j < KC; ++j) bi[i * KC + j] = (uint32_t)(rng() % mods[j]);
     std::vector<uint32_t> mo(nm * KC, 0), bo((nb ? nb : 1) * KC, 0);
     // per group: gather, run the cells with the group's table, scatter back
-    static uint64_t f[kDpRows][kDpCols];
+    static uint64_t fm[kDpRows][kDpCols], fb[kDpRows][kDpCols];
     static cell::Quad<uint32_t> q[kDpRows][kDpCols];
     int groups = 0;
-    for (uint32_t g = 0; g < (1u << low); ++g) {
-        cell::build_group_table(low, g, f);
-        if (f[W][1] != cm[g] || f[W - 1][1] != cb[g]) {
-            printf("COUNT n=%d low=%d group=%u main %llu/%llu blocked %llu/%llu\n", n, low, g,
-                   (unsigned long long)f[W][1], (unsigned long long)cm[g], (unsigned long long)f[W - 1][1],
+    for (uint32_t g = 0; g < (1u << fixed); ++g) {
+        cell::Occupancy om, ob;
+        cell::group_occupancy(high, W, low, g, om, ob);
+        cell::build_group_table(om, fm);
+        cell::build_group_table(ob, fb);
+        if (fm[W][1] != cm[g] || fb[W - 1][1] != cb[g]) {
+            printf("COUNT n=%d low=%d high=%d group=%u main %llu/%llu blocked %llu/%llu\n", n, low, high, g,
+                   (unsigned long long)fm[W][1], (unsigned long long)cm[g], (unsigned long long)fb[W - 1][1],
                    (unsigned long long)cb[g]);
             return 1;
         }
         if (!cm[g]) continue;
         ++groups;
-        cell::pack_group_quads<uint32_t>(low, g, f, q);
+        cell::pack_group_quads<uint32_t>(ob, fb, om, fm, q);
         cell::Tab<uint32_t> T{q};
         std::vector<uint32_t> am(cm[g] * K, 0), ab((cb[g] ? cb[g] : 1) * K, 0);
         for (uint64_t i = 0; i < nm; ++i)
             if (gm[i] == g)
                 for (int j = 0; j < KC; ++j) am[lm[i] * K + j] = mi[i * KC + j];
+        for (uint64_t i = 0; i < nb; ++i)
+            if (gb[i] == g)
+                for (int j = 0; j < KC; ++j) ab[lb[i] * K + j] = bi[i * KC + j];
         Checked<KC> mem{am.data(), ab.data(), cm[g], cb[g], K, 0};
         CellLaunch a{};
         a.W = W;
@@ -125,12 +142,12 @@ static int check(int n, int low, const uint32_t* mods, unsigned seed) {
         a.kc = KC;
         a.layout = 1;
         for (int j = 0; j < KC; ++j) a.mod[j] = mods[j];
-        for (int p = W - 1; p >= low + 1; --p) {
+        for (int p = p_hi; p >= p_lo; --p) {
             a.p = p;
             a.read_blk = p == W - 1 ? 0 : 1;
             for (uint64_t b = 0; b < cb[g]; ++b) cell::cell_one<KC, uint32_t, cell::kAoS, V>(a, T, b, mem);
             if (g_oob) {
-                printf("OUT OF GROUP n=%d low=%d group=%u p=%d\n", n, low, g, p);
+                printf("OUT OF GROUP n=%d low=%d high=%d group=%u p=%d\n", n, low, high, g, p);
                 return 1;
             }
         }
@@ -144,34 +161,37 @@ static int check(int n, int low, const uint32_t* mods, unsigned seed) {
     for (int j = 0; j < KC; ++j) {
         std::vector<uint32_t> em(nm), eb(nb ? nb : 1, 0);
         for (uint64_t i = 0; i < nm; ++i) em[i] = mi[i * KC + j];
-        if (or_apply_cells(n, mods[j], em.data(), eb.data(), W - 1, low + 1, 0, 1) != 0) return 2;
+        for (uint64_t i = 0; i < nb; ++i) eb[i] = bi[i * KC + j];
+        if (or_apply_cells(n, mods[j], em.data(), eb.data(), p_hi, p_lo, high ? 1 : 0, 1) != 0) return 2;
         for (uint64_t i = 0; i < nm; ++i)
             if (mo[i * KC + j] != em[i]) {
-                printf("MISMATCH main n=%d low=%d V=%d j=%d i=%llu got=%u want=%u\n", n, low, V, j,
+                printf("MISMATCH main n=%d low=%d high=%d V=%d j=%d i=%llu got=%u want=%u\n", n, low, high, V, j,
                        (unsigned long long)i, mo[i * KC + j], em[i]);
                 return 1;
             }
-        for (uint64_t i = 0; i < nb; ++i)
-            if (bo[i * KC + j] != eb[i]) {
-                printf("MISMATCH blocked n=%d low=%d V=%d j=%d i=%llu got=%u want=%u\n", n, low, V, j,
-                       (unsigned long long)i, bo[i * KC + j], eb[i]);
-                return 1;
-            }
+        if (!high)
+            for (uint64_t i = 0; i < nb; ++i)
+                if (bo[i * KC + j] != eb[i]) {
+                    printf("MISMATCH blocked n=%d low=%d V=%d j=%d i=%llu got=%u want=%u\n", n, low, V, j,
+                           (unsigned long long)i, bo[i * KC + j], eb[i]);
+                    return 1;
+                }
     }
-    printf("ok n=%d low=%d V=%d groups=%d cells=%d..%d\n", n, low, V, groups, W - 1, low + 1);
+    printf("ok n=%d low=%d %s V=%d groups=%d cells=%d..%d\n", n, low, high ? "high" : "low", V, groups, p_hi, p_lo);
     return 0;
 }
 
 // Words of `width` positions in the group, in lexicographic order (N < R < L
 // from the top, entered at height 1, ending at 0).
-static void group_words(int width, int low, uint32_t mask, const uint64_t (*f)[kDpCols], std::vector<uint64_t>& out) {
+static void group_words(int width, const cell::Occupancy& o, const uint64_t (*f)[kDpCols], std::vector<uint64_t>& out) {
     std::function<void(int, int, uint64_t)> rec = [&](int pos, int h, uint64_t word) {
         if (pos < 0) {
             if (h == 0) out.push_back(word);
             return;
         }
+        const int fix = o.at(pos);
         for (int v = 0; v < 3; ++v) {
-            if (pos < low && (v != 0) != (((mask >> pos) & 1u) != 0)) continue;
+            if (fix >= 0 && (v != 0) != (fix == 1)) continue;
             const int nh = h + (v == 2) - (v == 1);
             if (nh < 0 || nh >= kDpCols || f[pos][nh] == 0) continue;
             rec(pos - 1, nh, word | ((uint64_t)v << (2 * pos)));
@@ -180,16 +200,20 @@ static void group_words(int width, int low, uint32_t mask, const uint64_t (*f)[k
     rec(width - 1, 1, 0);
 }
 
-static int check_big(int n, int low, uint32_t g, const uint32_t* mods, unsigned seed) {
+static int check_big(int n, int low, bool high, uint32_t g, const uint32_t* mods, unsigned seed) {
     constexpr int KC = 3, K = 4;
     const int W = n + 1;
-    static uint64_t f[kDpRows][kDpCols];
+    const int p_hi = high ? low : W - 1, p_lo = high ? 1 : low + 1;
+    static uint64_t fm[kDpRows][kDpCols], fb[kDpRows][kDpCols];
     static cell::Quad<uint32_t> q[kDpRows][kDpCols];
-    cell::build_group_table(low, g, f);
+    cell::Occupancy om, ob;
+    cell::group_occupancy(high, W, low, g, om, ob);
+    cell::build_group_table(om, fm);
+    cell::build_group_table(ob, fb);
     std::vector<uint64_t> wm, wb;
-    group_words(W, low, g, f, wm);
-    group_words(W - 1, low, g, f, wb);
-    if (wm.size() != f[W][1] || wb.size() != f[W - 1][1]) {
+    group_words(W, om, fm, wm);
+    group_words(W - 1, ob, fb, wb);
+    if (wm.size() != fm[W][1] || wb.size() != fb[W - 1][1]) {
         printf("COUNT big n=%d low=%d group=%u\n", n, low, g);
         return 1;
     }
@@ -213,8 +237,11 @@ static int check_big(int n, int low, uint32_t g, const uint32_t* mods, unsigned 
     std::vector<uint32_t> mi(nm * K, 0), bi((nb ? nb : 1) * K, 0);
     for (uint64_t i = 0; i < nm; ++i)
         for (int j = 0; j < KC; ++j) mi[i * K + j] = (uint32_t)(rng() % mods[j]);
+    if (high)
+        for (uint64_t i = 0; i < nb; ++i)
+            for (int j = 0; j < KC; ++j) bi[i * K + j] = (uint32_t)(rng() % mods[j]);
     std::vector<uint32_t> em = mi, eb = bi;  // the oracle's run
-    cell::pack_group_quads<uint32_t>(low, g, f, q);
+    cell::pack_group_quads<uint32_t>(ob, fb, om, fm, q);
     cell::Tab<uint32_t> T{q};
     Checked<KC> mem{mi.data(), bi.data(), nm, nb, K, 0};
     CellLaunch a{};
@@ -223,7 +250,7 @@ static int check_big(int n, int low, uint32_t g, const uint32_t* mods, unsigned 
     a.kc = KC;
     a.layout = 1;
     for (int j = 0; j < KC; ++j) a.mod[j] = mods[j];
-    for (int p = W - 1; p >= low + 1; --p) {
+    for (int p = p_hi; p >= p_lo; --p) {
         a.p = p;
         a.read_blk = p == W - 1 ? 0 : 1;
         for (uint64_t b = 0; b < nb; ++b) cell::cell_one<KC, uint32_t, cell::kAoS, 2>(a, T, b, mem);
@@ -273,17 +300,18 @@ static int check_big(int n, int low, uint32_t g, const uint32_t* mods, unsigned 
                         return 1;
                     }
     }
-    printf("ok big n=%d low=%d group=%u main=%llu blocked=%llu cells=%d..%d\n", n, low, g,
-           (unsigned long long)nm, (unsigned long long)nb, W - 1, low + 1);
+    printf("ok big n=%d low=%d %s group=%u main=%llu blocked=%llu cells=%d..%d\n", n, low, high ? "high" : "low",
+           g, (unsigned long long)nm, (unsigned long long)nb, p_hi, p_lo);
     return 0;
 }
 
 int main(int argc, char** argv) {
     const uint32_t bmods[3] = {4294967291u, 2u, 4294966997u};
-    if (argc > 3 && !strcmp(argv[1], "big")) {
+    if (argc > 3 && (!strcmp(argv[1], "big") || !strcmp(argv[1], "bighigh"))) {
+        const bool high = !strcmp(argv[1], "bighigh");
         const int n = atoi(argv[2]), low = atoi(argv[3]);
         for (int i = 4; i < argc; ++i)
-            if (check_big(n, low, (uint32_t)strtoul(argv[i], nullptr, 10), bmods, 77u + i)) return 1;
+            if (check_big(n, low, high, (uint32_t)strtoul(argv[i], nullptr, 10), bmods, 77u + i)) return 1;
         printf("ALL OK\n");
         return 0;
     }
@@ -291,12 +319,14 @@ int main(int argc, char** argv) {
     const uint32_t mods[3] = {4294967291u, 2u, 4294966997u};
     for (int n = 2; n <= nmax; ++n) {
         const int W = n + 1;
-        for (int low = 1; low <= W - 2; ++low) {
-            if (check<0>(n, low, mods, 100u * n + low)) return 1;
-            if (check<2>(n, low, mods, 200u * n + low)) return 1;
-            if (n % 2 == 0 && (check<1>(n, low, mods, 300u * n + low) || check<3>(n, low, mods, 400u * n + low)))
-                return 1;
-        }
+        for (int low = 1; low <= W - 2; ++low)
+            for (int high = 0; high < 2; ++high) {
+                if (check<0>(n, low, high, mods, 100u * n + low + 50 * high)) return 1;
+                if (check<2>(n, low, high, mods, 200u * n + low + 50 * high)) return 1;
+                if (n % 2 == 0 && (check<1>(n, low, high, mods, 300u * n + low + 50 * high) ||
+                                   check<3>(n, low, high, mods, 400u * n + low + 50 * high)))
+                    return 1;
+            }
     }
     printf("ALL OK\n");
     return 0;

@@ -1,15 +1,16 @@
 """Group sweeps on the GPU through the C ABI: the high cells W-1..low+1 of a
-row run over occupancy groups of the low positions, each group stored alone
-with 32-bit ranks local to the group.  Checked bit-exact against the oracle's
-full-state cells, against the full-state GPU path, and, at the n = 27 LOW14
-shape, against the same groups swept in other windows."""
+row over occupancy groups of the low positions (LOW groups) and the low cells
+low..1 over occupancy groups of the high positions (HIGH groups), each group
+stored alone with 32-bit ranks local to the group.  Checked bit-exact against
+the oracle's full-state cells, against the full-state GPU path, and, at the
+n = 27 LOW14 / HIGH13 shape, against the same groups swept in other windows."""
 from __future__ import annotations
 
 import numpy as np
 import pytest
 
 import ofp
-from group_masks import occupancy_masks, window_order
+from group_masks import group_masks, occupancy_masks, window_order
 from ofp.path_bound import PRIMES
 
 pytestmark = pytest.mark.gpu
@@ -95,3 +96,70 @@ def test_n27_low14_windows_agree(gpu_ctx):
     s1, _, sb1 = gpu_ctx.group_sweep(n, mods, 100, 2, low, seed=5, blocked_out=True)
     s2, _, sb2 = gpu_ctx.group_sweep(n, mods, 100, 2, low, seed=5, blocked_out=True)
     assert np.array_equal(sb1, sb2) and s1["blocked_count"] == int(ofp.group_plan(n, 4, 100, 2, low)[2].sum())
+
+
+@pytest.mark.parametrize("n,low,k", [(9, 4, 3), (12, 9, 1), (12, 5, 3), (13, 2, 2), (13, 7, 8), (11, 6, 14)])
+def test_high_group_sweep_equals_oracle(gpu_ctx, oracle, n, low, k):
+    """The low cells low..1 over HIGH groups, with blocked input (the high
+    half's output), against the oracle's full-state cells."""
+    mods = ([2] + list(PRIMES))[:k] if k > 1 else [PRIMES[1]]
+    nm, nb = ofp.state_counts(n)
+    fixed = n - low
+    mm, mb = group_masks(n, low, True)
+    om, ob = window_order(mm, 0, 1 << fixed), window_order(mb, 0, 1 << fixed)
+    main = _inputs(nm, mods, 2000 * n + low)
+    blk = _inputs(nb, mods, 3000 * n + low)
+    print(f"case high n={n} low={low} k={k} all groups", flush=True)
+    info, mo, _ = gpu_ctx.group_sweep(n, mods, 0, 1 << fixed, low, main_in=main[om], high=True,
+                                      blocked_in=blk[ob])
+    assert info["cells"] == low and len(info["cell_ms"]) == low
+    for j, m in enumerate(mods):
+        em, eb = np.ascontiguousarray(main[:, j]), np.ascontiguousarray(blk[:, j])
+        oracle.apply_cells(n, m, em, eb, low, 1, True, pull=True)
+        assert np.array_equal(mo[:, j], em[om]), (j, m)
+
+
+def test_row_of_group_halves_equals_full_state_gpu(gpu_ctx):
+    """n = 16 split at 8, 2 moduli: a whole row as the high half over LOW
+    groups, a transpose to HIGH-group order on the host, and the low half over
+    HIGH groups equals the full-state GPU row (ofp_apply_cells, pair passes)."""
+    n, low, mods = 16, 8, [PRIMES[0], PRIMES[5]]
+    nm, nb = ofp.state_counts(n)
+    main = _inputs(nm, mods, 161)
+    lm, lb = group_masks(n, low, False)
+    hm, hb = group_masks(n, low, True)
+    olm, olb = window_order(lm, 0, 1 << low), window_order(lb, 0, 1 << low)
+    ohm, ohb = window_order(hm, 0, 1 << (n - low)), window_order(hb, 0, 1 << (n - low))
+    print("case n=16 high half over LOW groups", flush=True)
+    _, m1, b1 = gpu_ctx.group_sweep(n, mods, 0, 1 << low, low, main_in=main[olm], blocked_out=True)
+    gm, gb = np.zeros_like(main), np.zeros((nb, 2), np.uint32)
+    gm[olm], gb[olb] = m1, b1
+    print("case n=16 low half over HIGH groups", flush=True)
+    _, m2, _ = gpu_ctx.group_sweep(n, mods, 0, 1 << (n - low), low, main_in=gm[ohm], high=True,
+                                   blocked_in=gb[ohb])
+    got = np.zeros_like(main)
+    got[ohm] = m2
+    em, eb = main.copy(), np.zeros((nb, 2), np.uint32)
+    print("case n=16 full-state row", flush=True)
+    gpu_ctx.apply_cells(n, mods, em, eb, n, 1, blocked_valid=False)
+    assert np.array_equal(got, em)
+
+
+def test_n27_high13_windows_agree(gpu_ctx):
+    """W = 28, HIGH13, 4 moduli: groups 0..7 as one window and as two give the
+    same residues, with host inputs; seeded windows are deterministic."""
+    n, low, mods = 27, 14, list(PRIMES[:4])
+    info, mc, bc = ofp.group_plan(n, 4, 0, 8, low, high=True)
+    main = _inputs(info["main_count"], mods, 270)
+    blk = _inputs(info["blocked_count"], mods, 271)
+    print("case n=27 HIGH groups 0..7 one window", flush=True)
+    i1, whole, _ = gpu_ctx.group_sweep(n, mods, 0, 8, low, main_in=main, high=True, blocked_in=blk)
+    assert i1["cells"] == 14 and i1["kernel_s"] > 0
+    m0, b0 = int(mc[:3].sum()), int(bc[:3].sum())
+    print("case n=27 HIGH groups 0..2 and 3..7", flush=True)
+    _, a, _ = gpu_ctx.group_sweep(n, mods, 0, 3, low, main_in=main[:m0], high=True, blocked_in=blk[:b0])
+    _, b, _ = gpu_ctx.group_sweep(n, mods, 3, 5, low, main_in=main[m0:], high=True, blocked_in=blk[b0:])
+    assert np.array_equal(whole, np.concatenate([a, b])) and not np.array_equal(whole, main)
+    print("case n=27 HIGH seeded window", flush=True)
+    s1, _, _ = gpu_ctx.group_sweep(n, mods, 4000, 2, low, seed=9, high=True)
+    assert s1["launches"] == 2 * 14
