@@ -273,8 +273,8 @@ def kernel_rows(cell_times: list[list[tuple[int, float, str]]], kc: int, states:
                 traffic: dict | None) -> dict:
     """Per-kernel statistics over rows of time_cells() output: one entry per cell
     launched on its own (one launch when k <= 16; a cell of k > 16 moduli sums its
-    launches) and one per pair pass (cells p, p-1; its seven launches, one per
-    window height change, together).  Traffic per pair pass = the pair kernel's
+    launches) and one per pair pass (cells p, p-1; its five launches, one per
+    height change d = -2..2 over the two cells, together).  Traffic per pair pass = the pair kernel's
     DRAM bytes over one row / the passes in a row."""
     by: dict[str, list[float]] = {}
     cells: dict[str, set] = {}
@@ -296,7 +296,7 @@ def kernel_rows(cell_times: list[list[tuple[int, float, str]]], kc: int, states:
         t = (traffic or {}).get(f"{name}<{kc}>")
         if name == "pair_kernel":
             ent["cells"] = [f"{p},{p - 1}" for p in ent["cells"]]
-            ent["unit"] = "GB/s per pass (two cells, seven launches)"
+            ent["unit"] = "GB/s per pass (two cells, five launches)"
             if t and passes_per_row:
                 t = {"read": t["read_total"] / passes_per_row, "write": t["write_total"] / passes_per_row}
         if t:
@@ -474,8 +474,7 @@ def main() -> int:
     shard = None
     if not args.no_shard:
         from ofp import shard as shard_mod
-        with ofp.Context(device=0) as sctx:
-            autotune.apply(sctx, 22, 4)
+        with ofp.Context(device=0) as sctx:  # the library's defaults (no sorted cells at 4 moduli)
             shard = shard_mod.measure_share(sctx, share=0)
         log(f"bench: n=27 share 0 of 8, whole row streamed in {shard['high']['windows']}+{shard['low']['windows']} "
             f"windows: {shard['row_kernel_s']:.2f} s kernel ({shard['row_wall_s']:.1f} s wall); high cells "
@@ -522,7 +521,7 @@ def main() -> int:
         "batched_vs_plain_residues_per_hour": (residues_per_hour / plain["residues_per_hour"]) if plain else None,
         "roofline": roof,
         "roofline_kernels": list(kern.values()) + ([capped] if capped else []),
-        "counter_run": {kk: vv for kk, vv in (traffic or {}).items() if kk != "kernels"} if traffic else None,
+        "counter_run": traffic,
         "hbm_copy": copy,
         "cpu_baseline": cpu,
         "plain_small": small,

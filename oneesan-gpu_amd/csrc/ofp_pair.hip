@@ -1,12 +1,14 @@
-// ofp_pair.hip -- the pair pass: cells p and p-1 of a row in one out-of-place
-// launch per window height change d (body and design in ofp_pair_impl.h).
+// ofp_pair.hip -- the pair pass: cells p and p-1 of a row applied in place,
+// one launch per window height change d (body and design in ofp_pair_impl.h).
 //
 // Each thread takes one class (P', d, S'), decoded from its class number with
 // the class completion table of (p, d) and the height table, both in LDS; the
 // classes are dealt to the XCDs in pieces exactly as the cell kernels deal
-// their blocked words (cell::Sched).  Loads come from the old copy of the
-// state and stores go to the new one, so the classes need no ordering among
-// themselves and the launch has no atomics.
+// their blocked words (cell::Sched).  Every word belongs to one class and only
+// that class's thread writes it; the words a class reads from other classes
+// still hold their values from before the pass when it reads them (launch
+// order d = -1, 0, 1, then -2, 2), so the launches need no atomics and no
+// second copy of the state.
 #include <cstring>
 
 #include "ofp_pair_impl.h"

@@ -16,8 +16,10 @@
 // reads before cell p ([P', x, y, S'], x = position p-1) and writes after cell
 // p-1 ([P', a, x', S'], a = position p, x' = position p-2) are the same index
 // set, so each class owns its blocked words too.  One thread takes one class:
-// it loads the class's words, applies both cells in registers and writes the
-// whole class back (unchanged words included).
+// it loads the class's words, applies both cells in registers and stores the
+// words the two cells change (words they keep, such as [P', R, R, N, S'], are
+// not stored; classes with d = +-3, [R, R, R] and [L, L, L] with their blocked
+// [R, R] and [L, L], change nothing and get no launch).
 //
 // Closure sources of cell p are needed before cell p, those of cell p-1 after
 // cell p; sources in other classes that cell p changes are rebuilt from their

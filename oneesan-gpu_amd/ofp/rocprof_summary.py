@@ -18,13 +18,16 @@ mean counter value per kernel (KiB per dispatch for FETCH_SIZE / WRITE_SIZE)
 and, with --scale, the calibrated bytes (the scale is what a device-to-device
 copy of known size reads for the same counter).
 
-With --cells C (cells per row, W-1) the cell launches of each moduli count KC
-(full-state `cell_kernel` / `cell_sorted_kernel` and height-capped
-`cell_capped_kernel` together, in dispatch order, one launch per cell when
-KC <= 16) are cut into rows of C launches (--rows R keeps the first R): `trace` lists each row's time and how many of its cells ran capped,
-then the per-cell times (p = W-1 .. 1) of the median full-state row; `counter`
-gives the mean over full-state rows' dispatches and the per-cell values of the
-last full-state row.
+With --cells C (cells per row, W-1) the launches of each moduli count KC are
+cut into rows of C cells (--rows R keeps the first R), in dispatch order, one
+launch per cell when KC <= 16: a full-state `cell_kernel` / `cell_sorted_kernel`
+or height-capped `cell_capped_kernel` launch is one cell, and the consecutive
+`pair_kernel` launches of one pair pass (one per height change d, in the order
+-1, 0, 1, -2, 2; a pass that has no classes of some d skips it) are two cells.
+`trace` lists each row's time and how many of its cells ran capped, then the
+per-step times (p = W-1 .. 1; a pair pass as "p,p-1") of the median full-state
+row; `counter` gives the mean over full-state rows' dispatches and the per-cell
+values of the last full-state row (single-cell launches only).
 
 `valu` reads one run collected with
   --pmc SQ_ACTIVE_INST_VALU SQ_THREAD_CYCLES_VALU SQ_INSTS_VALU SQ_INSTS_SALU
@@ -53,8 +56,49 @@ def _short(name: str) -> str:
 
 
 def _kc(name: str) -> int | None:
-    m = re.search(r"cell_(?:capped_|sorted_)?kernel<(\d+)", name)
+    m = re.search(r"(?:cell_(?:capped_|sorted_)?|pair_)kernel<(\d+)", name)
     return int(m.group(1)) if m else None
+
+
+_PAIR_ORDER = {-1: 0, 0: 1, 1: 2, -2: 3, 2: 4, -3: 5, 3: 6}
+
+
+def _pair_d(name: str) -> int | None:
+    """Height change d of a pair_kernel launch (its third template argument)."""
+    m = re.search(r"pair_kernel<\d+,\s*[^,]+,\s*(-?\d+)", name)
+    return int(m.group(1)) if m else None
+
+
+def _steps(seq):
+    """seq: [(t0, ms, capped, d)] (d None for a one-cell launch) -> steps
+    [(t0, ms, capped, ncells)]: a one-cell launch is a step of one cell, the
+    consecutive pair launches of one pass a step of two."""
+    out = []
+    last = None  # order index of the previous pair launch of the current pass
+    for t0, ms, capped, d in sorted(seq):
+        if d is None:
+            out.append([t0, ms, capped, 1])
+            last = None
+            continue
+        o = _PAIR_ORDER.get(d, 0)
+        if last is not None and o > last:
+            out[-1][1] += ms
+        else:
+            out.append([t0, ms, capped, 2])
+        last = o
+    return [tuple(x) for x in out]
+
+
+def _step_rows(seq, cells):
+    """Steps cut into rows of `cells` cells."""
+    rows, cur, n = [], [], 0
+    for st in _steps(seq):
+        cur.append(st)
+        n += st[3]
+        if n >= cells:
+            rows.append(cur)
+            cur, n = [], 0
+    return rows[:MAX_ROWS] if MAX_ROWS else rows
 
 
 def kernel_family(name: str) -> str | None:
@@ -98,14 +142,14 @@ def trace(path: str, cells: int | None) -> None:
         by[name].append((t1 - t0) / 1e6)
         kc = _kc(name)
         if kc is not None:
-            fam[kc].append((t0, (t1 - t0) / 1e6, "capped" in name))
+            fam[kc].append((t0, (t1 - t0) / 1e6, "capped" in name, _pair_d(name)))
     print(f"{'kernel':48s} {'calls':>6s} {'mean_ms':>10s} {'min_ms':>9s} {'max_ms':>9s}")
     for k, v in sorted(by.items(), key=lambda kv: -sum(kv[1])):
         print(f"{k[:48]:48s} {len(v):6d} {sum(v) / len(v):10.3f} {min(v):9.3f} {max(v):9.3f}")
     if not cells:
         return
     for kc, seq in sorted(fam.items(), reverse=True):
-        rws = _rows(seq, cells)
+        rws = _step_rows(seq, cells)
         if not rws:
             continue
         print(f"KC={kc}: {len(rws)} rows of {cells} cells; row ms (capped cells):")
@@ -114,10 +158,12 @@ def trace(path: str, cells: int | None) -> None:
         if full:
             full.sort(key=lambda r: sum(x[1] for x in r))
             med = full[len(full) // 2]
-            print(f"  median full-state row {sum(x[1] for x in med):.1f} ms, per-cell ms p={cells}..1: "
-                  + " ".join(f"{x[1]:.1f}" for x in med))
-            print(f"  mean full-state launch {statistics.fmean(x[1] for r in full for x in r):.3f} ms "
-                  f"over {sum(len(r) for r in full)} launches")
+            parts, p = [], cells
+            for x in med:
+                parts.append(f"{p},{p - 1}:{x[1]:.1f}" if x[3] == 2 else f"{p}:{x[1]:.1f}")
+                p -= x[3]
+            print(f"  median full-state row {sum(x[1] for x in med):.1f} ms over {len(full)} full-state rows; "
+                  f"per step (p: ms): " + " ".join(parts))
 
 
 def counter(path: str, scale: float, cells: int | None) -> None:

@@ -104,14 +104,17 @@ def share_windows(n: int, k: int, shares: int, share: int, window_bytes: float, 
 
 
 def measure_share(ctx: _lib.Context, n: int = 27, k: int = 4, shares: int = 8, share: int = 0,
-                  window_bytes: float = 136e9, seed: int = 20261202, log=None) -> dict:
-    """Stream every window of one share through both halves of a row."""
+                  window_bytes: float = 136e9, seed: int = 20261202, log=None, max_windows: int = 0) -> dict:
+    """Stream every window of one share through both halves of a row
+    (max_windows > 0: only the first windows of each half, for profiling runs)."""
     mods = PRIMES[:k]
     W = n + 1
     t0 = time.perf_counter()
     halves = {}
     for name, high in (("high", False), ("low", True)):
         low, windows = share_windows(n, k, shares, share, window_bytes, high=high)
+        if max_windows > 0:
+            windows = windows[:max_windows]
         cells = low if high else W - 1 - low
         h = {"placement": (f"HIGH{W - 1 - low} groups (occupancy of positions {low + 1}..{W - 1})" if high
                            else f"LOW{low} groups (occupancy of positions 0..{low - 1})"),
@@ -162,11 +165,20 @@ def main(argv=None) -> int:
     ap.add_argument("--slice", type=int, default=None)
     ap.add_argument("--share", type=int, default=None, help="stream a whole row of this share")
     ap.add_argument("--shares", type=int, default=8)
+    ap.add_argument("--max-windows", type=int, default=0, help="only the first windows of each half (profiling)")
+    ap.add_argument("--window", type=int, nargs=3, metavar=("HIGH", "FIRST", "GROUPS"), default=None,
+                    help="one window only: half (0: high cells / LOW groups, 1: low cells / HIGH groups), "
+                         "first group and group count (profiling)")
     a = ap.parse_args(argv)
     with _lib.Context() as ctx:
+        if a.window is not None:
+            high, g0, g = a.window
+            info, _, _ = ctx.group_sweep(a.n, PRIMES[:a.k], g0, g, 0, seed=20261202, high=bool(high))
+            print(json.dumps(info), flush=True)
+            return 0
         if a.share is not None:
             rep = measure_share(ctx, a.n, a.k, a.shares, a.share,
-                                log=lambda m: print(m, file=sys.stderr, flush=True))
+                                log=lambda m: print(m, file=sys.stderr, flush=True), max_windows=a.max_windows)
             stray = 0
         else:
             rep = measure(ctx, a.n, a.k, a.slices, a.slice)

@@ -30,7 +30,26 @@ def test_trace_rows_and_full_state_mean(tmp_path, capsys):
     out = capsys.readouterr().out
     assert "KC=14: 3 rows of 3 cells" in out
     assert "3.0(3) 33.0(0) 60.0(0)" in out
-    assert "mean full-state launch 15.500 ms over 6 launches" in out  # (10+11+12+3x20)/6
+    assert "median full-state row 60.0 ms over 2 full-state rows; per step (p: ms): 3:20.0 2:20.0 1:20.0" in out
+
+
+PAIR = "void ofp::(anonymous namespace)::pair_kernel<14, unsigned int, {d}, 0, 2>(ofp::pair::PairLaunch)"
+
+
+def test_trace_rows_with_pair_passes(tmp_path, capsys):
+    """A pair pass (its launches d = -1, 0, 1, -2, 2; here one is skipped) counts as
+    two cells; consecutive passes are told apart by the launch order."""
+    cells = 5
+    row = [(PAIR.format(d=d), ms) for d, ms in ((-1, 1.0), (0, 2.0), (1, 1.0), (-2, 0.5), (2, 0.5))]
+    row += [(PAIR.format(d=d), ms) for d, ms in ((-1, 1.5), (0, 2.5), (1, 1.0), (2, 1.0))]
+    row += [(CELL, 7.0)]
+    p = tmp_path / "t.csv"
+    _trace(p, [(CAPPED, 1.0)] * cells + row + row)
+    rs.trace(str(p), cells)
+    out = capsys.readouterr().out
+    assert "KC=14: 3 rows of 5 cells" in out
+    assert "5.0(5) 18.0(0) 18.0(0)" in out
+    assert "per step (p: ms): 5,4:5.0 3,2:6.0 1:7.0" in out
 
 
 def test_counter_calibration(tmp_path, capsys):
