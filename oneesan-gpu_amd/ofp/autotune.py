@@ -61,16 +61,20 @@ def save_cache(data: dict, path: Path = DEFAULT_CACHE) -> None:
 
 
 def lookup(n: int, k: int, path: Path = DEFAULT_CACHE) -> dict | None:
-    """Best known (threads, blocks_per_cu, layout) for (n, k): user cache first,
-    then the packaged table (exact key, else the nearest n with the same k>1-ness)."""
+    """Best known launch settings for (n, k): user cache first, then the packaged
+    table (exact key, else the entry of the same kind -- one modulus or several --
+    nearest in n, then in k)."""
     ent = load_cache(path).get(_key(n, k))
     if ent:
         return ent
     pk = load_cache(PACKAGED)
     if _key(n, k) in pk:
         return pk[_key(n, k)]
-    cands = [(abs(int(key.split(",")[0][2:]) - n), v) for key, v in pk.items()
-             if (int(key.split(",")[1][2:]) > 1) == (k > 1)]
+    cands = []
+    for key, v in pk.items():
+        kn, kk = int(key.split(",")[0][2:]), int(key.split(",")[1][2:])
+        if (kk > 1) == (k > 1):
+            cands.append(((abs(kn - n), abs(kk - k), kn, kk), v))
     return min(cands, key=lambda t: t[0])[1] if cands else None
 
 

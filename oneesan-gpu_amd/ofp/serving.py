@@ -12,6 +12,9 @@ its own interleaved state) and
   * batches at admission: all pending requests with the same n are merged into
     ONE job whose moduli are the union of theirs (duplicates shared), up to 16
     moduli (one cell launch), so their residues ride in one interleaved sweep;
+  * launches each job with the tuned settings of its (n, k) (ofp.autotune's
+    table; they are read when the rows are enqueued, so jobs of different
+    sizes in flight each keep their own);
   * keeps jobs overlapping on the GPU: each job's W row sweeps are enqueued on
     its own stream as soon as it is admitted, the host only polls and retires;
   * retires a finished job: reads its residues, answers every request it
@@ -28,7 +31,7 @@ import statistics
 import time
 from dataclasses import dataclass, field
 
-from . import _lib
+from . import _lib, autotune
 from .path_bound import PRIMES
 
 MAX_MODULI_PER_JOB = 16
@@ -87,6 +90,7 @@ class Server:
     budget_bytes: int
     max_jobs: int = 8
     merge: bool = True
+    tuned: bool = True
     pending: list[Request] = field(default_factory=list)
     active: list[_Active] = field(default_factory=list)
     used_bytes: int = 0
@@ -115,6 +119,8 @@ class Server:
                 raise RuntimeError(f"request n={head.n} k={len(mods)} needs {need} bytes > budget")
             if self.used_bytes + need > self.budget_bytes:
                 return  # wait for memory: admission is FIFO on the head request
+            if self.tuned:
+                autotune.apply(self.ctx, head.n, len(mods))
             job = _lib.Job(self.ctx, head.n, mods)
             job.enqueue(head.n + 1)  # all W rows, asynchronously on the job's stream
             for r in group:
@@ -205,15 +211,16 @@ def main(argv=None) -> int:
     ap.add_argument("--k-hi", type=int, default=8)
     ap.add_argument("--max-jobs", type=int, default=8)
     ap.add_argument("--no-merge", action="store_true")
+    ap.add_argument("--untuned", action="store_true", help="run every job with the library's default settings")
     ap.add_argument("--check", action="store_true",
                     help="re-solve every request alone afterwards and require identical residues")
     a = ap.parse_args(argv)
     reqs = poisson_stream(a.requests, a.rate, a.seed, a.n_lo, a.n_hi, 1, a.k_hi)
     with _lib.Context() as ctx:
-        srv = Server(ctx, default_budget(ctx), max_jobs=a.max_jobs, merge=not a.no_merge)
+        srv = Server(ctx, default_budget(ctx), max_jobs=a.max_jobs, merge=not a.no_merge, tuned=not a.untuned)
         rep = srv.run(reqs, log=lambda m: print(m, file=sys.stderr, flush=True))
         rep["config"] = {"rate": a.rate, "seed": a.seed, "n": [a.n_lo, a.n_hi], "k": [1, a.k_hi],
-                         "max_jobs": a.max_jobs, "merge": not a.no_merge}
+                         "max_jobs": a.max_jobs, "merge": not a.no_merge, "tuned": not a.untuned}
         if a.check:
             bad = 0
             for r in reqs:

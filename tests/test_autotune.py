@@ -31,19 +31,34 @@ def test_lookup_prefers_user_cache_then_packaged(tmp_path):
 
 
 def test_lookup_falls_back_to_nearest_n_same_kind(tmp_path):
-    """With no exact entry: the packaged entry of the nearest n whose moduli count is
-    of the same kind (one modulus, or several)."""
+    """With no exact entry: the packaged entry whose moduli count is of the same
+    kind (one modulus, or several), nearest in n, then in k."""
     from ofp import autotune
     path = tmp_path / "none.json"
     packaged = autotune.load_cache(autotune.PACKAGED)
+    keys = [(int(key.split(",")[0][2:]), int(key.split(",")[1][2:])) for key in packaged]
 
     def nearest(n, k):
-        cands = [(abs(int(key.split(",")[0][2:]) - n), key) for key in packaged
-                 if (int(key.split(",")[1][2:]) > 1) == (k > 1)]
-        return packaged[min(cands)[1]]
-    for n, k in ((3, 1), (16, 1), (16, 7), (27, 4), (21, 2)):
+        c = min((abs(kn - n), abs(kk - k), kn, kk) for kn, kk in keys if (kk > 1) == (k > 1))
+        return packaged[f"n={c[2]},k={c[3]}"]
+    for n, k in ((3, 1), (16, 1), (16, 7), (27, 4), (21, 2), (13, 3), (23, 12)):
         want = packaged.get(f"n={n},k={k}") or nearest(n, k)
         assert autotune.lookup(n, k, path) == want, (n, k)
+
+
+def test_lookup_ties_in_n_go_to_nearest_k(tmp_path):
+    from ofp import autotune
+    path = tmp_path / "none.json"
+    pk = tmp_path / "packaged.json"
+    autotune.save_cache({"n=20,k=2": {"threads": 1}, "n=20,k=8": {"threads": 2}, "n=20,k=1": {"threads": 3}}, pk)
+    old = autotune.PACKAGED
+    try:
+        autotune.PACKAGED = pk
+        assert autotune.lookup(20, 7, path)["threads"] == 2
+        assert autotune.lookup(21, 3, path)["threads"] == 1
+        assert autotune.lookup(19, 1, path)["threads"] == 3
+    finally:
+        autotune.PACKAGED = old
 
 
 @pytest.mark.gpu

@@ -26,12 +26,16 @@ def _golden(n, p):
 
 @pytest.mark.parametrize("merge,max_jobs,budget_gib", [(True, 4, 64), (False, 6, 64), (True, 8, 0.02)])
 def test_served_equals_solo(gpu_ctx, merge, max_jobs, budget_gib):
+    """Served with the tuned settings of each job's (n, k), in a context of its own;
+    the solo solves run with the library's defaults."""
+    import ofp
     from ofp import serving
     reqs = serving.poisson_stream(16, rate=200.0, seed=7 + max_jobs, n_lo=8, n_hi=14, k_lo=1, k_hi=5)
     budget = int(budget_gib * 2**30)
-    srv = serving.Server(gpu_ctx, budget, max_jobs=max_jobs, merge=merge)
     print(f"case merge={merge} max_jobs={max_jobs} budget={budget}", flush=True)
-    rep = srv.run(reqs)
+    with ofp.Context() as sctx:
+        srv = serving.Server(sctx, budget, max_jobs=max_jobs, merge=merge)
+        rep = srv.run(reqs)
     assert rep["requests"] == 16
     assert srv.peak_bytes <= budget
     if merge:

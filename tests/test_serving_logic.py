@@ -43,7 +43,7 @@ def test_admission_merge_and_budget(monkeypatch):
         (10, PRIMES[:3]), (11, PRIMES[2:4]), (10, PRIMES[1:5]), (10, PRIMES[:1]), (12, PRIMES[:8]),
         (10, PRIMES[10:16]), (10, PRIMES[:16])])]
     budget = serving.state_bytes(12, 8) + serving.state_bytes(10, 16) + 1
-    srv = serving.Server(ctx=None, budget_bytes=budget, max_jobs=2, merge=True)
+    srv = serving.Server(ctx=None, budget_bytes=budget, max_jobs=2, merge=True, tuned=False)
     rep = srv.run(reqs, poll_s=0.0)
     assert rep["requests"] == len(reqs)
     assert srv.peak_bytes <= budget and FakeJob.peak <= 2
