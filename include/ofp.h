@@ -347,6 +347,40 @@ ofp_status ofp_group_plan(ofp_ctx* ctx, const ofp_group_spec* spec, ofp_group_in
 ofp_status ofp_group_sweep(ofp_ctx* ctx, const ofp_group_spec* spec, ofp_group_info* info,
                            uint32_t* main_io, uint32_t* blocked_io);
 
+/* ---- exchange: a LOW window's words to the shares of their HIGH groups ----
+ * Between the two halves of a row every word moves from its LOW group to its
+ * HIGH group (main words: occupancy of main positions low+1..W-1; blocked
+ * words: of blocked positions low..W-2).  With the HIGH groups owned by
+ * `shares` GPUs (runs of consecutive masks, cut at s/shares of the main +
+ * blocked words, s = 1..shares-1), this writes every word of the LOW window
+ * that the last ofp_group_sweep of this context left on the device (spec must
+ * name that window, high = 0) into a staging bucket per destination share, as
+ * a record (index of the word in that share's HIGH placement -- the window
+ * layout ofp_group_sweep takes for the share's masks -- and its residues).
+ * Records keep the window order inside a bucket; bucket s follows buckets
+ * 0..s-1.  On one GPU the buckets of the other shares stand for the send
+ * buffers of the per-row all-to-all.  Needs fewer than 17 fixed HIGH
+ * positions.  Reference analogue: the per-row 8x8 bucket transpose between the
+ * H-major and L-major placements (docs/b300-bucket-snake.md:7-20), whose
+ * owner pairs B[a, b] are this build's (HIGH share, LOW share) pairs. */
+#define OFP_MAX_SHARES 16
+typedef struct ofp_exchange_info {
+    uint64_t main_count, blocked_count;           /* words of the window */
+    int shares;
+    uint32_t share_first[OFP_MAX_SHARES + 1];     /* share s: HIGH masks share_first[s]..share_first[s+1]-1 */
+    uint64_t main_records[OFP_MAX_SHARES];        /* records per destination share: main words */
+    uint64_t blocked_records[OFP_MAX_SHARES];     /*                                 blocked words */
+    uint64_t staging_bytes;                       /* device bytes of the staging buckets */
+    uint64_t moved_bytes;                         /* residues read + records written */
+    double kernel_s;                              /* count, scan and write launches, HIP events */
+} ofp_exchange_info;
+/* main_index / blocked_index (main_count / blocked_count entries) and
+ * main_res / blocked_res (x k residues): optional host copies of the buckets,
+ * all buckets in order. */
+ofp_status ofp_group_exchange(ofp_ctx* ctx, const ofp_group_spec* spec, int shares, ofp_exchange_info* info,
+                              uint64_t* main_index, uint32_t* main_res, uint64_t* blocked_index,
+                              uint32_t* blocked_res);
+
 /* Counts only, no device needed: main / blocked word counts for n. */
 ofp_status ofp_state_counts(int n, uint64_t* main_states, uint64_t* blocked_states);
 

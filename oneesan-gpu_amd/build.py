@@ -34,7 +34,7 @@ HIP_FLAGS = [
     f"-I{INCLUDE}",
 ]
 
-LIB_SOURCES = ["ofp_cell.hip", "ofp_pair.hip", "ofp_api.hip"]
+LIB_SOURCES = ["ofp_cell.hip", "ofp_pair.hip", "ofp_exchange.hip", "ofp_api.hip"]
 
 
 def _run(cmd: list[str]) -> None:

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "../../include/ofp.h"
+#include "ofp_exchange_impl.h"
 #include "ofp_internal.h"
 #include "ofp_pair_impl.h"
 
@@ -79,6 +80,18 @@ struct ofp_ctx {
     int max_batch = 0;     // moduli per sweep of ofp_solve (0: as many as fit HBM)
     char* win = nullptr;   // window buffer of shard and group sweeps, reused from call to call
     size_t win_bytes = 0;
+    // The LOW window the last group sweep left in `win` (what ofp_group_exchange reads).
+    struct LowWindow {
+        bool valid = false;
+        int n = 0, k = 0, low = 0;
+        uint32_t first = 0, groups = 0;
+        uint32_t* main = nullptr;
+        uint32_t* blk = nullptr;
+        char* tables = nullptr;
+        std::vector<uint64_t> mc, bc;  // words per group
+    } last_low;
+    char* stage = nullptr;  // exchange buffer: plan tables and staging buckets, reused
+    size_t stage_bytes = 0;
     void* scratch = nullptr;  // device address of the static state array on this device
     Sweep fixed;           // ofp_solve_static's state, inside that array
     std::string err;
@@ -231,6 +244,10 @@ void release_window(ofp_ctx* c) {
     if (c->win) (void)hipFree(c->win);
     c->win = nullptr;
     c->win_bytes = 0;
+    if (c->stage) (void)hipFree(c->stage);
+    c->stage = nullptr;
+    c->stage_bytes = 0;
+    c->last_low.valid = false;
 }
 
 // Carve consecutive 256-byte aligned pieces out of a window buffer.
@@ -1019,7 +1036,7 @@ ofp_status ofp_solve(ofp_ctx* c, int n, const uint32_t* moduli, int k, uint32_t*
     size_t fr = 0, tot = 0;
     OFP_HIP(c, hipSetDevice(c->device), "hipSetDevice");
     OFP_HIP(c, hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
-    const uint64_t avail = fr + c->st.main_cap + c->st.blk_cap + c->win_bytes;
+    const uint64_t avail = fr + c->st.main_cap + c->st.blk_cap + c->win_bytes + c->stage_bytes;
     int g = c->max_batch > 0 ? std::min(k, c->max_batch) : k;
     while (g > 1 && state_bytes(c->layout, n, g) + c->reserve > avail) --g;
     if (state_bytes(c->layout, n, g) + c->reserve > avail)
@@ -1187,6 +1204,7 @@ ofp_status ofp_shard_sweep(ofp_ctx* c, const ofp_shard_spec* sp, ofp_shard_info*
     OFP_HIP(c, hipSetDevice(c->device), "hipSetDevice");
     const size_t mbytes = std::max<uint64_t>(M, 1) * Kp * 4, bbytes = std::max<uint64_t>(B, 1) * Kp * 4;
     char* wbuf = nullptr;
+    c->last_low.valid = false;
     st = window_buffer(c, (mbytes + 255) / 256 * 256 + (bbytes + 255) / 256 * 256 + 512, &wbuf);
     if (st) return st;
     Carve cv{wbuf};
@@ -1339,6 +1357,7 @@ ofp_status ofp_group_sweep(ofp_ctx* c, const ofp_group_spec* sp, ofp_group_info*
     const size_t mbytes = std::max<uint64_t>(M, 1) * Kp * 4, bbytes = std::max<uint64_t>(B, 1) * Kp * 4;
     const size_t tbytes = (size_t)G * kGroupTableBytes;
     char* wbuf = nullptr;
+    c->last_low.valid = false;
     st = window_buffer(c, (mbytes + 255) / 256 * 256 + (bbytes + 255) / 256 * 256 + tbytes + 256, &wbuf);
     if (st) return st;
     Carve cv{wbuf};
@@ -1427,6 +1446,20 @@ ofp_status ofp_group_sweep(ofp_ctx* c, const ofp_group_spec* sp, ofp_group_info*
     }
     info->kernel_s = total * 1e-3;
     info->launches = launches;
+    if (!sp->high) {
+        ofp_ctx::LowWindow& lw = c->last_low;
+        lw.n = sp->n;
+        lw.k = k;
+        lw.low = L;
+        lw.first = sp->first_group;
+        lw.groups = G;
+        lw.main = dm;
+        lw.blk = db;
+        lw.tables = dtab;
+        lw.mc = mc;
+        lw.bc = bc;
+        lw.valid = true;
+    }
     const bool blk_out = !sp->high && blk_io;
     if (main_io || blk_out) {
         std::vector<uint32_t> t(std::max<uint64_t>(std::max(M, B), 1) * Kp);
@@ -1443,6 +1476,221 @@ ofp_status ofp_group_sweep(ofp_ctx* c, const ofp_group_spec* sp, ofp_group_info*
     }
 #undef OFP_HIP_R
     release();
+    return OFP_OK;
+}
+
+// ---- exchange: the words of a LOW window to their HIGH groups' shares ----
+
+namespace {
+
+// The exchange buffer: grown on demand, kept from call to call like the window.
+ofp_status stage_buffer(ofp_ctx* c, size_t bytes, char** out) {
+    bytes = (bytes + 255) / 256 * 256;
+    if (c->stage && c->stage_bytes >= bytes) {
+        *out = c->stage;
+        return OFP_OK;
+    }
+    if (c->stage) {
+        OFP_HIP(c, hipFree(c->stage), "free exchange buffer");
+        c->stage = nullptr;
+        c->stage_bytes = 0;
+    }
+    size_t fr = 0, tot = 0;
+    OFP_HIP(c, hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
+    if (bytes + c->reserve > fr) return fail(c, OFP_E_MEMORY, "exchange staging does not fit HBM beside the window");
+    OFP_HIP(c, hipMalloc(&c->stage, bytes), "exchange buffer");
+    c->stage_bytes = bytes;
+    *out = c->stage;
+    return OFP_OK;
+}
+
+// Tiles of one kind of word of a window: every group cut into runs of at most
+// kXTileWords words.
+void window_tiles(const std::vector<uint64_t>& counts, std::vector<XTile>& t) {
+    t.clear();
+    uint64_t word0 = 0;
+    for (uint32_t g = 0; g < counts.size(); ++g) {
+        for (uint64_t r = 0; r < counts[g]; r += kXTileWords) {
+            XTile x{};
+            x.word0 = word0 + r;
+            x.group = g;
+            x.rank0 = (uint32_t)r;
+            x.len = (uint32_t)std::min<uint64_t>(kXTileWords, counts[g] - r);
+            t.push_back(x);
+        }
+        word0 += counts[g];
+    }
+}
+
+}  // namespace
+
+ofp_status ofp_group_exchange(ofp_ctx* c, const ofp_group_spec* sp, int shares, ofp_exchange_info* info,
+                              uint64_t* main_index, uint32_t* main_res, uint64_t* blocked_index,
+                              uint32_t* blocked_res) {
+    if (!c || !sp || !info) return fail(c, OFP_E_ARG, "ctx, spec and info are required");
+    if (shares < 1 || shares > kMaxShares) return fail(c, OFP_E_ARG, "shares must be in 1..16");
+    int L = 0;
+    ofp_status st = group_check(c, sp, &L);
+    if (st) return st;
+    if (sp->high) return fail(c, OFP_E_ARG, "the exchange takes a LOW window (high = 0)");
+    const ofp_ctx::LowWindow& lw = c->last_low;
+    if (!lw.valid || lw.n != sp->n || lw.k != sp->k || lw.low != L || lw.first != sp->first_group ||
+        lw.groups != sp->groups)
+        return fail(c, OFP_E_ARG, "no such LOW window in the context: sweep it with ofp_group_sweep first");
+    const int W = sp->n + 1, k = sp->k, Kp = interleave_stride(1, k), nf = W - 1 - L;
+    if (nf > 16) return fail(c, OFP_E_ARG, "the exchange takes at most 16 fixed HIGH positions");
+    const uint32_t NM = 1u << nf;
+    // HIGH groups: word counts, the shares (consecutive masks, cut by main +
+    // blocked words at s/shares of the total: the split ofp.shard uses) and
+    // each group's offset in its share's placement.
+    std::vector<uint64_t> hm(NM), hb(NM);
+    for (uint32_t m = 0; m < NM; ++m) {
+        group_counts(true, W, L, m, &hm[m], &hb[m]);
+        if ((hm[m] >> 32) || (hb[m] >> 32))
+            return fail(c, OFP_E_ARG, "a HIGH group of the split holds 2^32 or more words: use a lower split");
+    }
+    std::vector<double> cum(NM + 1, 0.0);
+    for (uint32_t m = 0; m < NM; ++m) cum[m + 1] = cum[m] + (double)hm[m] + (double)hb[m];
+    uint32_t first[kMaxShares + 1] = {};
+    for (int s = 0; s < shares; ++s) {
+        const double x = cum[NM] * s / shares;
+        first[s] = (uint32_t)(std::lower_bound(cum.begin(), cum.end(), x) - cum.begin());
+    }
+    first[shares] = NM;
+    std::vector<uint64_t> base_m(NM), base_b(NM);
+    for (int s = 0; s < shares; ++s) {
+        uint64_t rm = 0, rb = 0;
+        for (uint32_t m = first[s]; m < first[s + 1]; ++m) {
+            base_m[m] = rm;
+            base_b[m] = rb;
+            rm += hm[m];
+            rb += hb[m];
+        }
+    }
+    uint64_t d[kDpRows][kDpCols];
+    build_height_table(d);
+    const size_t rows = (size_t)(NM - 1) * kDpCols;
+    std::vector<uint32_t> ht_m(std::max<size_t>(rows, 1)), ht_b(std::max<size_t>(rows, 1));
+    xchg::build_high_rows(d, L + 1, nf, ht_m.data());
+    xchg::build_high_rows(d, L, nf, ht_b.data());
+    std::vector<XTile> tm, tb;
+    window_tiles(lw.mc, tm);
+    window_tiles(lw.bc, tb);
+    uint64_t M = 0, B = 0;
+    for (uint64_t x : lw.mc) M += x;
+    for (uint64_t x : lw.bc) B += x;
+
+    OFP_HIP(c, hipSetDevice(c->device), "hipSetDevice");
+    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t b_ht = al(ht_m.size() * 4), b_base = al((size_t)NM * 8);
+    const size_t b_tm = al(std::max<size_t>(tm.size(), 1) * sizeof(XTile)),
+                 b_tb = al(std::max<size_t>(tb.size(), 1) * sizeof(XTile));
+    const size_t b_cm = al(std::max<size_t>(tm.size(), 1) * shares * 4),
+                 b_cb = al(std::max<size_t>(tb.size(), 1) * shares * 4);
+    const size_t b_om = al(std::max<size_t>(tm.size(), 1) * shares * 8),
+                 b_ob = al(std::max<size_t>(tb.size(), 1) * shares * 8);
+    const size_t b_tot = al(2 * kMaxShares * 8);
+    const size_t b_im = al(std::max<uint64_t>(M, 1) * 8), b_ib = al(std::max<uint64_t>(B, 1) * 8);
+    const size_t b_rm = al(std::max<uint64_t>(M, 1) * Kp * 4), b_rb = al(std::max<uint64_t>(B, 1) * Kp * 4);
+    const size_t need = 2 * b_ht + 2 * b_base + b_tm + b_tb + b_cm + b_cb + b_om + b_ob + b_tot + b_im + b_ib +
+                        b_rm + b_rb;
+    char* buf = nullptr;
+    st = stage_buffer(c, need, &buf);
+    if (st) return st;
+    Carve cv{buf};
+    uint32_t* d_htm = cv.take<uint32_t>(b_ht);
+    uint32_t* d_htb = cv.take<uint32_t>(b_ht);
+    uint64_t* d_bm = cv.take<uint64_t>(b_base);
+    uint64_t* d_bb = cv.take<uint64_t>(b_base);
+    XTile* d_tm = cv.take<XTile>(b_tm);
+    XTile* d_tb = cv.take<XTile>(b_tb);
+    uint32_t* d_cm = cv.take<uint32_t>(b_cm);
+    uint32_t* d_cb = cv.take<uint32_t>(b_cb);
+    uint64_t* d_om = cv.take<uint64_t>(b_om);
+    uint64_t* d_ob = cv.take<uint64_t>(b_ob);
+    uint64_t* d_tot = cv.take<uint64_t>(b_tot);
+    uint64_t* d_im = cv.take<uint64_t>(b_im);
+    uint64_t* d_ib = cv.take<uint64_t>(b_ib);
+    uint32_t* d_rm = cv.take<uint32_t>(b_rm);
+    uint32_t* d_rb = cv.take<uint32_t>(b_rb);
+    OFP_HIP(c, hipMemcpy(d_htm, ht_m.data(), ht_m.size() * 4, hipMemcpyHostToDevice), "upload group rows");
+    OFP_HIP(c, hipMemcpy(d_htb, ht_b.data(), ht_b.size() * 4, hipMemcpyHostToDevice), "upload group rows");
+    OFP_HIP(c, hipMemcpy(d_bm, base_m.data(), (size_t)NM * 8, hipMemcpyHostToDevice), "upload bases");
+    OFP_HIP(c, hipMemcpy(d_bb, base_b.data(), (size_t)NM * 8, hipMemcpyHostToDevice), "upload bases");
+    if (!tm.empty())
+        OFP_HIP(c, hipMemcpy(d_tm, tm.data(), tm.size() * sizeof(XTile), hipMemcpyHostToDevice), "upload tiles");
+    if (!tb.empty())
+        OFP_HIP(c, hipMemcpy(d_tb, tb.data(), tb.size() * sizeof(XTile), hipMemcpyHostToDevice), "upload tiles");
+
+    auto launch_of = [&](bool main_kind) {
+        XchgLaunch a{};
+        a.res = main_kind ? lw.main : lw.blk;
+        a.gtab = lw.tables;
+        a.tiles = main_kind ? d_tm : d_tb;
+        a.ntiles = (uint32_t)(main_kind ? tm.size() : tb.size());
+        a.K = Kp;
+        a.top = main_kind ? W - 1 : W - 2;
+        a.base = main_kind ? L + 1 : L;
+        a.nf = nf;
+        a.main = main_kind ? 1 : 0;
+        a.ht = main_kind ? d_htm : d_htb;
+        a.dplain = c->dp32;
+        a.base_of = main_kind ? d_bm : d_bb;
+        for (int s = 0; s <= shares; ++s) a.first[s] = first[s];
+        a.shares = shares;
+        a.counts = main_kind ? d_cm : d_cb;
+        a.offs = main_kind ? d_om : d_ob;
+        a.totals = d_tot + (main_kind ? 0 : kMaxShares);
+        a.out_idx = main_kind ? d_im : d_ib;
+        a.out_res = main_kind ? d_rm : d_rb;
+        return a;
+    };
+    const XchgLaunch am = launch_of(true), ab = launch_of(false);
+    OFP_HIP(c, hipEventRecord(c->ev0, c->stream), "event");
+    OFP_HIP(c, launch_exchange_count(am, c->stream), "exchange count");
+    OFP_HIP(c, launch_exchange_count(ab, c->stream), "exchange count");
+    OFP_HIP(c, launch_exchange_scan(am, c->stream), "exchange scan");
+    OFP_HIP(c, launch_exchange_scan(ab, c->stream), "exchange scan");
+    OFP_HIP(c, launch_exchange_write(am, c->stream), "exchange write");
+    OFP_HIP(c, launch_exchange_write(ab, c->stream), "exchange write");
+    OFP_HIP(c, hipEventRecord(c->ev1, c->stream), "event");
+    OFP_HIP(c, hipEventSynchronize(c->ev1), "exchange sync");
+    float ms = 0;
+    OFP_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1), "event time");
+    uint64_t tot[2 * kMaxShares] = {};
+    OFP_HIP(c, hipMemcpy(tot, d_tot, sizeof tot, hipMemcpyDeviceToHost), "read bucket counts");
+    *info = ofp_exchange_info{};
+    info->main_count = M;
+    info->blocked_count = B;
+    info->shares = shares;
+    uint64_t sm = 0, sb = 0;
+    for (int s = 0; s <= shares; ++s) info->share_first[s] = first[s];
+    for (int s = 0; s < shares; ++s) {
+        info->main_records[s] = tot[s];
+        info->blocked_records[s] = tot[kMaxShares + s];
+        sm += tot[s];
+        sb += tot[kMaxShares + s];
+    }
+    if (sm != M || sb != B) return fail(c, OFP_E_RUNTIME, "exchange bucket counts do not add up to the window");
+    info->staging_bytes = b_im + b_ib + b_rm + b_rb;
+    info->moved_bytes = (M + B) * (uint64_t)Kp * 4 + (M + B) * (8 + (uint64_t)Kp * 4);
+    info->kernel_s = ms * 1e-3;
+    if (main_index && M) OFP_HIP(c, hipMemcpy(main_index, d_im, M * 8, hipMemcpyDeviceToHost), "download records");
+    if (blocked_index && B)
+        OFP_HIP(c, hipMemcpy(blocked_index, d_ib, B * 8, hipMemcpyDeviceToHost), "download records");
+    if ((main_res && M) || (blocked_res && B)) {
+        std::vector<uint32_t> t(std::max(M, B) * (size_t)Kp);
+        if (main_res && M) {
+            OFP_HIP(c, hipMemcpy(t.data(), d_rm, M * Kp * 4, hipMemcpyDeviceToHost), "download records");
+            for (uint64_t i = 0; i < M; ++i)
+                for (int j = 0; j < k; ++j) main_res[i * k + j] = t[i * Kp + j];
+        }
+        if (blocked_res && B) {
+            OFP_HIP(c, hipMemcpy(t.data(), d_rb, B * Kp * 4, hipMemcpyDeviceToHost), "download records");
+            for (uint64_t i = 0; i < B; ++i)
+                for (int j = 0; j < k; ++j) blocked_res[i * k + j] = t[i * Kp + j];
+        }
+    }
     return OFP_OK;
 }
 
@@ -1530,6 +1778,7 @@ void ofp_destroy(ofp_ctx* c) {
     if (c->dpc64) (void)hipFree(c->dpc64);
     if (c->ct32) (void)hipFree(c->ct32);
     if (c->win) (void)hipFree(c->win);
+    if (c->stage) (void)hipFree(c->stage);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);

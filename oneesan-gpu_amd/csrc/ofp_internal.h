@@ -106,4 +106,37 @@ uint64_t pair_class_count(int W, int p, int d);
 // One pair-pass launch (a.kc in {1, 2, 4, ..., 16} moduli, 32-bit ranks).
 hipError_t launch_pair(const pair::PairLaunch& a, bool wide_index, int threads, int blocks, hipStream_t s);
 
+// ---- exchange between the two placements of a row (ofp_exchange.hip) ----
+// A tile: `len` consecutive words of one group of a LOW window (local ranks
+// rank0.., window index word0..).
+struct XTile {
+    uint64_t word0;
+    uint32_t group, rank0, len, pad;
+};
+constexpr int kMaxShares = 16;
+constexpr uint32_t kXTileWords = 8192;  // words per tile (at most)
+struct XchgLaunch {
+    const uint32_t* res;     // the window's residues of this kind (main or blocked), stride K
+    const char* gtab;        // the window's packed 32-bit group tables (kDpRows x kDpCols quads each)
+    const XTile* tiles;
+    uint32_t ntiles;
+    int K;                   // interleave stride: a record carries K residues
+    int top, base, nf;       // top position of the kind, first fixed HIGH position, fixed positions
+    int main;                // 1: main words (width W), 0: blocked words (width W-1)
+    const uint32_t* ht;      // group-row table of this kind (ofp_exchange_impl.h)
+    const void* dplain;      // plain packed 32-bit height table
+    const uint64_t* base_of; // per HIGH mask: offset of its group in its share's placement
+    uint32_t first[kMaxShares + 1];  // share s owns HIGH masks first[s]..first[s+1]-1
+    int shares;
+    uint32_t* counts;        // [shares][ntiles] words per destination share and tile
+    uint64_t* offs;          // [shares][ntiles] exclusive prefix within the share
+    uint64_t* totals;        // [shares] records per share
+    uint64_t* out_idx;       // records: destination index in the share's HIGH placement
+    uint32_t* out_res;       //          and the K residues; bucket s after buckets 0..s-1
+};
+// Count, scan and write launches of one kind (in that order on one stream).
+hipError_t launch_exchange_count(const XchgLaunch& a, hipStream_t s);
+hipError_t launch_exchange_scan(const XchgLaunch& a, hipStream_t s);
+hipError_t launch_exchange_write(const XchgLaunch& a, hipStream_t s);
+
 }  // namespace ofp

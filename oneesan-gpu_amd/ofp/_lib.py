@@ -30,7 +30,7 @@ EXPORTS = (
     "ofp_job_create", "ofp_job_enqueue", "ofp_job_info_get", "ofp_job_result", "ofp_job_destroy",
     "ofp_shard_plan", "ofp_shard_sweep", "ofp_group_plan", "ofp_group_sweep", "ofp_set_height_caps", "ofp_height_cap", "ofp_set_xcd_pieces",
     "ofp_set_sorted_cells", "ofp_time_cells", "ofp_set_index_width", "ofp_set_max_batch",
-    "ofp_moduli_groups", "ofp_solve_static", "ofp_set_pair_passes",
+    "ofp_moduli_groups", "ofp_solve_static", "ofp_set_pair_passes", "ofp_group_exchange",
 )
 
 
@@ -155,6 +155,32 @@ class GroupInfo(ctypes.Structure):
         return d
 
 
+OFP_MAX_SHARES = 16
+
+
+class ExchangeInfo(ctypes.Structure):
+    _fields_ = [
+        ("main_count", ctypes.c_uint64),
+        ("blocked_count", ctypes.c_uint64),
+        ("shares", ctypes.c_int),
+        ("share_first", ctypes.c_uint32 * (OFP_MAX_SHARES + 1)),
+        ("main_records", ctypes.c_uint64 * OFP_MAX_SHARES),
+        ("blocked_records", ctypes.c_uint64 * OFP_MAX_SHARES),
+        ("staging_bytes", ctypes.c_uint64),
+        ("moved_bytes", ctypes.c_uint64),
+        ("kernel_s", ctypes.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        s = self.shares
+        return {"main_count": self.main_count, "blocked_count": self.blocked_count, "shares": s,
+                "share_first": [int(self.share_first[i]) for i in range(s + 1)],
+                "main_records": [int(self.main_records[i]) for i in range(s)],
+                "blocked_records": [int(self.blocked_records[i]) for i in range(s)],
+                "staging_bytes": self.staging_bytes, "moved_bytes": self.moved_bytes,
+                "kernel_s": self.kernel_s}
+
+
 _lib = None
 
 
@@ -209,6 +235,8 @@ def load() -> ctypes.CDLL:
         "ofp_shard_sweep": (ctypes.c_int32, [vp, ctypes.POINTER(ShardSpec), ctypes.POINTER(ShardInfo), u32p, u32p]),
         "ofp_group_plan": (ctypes.c_int32, [vp, ctypes.POINTER(GroupSpec), ctypes.POINTER(GroupInfo), u64p, u64p]),
         "ofp_group_sweep": (ctypes.c_int32, [vp, ctypes.POINTER(GroupSpec), ctypes.POINTER(GroupInfo), u32p, u32p]),
+        "ofp_group_exchange": (ctypes.c_int32, [vp, ctypes.POINTER(GroupSpec), ctypes.c_int,
+                                                ctypes.POINTER(ExchangeInfo), u64p, u32p, u64p, u32p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -456,6 +484,37 @@ class Context:
                                        bo.ctypes.data_as(u32p) if bo is not None else None)
         self._check(st, "ofp_group_sweep")
         return info.as_dict(), mo, (bo if blocked_out else None)
+
+    def group_exchange(self, n: int, moduli: Sequence[int], first_group: int, groups: int, low: int = 0,
+                       shares: int = 8, download: bool = False):
+        """Write the words of the LOW window this context swept last (groups
+        first_group..first_group+groups-1, the high half of a row) into staging
+        buckets, one per share of the HIGH groups (ofp_group_exchange).  Returns
+        (info, buckets): buckets is None unless download is set, else a dict with
+        'main_index', 'main_res', 'blocked_index', 'blocked_res' (all buckets in
+        order; bucket s of kind x holds info[x + '_records'][s] records)."""
+        import numpy as np
+        k = len(moduli)
+        mods = _u32_array(moduli)
+        spec = GroupSpec(n, k, ctypes.cast(mods, ctypes.POINTER(ctypes.c_uint32)), low, 0, first_group, groups, 0)
+        ginfo = GroupInfo()
+        self._check(self._lib.ofp_group_plan(self._h, ctypes.byref(spec), ctypes.byref(ginfo), None, None),
+                    "ofp_group_plan")
+        u32p = ctypes.POINTER(ctypes.c_uint32)
+        u64p = ctypes.POINTER(ctypes.c_uint64)
+        out = None
+        args = [None, None, None, None]
+        if download:
+            out = {"main_index": np.zeros(ginfo.main_count, np.uint64),
+                   "main_res": np.zeros((ginfo.main_count, k), np.uint32),
+                   "blocked_index": np.zeros(ginfo.blocked_count, np.uint64),
+                   "blocked_res": np.zeros((ginfo.blocked_count, k), np.uint32)}
+            args = [out["main_index"].ctypes.data_as(u64p), out["main_res"].ctypes.data_as(u32p),
+                    out["blocked_index"].ctypes.data_as(u64p), out["blocked_res"].ctypes.data_as(u32p)]
+        info = ExchangeInfo()
+        self._check(self._lib.ofp_group_exchange(self._h, ctypes.byref(spec), shares, ctypes.byref(info), *args),
+                    "ofp_group_exchange")
+        return info.as_dict(), out
 
     def set_variant(self, variant: int) -> None:
         """Cell-kernel variant (state-major layout): 0 = early orbit loads, 1 = late,
