@@ -35,7 +35,9 @@ Measured in one process, in this order:
     residues/hour of the plain form, and its per-kernel launch times.
   * plain_small: whole one-modulus solves at n=12 and n=18 (median of three).
   * shard_n27: one GPU's share of an n=27 row, both halves, every window of the
-    share streamed through HBM (ofp.shard.measure_share).
+    share streamed through HBM, and between the halves the exchange of every
+    LOW window's words into staging buckets by destination share
+    (ofp.shard.measure_share).
   * cpu_baseline: the CPU oracle's pull-form step on a bounded sample of the
     n=22 cell step, its whole n=12 solve (median of three) and one whole n=16
     solve, on the host cores.
@@ -479,6 +481,11 @@ def main() -> int:
         log(f"bench: n=27 share 0 of 8, whole row streamed in {shard['high']['windows']}+{shard['low']['windows']} "
             f"windows: {shard['row_kernel_s']:.2f} s kernel ({shard['row_wall_s']:.1f} s wall); high cells "
             f"{shard['high']['roofline']['frac']:.3f}, low cells {shard['low']['roofline']['frac']:.3f} of HBM peak")
+        x = shard.get("exchange")
+        if x:
+            log(f"bench: n=27 share 0 exchange: {x['windows']} windows, {x['kernel_s']:.2f} s, "
+                f"{x['roofline']['achieved']:.0f} GB/s ({x['roofline']['frac']:.3f} of peak), "
+                f"{x['cross_records'] / max(x['words'], 1):.3f} of the records for other shares")
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -233,6 +233,13 @@ ofp_status window_buffer(ofp_ctx* c, size_t bytes, char** out) {
     }
     size_t fr = 0, tot = 0;
     OFP_HIP(c, hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
+    if (bytes + c->reserve > fr && c->stage) {
+        // the staging of the last exchange gives way to a larger window
+        OFP_HIP(c, hipFree(c->stage), "free exchange buffer");
+        c->stage = nullptr;
+        c->stage_bytes = 0;
+        OFP_HIP(c, hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
+    }
     if (bytes + c->reserve > fr) return fail(c, OFP_E_MEMORY, "window does not fit HBM");
     OFP_HIP(c, hipMalloc(&c->win, bytes), "window buffer");
     c->win_bytes = bytes;

@@ -11,14 +11,15 @@
 //
 // Three launches per kind of word (main, blocked), one workgroup per tile of
 // at most kXTileWords words of one group:
-//   count  -- decode the fixed positions of each word, count its destination
-//             share (LDS counters per tile);
+//   count  -- decode the fixed positions of each word and count its
+//             destination share (one wave64 ballot per share and round);
 //   scan   -- one workgroup per share: exclusive prefix of the tile counts;
-//   write  -- decode each word whole, rank it in its HIGH group, and write its
-//             record.  Records keep the window order inside a bucket: each
-//             round of 256 words is compacted with one wave64 ballot per share
-//             (a lane's slot = the tile's cursor + the set lanes below it + the
-//             counts of the lower waves), so the buckets are deterministic.
+//   write  -- decode each word whole, ranking it in its HIGH group on the way
+//             (xchg::decode_rank), and write its record.  Records keep the
+//             window order inside a bucket: each round of 256 words is
+//             compacted with one wave64 ballot per share (a lane's slot = the
+//             tile's cursor + the set lanes below it + the counts of the lower
+//             waves), so the buckets are deterministic.
 // The residues are read once in window order (coalesced) and written once as
 // records; records of one share in one round are contiguous.
 #include "ofp_exchange_impl.h"
@@ -47,12 +48,23 @@ __global__ __launch_bounds__(kXThreads) void xchg_count_kernel(XchgLaunch a) {
     load_quads(tq, group_table(a, t.group));
     if (threadIdx.x < kMaxShares) cnt[threadIdx.x] = 0;
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < t.len; i += blockDim.x) {
-        const uint64_t w = a.main ? xchg::decode_group_word<true>(tq, a.top, a.base, t.rank0 + i)
-                                  : xchg::decode_group_word<false>(tq, a.top, a.base, t.rank0 + i);
-        const int s = xchg::share_of(xchg::high_mask(w, a.base, a.nf), a.first, a.shares);
-        atomicAdd(&cnt[s], 1u);
+    // lane q of each wave counts share q
+    const int lane = threadIdx.x & 63;
+    uint32_t mine = 0;
+    for (uint32_t r0 = 0; r0 < t.len; r0 += kXThreads) {
+        const uint32_t i = r0 + threadIdx.x;
+        int s = -1;
+        if (i < t.len) {
+            const uint64_t w = a.main ? xchg::decode_group_word<true>(tq, a.top, a.base, t.rank0 + i)
+                                      : xchg::decode_group_word<false>(tq, a.top, a.base, t.rank0 + i);
+            s = xchg::share_of(xchg::high_mask(w, a.base, a.nf), a.first, a.shares);
+        }
+        for (int q = 0; q < a.shares; ++q) {
+            const uint32_t c = (uint32_t)__popcll(__ballot(s == q));
+            if (lane == q) mine += c;
+        }
     }
+    if (lane < a.shares) atomicAdd(&cnt[lane], mine);
     __syncthreads();
     if ((int)threadIdx.x < a.shares) a.counts[(size_t)threadIdx.x * a.ntiles + blockIdx.x] = cnt[threadIdx.x];
 }
@@ -105,11 +117,12 @@ __global__ __launch_bounds__(kXThreads) void xchg_write_kernel(XchgLaunch a) {
         int s = -1;
         uint64_t dst = 0;
         if (valid) {
-            const uint64_t w = a.main ? xchg::decode_group_word<true>(tq, a.top, 0, t.rank0 + i)
-                                      : xchg::decode_group_word<false>(tq, a.top, 0, t.rank0 + i);
-            const uint32_t mask = xchg::high_mask(w, a.base, a.nf);
+            uint64_t w;
+            uint32_t mask;
+            const uint32_t rank = a.main ? xchg::decode_rank<true>(tq, dq, a.ht, a.top, a.base, t.rank0 + i, w, mask)
+                                         : xchg::decode_rank<false>(tq, dq, a.ht, a.top, a.base, t.rank0 + i, w, mask);
             s = xchg::share_of(mask, a.first, a.shares);
-            dst = a.base_of[mask] + xchg::high_rank(w, mask, a.top, a.base, a.ht, dq);
+            dst = a.base_of[mask] + rank;
         }
         uint32_t pre = 0;
         for (int q = 0; q < a.shares; ++q) {

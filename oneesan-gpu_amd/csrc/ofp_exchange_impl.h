@@ -92,6 +92,52 @@ OFP_HD uint32_t high_rank(uint64_t word, uint32_t mask, int top, int base, const
     return rank;
 }
 
+// decode_group_word and high_mask / high_rank in one walk from the top: the
+// fixed HIGH positions first (their L's and the heights there are recorded,
+// since the group row of position base+j depends on the mask bits below j, not
+// yet decoded), then the free positions, whose plain-table contributions are
+// summed as they are decoded; the recorded L's are added last.  A legal word
+// of W <= 28 positions stays at height <= 14, so h-1 fits 4 bits per position.
+template <bool Main>
+OFP_HD uint32_t decode_rank(const Quad<uint32_t> (*q)[kDpCols], const Quad<uint32_t> (*d)[kDpCols],
+                            const uint32_t* ht, int top, int base, uint32_t r, uint64_t& word_out,
+                            uint32_t& mask_out) {
+    uint64_t word = 0, hpack = 0;
+    uint32_t mask = 0, lmask = 0, rank = 0;
+    int h = 1;
+    for (int pos = top; pos >= base; --pos) {
+        const Z2<uint32_t> e = (Main && pos >= 1) ? cell::ldm(q[pos - 1][h]) : cell::ldz(q[pos][h]);
+        const bool g0 = r >= e.zn, g1 = r >= e.zc;
+        r -= g1 ? e.zc : (g0 ? e.zn : 0u);
+        const int j = pos - base;
+        mask |= (uint32_t)g0 << j;
+        if (g1 && h >= 1) {
+            lmask |= 1u << j;
+            hpack |= (uint64_t)(h - 1) << (4 * j);
+        }
+        word |= (uint64_t)((int)g0 + (int)g1) << (2 * pos);
+        h += g1 ? 1 : (g0 ? -1 : 0);
+    }
+    for (int pos = base - 1; pos >= 0; --pos) {
+        const Z2<uint32_t> e = (Main && pos >= 1) ? cell::ldm(q[pos - 1][h]) : cell::ldz(q[pos][h]);
+        const Z2<uint32_t> f = cell::ldz(d[pos][h]);
+        const bool g0 = r >= e.zn, g1 = r >= e.zc;
+        r -= g1 ? e.zc : (g0 ? e.zn : 0u);
+        rank += g1 ? f.zc : (g0 ? f.zn : 0u);
+        word |= (uint64_t)((int)g0 + (int)g1) << (2 * pos);
+        h += g1 ? 1 : (g0 ? -1 : 0);
+    }
+    while (lmask) {
+        const int j = __builtin_ctz(lmask);
+        lmask &= lmask - 1u;
+        const uint32_t below = mask & ((1u << j) - 1u);
+        rank += ht[(size_t)((1u << j) - 1u + below) * kDpCols + (uint32_t)((hpack >> (4 * j)) & 15u)];
+    }
+    word_out = word;
+    mask_out = mask;
+    return rank;
+}
+
 // Share (destination shard) of a HIGH mask: shares own consecutive runs of
 // masks, share s the masks first[s]..first[s+1]-1.
 OFP_HD int share_of(uint32_t mask, const uint32_t* first, int shares) {

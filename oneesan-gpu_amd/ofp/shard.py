@@ -21,13 +21,17 @@ is 1/`shares` of each half's groups by word count (consecutive masks).
 
 `measure_share` streams every window of one share through both halves,
 seeding each window in HBM, and reports the kernel time of the share's whole
-row.  `measure` times one window of the older prefix-class placement of the
-low cells (cells 14..1 over 1/`slices` of the state, 64-bit global ranks,
-main words of other shards read as seeded residues).
+row.  Between the halves, each LOW window's words are written into staging
+buckets, one per share of the HIGH groups (ofp_group_exchange): the records
+for the other shares are the send side of the per-row all-to-all.  `measure`
+times one window of the older prefix-class placement of the low cells (cells
+14..1 over 1/`slices` of the state, 64-bit global ranks, main words of other
+shards read as seeded residues).
 
-Left out (stated in the report): the per-row all-to-all that moves the state
-between the two placements, and the host or peer traffic that would stage the
-non-resident windows (each window here is seeded in place).
+Left out (stated in the report): sending the staging buckets to the other GPUs
+(the all-to-all itself; one GPU here), and the host or peer traffic that would
+stage the non-resident windows (each window here is seeded in place, and the
+low half's windows are seeded rather than assembled from the buckets).
 
 Roofline bytes per launch: 8 B x k x (main + blocked words of the window).
 """
@@ -104,15 +108,23 @@ def share_windows(n: int, k: int, shares: int, share: int, window_bytes: float, 
 
 
 def measure_share(ctx: _lib.Context, n: int = 27, k: int = 4, shares: int = 8, share: int = 0,
-                  window_bytes: float = 136e9, seed: int = 20261202, log=None, max_windows: int = 0) -> dict:
+                  window_bytes: float = 136e9, seed: int = 20261202, log=None, max_windows: int = 0,
+                  exchange: bool = True, exchange_window_bytes: float = 104e9) -> dict:
     """Stream every window of one share through both halves of a row
-    (max_windows > 0: only the first windows of each half, for profiling runs)."""
+    (max_windows > 0: only the first windows of each half, for profiling runs).
+    exchange: after the high half of each LOW window, write its words into the
+    staging buckets of the shares owning their HIGH groups (ofp_group_exchange);
+    the staging sits beside the window in HBM, so those windows are cut to
+    exchange_window_bytes."""
     mods = PRIMES[:k]
     W = n + 1
     t0 = time.perf_counter()
     halves = {}
+    xch = {"windows": 0, "words": 0, "kernel_s": 0.0, "moved_bytes": 0, "own_records": 0, "cross_records": 0,
+           "max_staging_bytes": 0}
     for name, high in (("high", False), ("low", True)):
-        low, windows = share_windows(n, k, shares, share, window_bytes, high=high)
+        wb = exchange_window_bytes if (exchange and not high) else window_bytes
+        low, windows = share_windows(n, k, shares, share, wb, high=high)
         if max_windows > 0:
             windows = windows[:max_windows]
         cells = low if high else W - 1 - low
@@ -122,6 +134,19 @@ def measure_share(ctx: _lib.Context, n: int = 27, k: int = 4, shares: int = 8, s
              "words": 0, "kernel_s": 0.0, "launches": 0, "max_window_bytes": 0, "cell_ms": [0.0] * cells}
         for g0, g in windows:
             info, _, _ = ctx.group_sweep(n, mods, g0, g, low, seed=seed, high=high)
+            if exchange and not high:
+                x, _ = ctx.group_exchange(n, mods, g0, g, low, shares=shares)
+                xch["windows"] += 1
+                xch["words"] += x["main_count"] + x["blocked_count"]
+                xch["kernel_s"] += x["kernel_s"]
+                xch["moved_bytes"] += x["moved_bytes"]
+                own = x["main_records"][share] + x["blocked_records"][share]
+                xch["own_records"] += own
+                xch["cross_records"] += x["main_count"] + x["blocked_count"] - own
+                xch["max_staging_bytes"] = max(xch["max_staging_bytes"], x["staging_bytes"])
+                if log:
+                    log(f"share {share}: exchange of groups {g0}..{g0 + g - 1}: {x['kernel_s']:.3f} s, "
+                        f"{x['moved_bytes'] / x['kernel_s'] / 1e9:.0f} GB/s")
             h["groups"] += g
             h["words"] += info["main_count"] + info["blocked_count"]
             h["kernel_s"] += info["kernel_s"]
@@ -140,6 +165,15 @@ def measure_share(ctx: _lib.Context, n: int = 27, k: int = 4, shares: int = 8, s
     wall = time.perf_counter() - t0
     nm, nb = _lib.state_counts(n)
     hi, lo = halves["high"], halves["low"]
+    if exchange and xch["kernel_s"] > 0:
+        gbps = xch["moved_bytes"] / xch["kernel_s"] / 1e9
+        xch["roofline"] = {"bound": "hbm", "achieved": gbps, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": gbps / HBM_PEAK_GBS,
+                           "bytes": "per word: its residues read (4 B x stride) + its record written "
+                                    "(8 B index + 4 B x stride)"}
+        xch["what"] = (f"each LOW window's words after the high half into {shares} staging buckets by the share "
+                       f"owning their HIGH group; the cross-share records stand for the all-to-all's send "
+                       f"buffers, the own share's for its local placement")
     return {
         "workload": f"n={n} row of one GPU's share ({share} of {shares}), {k} moduli interleaved, seeded "
                     f"inputs: high cells {hi['cells']} over {hi['placement']} in {hi['windows']} windows, "
@@ -151,8 +185,12 @@ def measure_share(ctx: _lib.Context, n: int = 27, k: int = 4, shares: int = 8, s
         "transitions_per_s": (hi["transitions_per_s"] * hi["kernel_s"] + lo["transitions_per_s"] * lo["kernel_s"])
         / (hi["kernel_s"] + lo["kernel_s"]),
         "high": hi, "low": lo,
-        "left_out": "the per-row all-to-all between the two placements, and the host/peer staging of the "
-                    "non-resident windows (each window is seeded in HBM); wall time includes the seeding",
+        "exchange": xch if exchange else None,
+        "left_out": ("the transfer of the staging buckets to the other GPUs (the all-to-all itself), and the "
+                     "host/peer staging of the non-resident windows (each window is seeded in HBM); wall time "
+                     "includes the seeding") if exchange else
+                    ("the per-row all-to-all between the two placements, and the host/peer staging of the "
+                     "non-resident windows (each window is seeded in HBM); wall time includes the seeding"),
     }
 
 
@@ -166,6 +204,7 @@ def main(argv=None) -> int:
     ap.add_argument("--share", type=int, default=None, help="stream a whole row of this share")
     ap.add_argument("--shares", type=int, default=8)
     ap.add_argument("--max-windows", type=int, default=0, help="only the first windows of each half (profiling)")
+    ap.add_argument("--no-exchange", action="store_true", help="leave out the exchange after the high half")
     ap.add_argument("--window", type=int, nargs=3, metavar=("HIGH", "FIRST", "GROUPS"), default=None,
                     help="one window only: half (0: high cells / LOW groups, 1: low cells / HIGH groups), "
                          "first group and group count (profiling)")
@@ -178,7 +217,8 @@ def main(argv=None) -> int:
             return 0
         if a.share is not None:
             rep = measure_share(ctx, a.n, a.k, a.shares, a.share,
-                                log=lambda m: print(m, file=sys.stderr, flush=True), max_windows=a.max_windows)
+                                log=lambda m: print(m, file=sys.stderr, flush=True), max_windows=a.max_windows,
+                                exchange=not a.no_exchange)
             stray = 0
         else:
             rep = measure(ctx, a.n, a.k, a.slices, a.slice)

@@ -1,6 +1,6 @@
 // Host check of the exchange's index math (oneesan-gpu_amd/csrc/
-// ofp_exchange_impl.h): decode_group_word, high_mask, high_rank and
-// build_high_rows, built for the CPU.
+// ofp_exchange_impl.h): decode_group_word, high_mask, high_rank, the fused
+// decode_rank and build_high_rows, built for the CPU.
 //
 // `exchange_host_check small NMAX`: for every n <= NMAX and every split low,
 // every main and blocked word is enumerated in global rank order (the oracle's
@@ -116,8 +116,13 @@ static int small(int nmax) {
                     const uint32_t hm = xchg::high_mask(w, base, nf);
                     const uint32_t hr =
                         xchg::high_rank(w, hm, top, base, mainw ? htm.data() : htb.data(), g_plain);
+                    uint64_t fw = 0;
+                    uint32_t fm = 0;
+                    const uint32_t fr =
+                        mainw ? xchg::decode_rank<true>(q, g_plain, htm.data(), top, base, (uint32_t)lrank[i], fw, fm)
+                              : xchg::decode_rank<false>(q, g_plain, htb.data(), top, base, (uint32_t)lrank[i], fw, fm);
                     if (got != w || xchg::high_mask(top_only, base, nf) != hm || hm != occ(w, base, nf) ||
-                        hr != hrank[i]) {
+                        hr != hrank[i] || fw != w || fm != hm || fr != hr) {
                         printf("MISMATCH n=%d low=%d kind=%s rank=%llu word=%llx decoded=%llx high mask %u/%u rank "
                                "%u/%llu\n",
                                n, low, mainw ? "main" : "blocked", (unsigned long long)i, (unsigned long long)w,
@@ -164,7 +169,11 @@ static int big(int n, int low, int samples, unsigned seed) {
             cell::build_group_table(mainw ? om : ob, f);
             const uint64_t want = group_rank(w, top, mainw ? om : ob, f);
             const uint32_t hr = xchg::high_rank(w, hm, top, base, mainw ? htm.data() : htb.data(), g_plain);
-            if (lr >> 32 || got != w || want >> 32 || hr != (uint32_t)want) {
+            uint64_t fw = 0;
+            uint32_t fm = 0;
+            const uint32_t fr = mainw ? xchg::decode_rank<true>(q, g_plain, htm.data(), top, base, (uint32_t)lr, fw, fm)
+                                      : xchg::decode_rank<false>(q, g_plain, htb.data(), top, base, (uint32_t)lr, fw, fm);
+            if (lr >> 32 || got != w || want >> 32 || hr != (uint32_t)want || fw != w || fm != hm || fr != hr) {
                 printf("MISMATCH big n=%d low=%d kind=%s word=%llx low rank %llu decoded %llx high rank %u/%llu\n",
                        n, low, mainw ? "main" : "blocked", (unsigned long long)w, (unsigned long long)lr,
                        (unsigned long long)got, hr, (unsigned long long)want);
