@@ -241,8 +241,11 @@ ofp_status window_buffer(ofp_ctx* c, size_t bytes, char** out) {
         OFP_HIP(c, hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
     }
     if (bytes + c->reserve > fr) return fail(c, OFP_E_MEMORY, "window does not fit HBM");
-    OFP_HIP(c, hipMalloc(&c->win, bytes), "window buffer");
-    c->win_bytes = bytes;
+    // 1/16 headroom where it fits: windows of one stream differ by a few percent,
+    // and every regrowth frees touched pages and touches fresh ones
+    const size_t slack = bytes + c->reserve + bytes / 16 <= fr ? bytes / 16 / 256 * 256 : 0;
+    OFP_HIP(c, hipMalloc(&c->win, bytes + slack), "window buffer");
+    c->win_bytes = bytes + slack;
     *out = c->win;
     return OFP_OK;
 }
@@ -1505,8 +1508,9 @@ ofp_status stage_buffer(ofp_ctx* c, size_t bytes, char** out) {
     size_t fr = 0, tot = 0;
     OFP_HIP(c, hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
     if (bytes + c->reserve > fr) return fail(c, OFP_E_MEMORY, "exchange staging does not fit HBM beside the window");
-    OFP_HIP(c, hipMalloc(&c->stage, bytes), "exchange buffer");
-    c->stage_bytes = bytes;
+    const size_t slack = bytes + c->reserve + bytes / 16 <= fr ? bytes / 16 / 256 * 256 : 0;  // as the window's
+    OFP_HIP(c, hipMalloc(&c->stage, bytes + slack), "exchange buffer");
+    c->stage_bytes = bytes + slack;
     *out = c->stage;
     return OFP_OK;
 }

@@ -274,9 +274,60 @@ def dram(path: str) -> None:
               f"{(o['read'] + o['write']) / 1e9:10.3f}")
 
 
+_PAIR_D = re.compile(r"pair_kernel<(\d+), [^,]+, (-?\d+),")
+_ORDER = (-1, 0, 1, -2, 2)
+
+
+def launches(path: str) -> None:
+    """Every dispatch of a counter run with its DRAM bytes and duration; the
+    pair-pass dispatches are grouped into passes (their launches run in the
+    order d = -1, 0, 1, -2, 2, each present only where the pass has classes of
+    that d; a new pass starts where the order restarts)."""
+    per: dict[int, dict] = {}
+    for r in csv.DictReader(open(path)):
+        e = per.setdefault(int(r["Dispatch_Id"]), {"kernel": r["Kernel_Name"],
+                                                   "ns": int(r["End_Timestamp"]) - int(r["Start_Timestamp"])})
+        e[r["Counter_Name"]] = float(r["Counter_Value"])
+    print(f"{'dispatch':>8s} {'kernel':22s} {'d':>3s} {'ms':>8s} {'read_GB':>8s} {'write_GB':>8s} {'TB/s':>6s}")
+    pas = None
+    prev_kc, prev_i = None, 99
+
+    def close(p):
+        if p and p["n"] > 1:
+            print(f"{'':8s} {'  pass total':22s} {'':3s} {p['ms']:8.2f} {p['rd'] / 1e9:8.1f} {p['wr'] / 1e9:8.1f} "
+                  f"{(p['rd'] + p['wr']) / p['ms'] / 1e9:6.2f}")
+    for d in sorted(per):
+        e = per[d]
+        if not all(c in e for c in DRAM_COUNTERS):
+            continue
+        rd, wr, ms = 32.0 * e[DRAM_COUNTERS[0]], 32.0 * e[DRAM_COUNTERS[1]], e["ns"] / 1e6
+        fam = kernel_family(e["kernel"])
+        m = _PAIR_D.search(e["kernel"])
+        if m:
+            kc, dd = int(m.group(1)), int(m.group(2))
+            i = _ORDER.index(dd) if dd in _ORDER else 99
+            if kc != prev_kc or i <= prev_i:
+                close(pas)
+                pas = {"ms": 0.0, "rd": 0.0, "wr": 0.0, "n": 0}
+            prev_kc, prev_i = kc, i
+            pas["ms"] += ms
+            pas["rd"] += rd
+            pas["wr"] += wr
+            pas["n"] += 1
+            dtxt = str(dd)
+        else:
+            close(pas)
+            pas, prev_kc, prev_i = None, None, 99
+            dtxt = ""
+        if fam is None or ms <= 0:
+            continue
+        print(f"{d:8d} {fam:22s} {dtxt:>3s} {ms:8.2f} {rd / 1e9:8.1f} {wr / 1e9:8.1f} {(rd + wr) / ms / 1e9:6.2f}")
+    close(pas)
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["trace", "counter", "valu", "dram"])
+    ap.add_argument("mode", choices=["trace", "counter", "valu", "dram", "launches"])
     ap.add_argument("csv")
     ap.add_argument("--cells", type=int, default=None, help="cells per row (W-1) for the per-row listing")
     ap.add_argument("--rows", type=int, default=None,
@@ -295,6 +346,8 @@ def main(argv=None) -> int:
         counter(a.csv, a.scale, a.cells)
     elif a.mode == "dram":
         dram(a.csv)
+    elif a.mode == "launches":
+        launches(a.csv)
     else:
         if not a.cells:
             ap.error("valu needs --cells")

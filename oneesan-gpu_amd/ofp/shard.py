@@ -205,6 +205,8 @@ def main(argv=None) -> int:
     ap.add_argument("--shares", type=int, default=8)
     ap.add_argument("--max-windows", type=int, default=0, help="only the first windows of each half (profiling)")
     ap.add_argument("--no-exchange", action="store_true", help="leave out the exchange after the high half")
+    ap.add_argument("--exchange", action="store_true",
+                    help="with --window 0 ...: after the sweep, exchange the window's words into --shares buckets")
     ap.add_argument("--window", type=int, nargs=3, metavar=("HIGH", "FIRST", "GROUPS"), default=None,
                     help="one window only: half (0: high cells / LOW groups, 1: low cells / HIGH groups), "
                          "first group and group count (profiling)")
@@ -213,6 +215,8 @@ def main(argv=None) -> int:
         if a.window is not None:
             high, g0, g = a.window
             info, _, _ = ctx.group_sweep(a.n, PRIMES[:a.k], g0, g, 0, seed=20261202, high=bool(high))
+            if a.exchange and not high:
+                info["exchange"], _ = ctx.group_exchange(a.n, PRIMES[:a.k], g0, g, 0, shares=a.shares)
             print(json.dumps(info), flush=True)
             return 0
         if a.share is not None:
