@@ -123,3 +123,31 @@ def test_kernel_family_names():
     assert rs.kernel_family("void ofp::fill_remote_kernel()") is None
     assert rs.kernel_family("void ofp::(anonymous namespace)::pair_kernel<14, unsigned int, -1>"
                             "(ofp::pair::PairLaunch)") == "pair_kernel<14>"
+
+
+def test_launches_groups_pair_dispatches_into_passes(tmp_path, capsys):
+    """Per-dispatch listing: DRAM bytes = counter x 32, pair launches grouped into
+    passes where the d order restarts, other kernels listed alone."""
+    p = tmp_path / "l.csv"
+    M = 10**7
+    seq = [(PAIR.format(d=-1), 2.0, 100 * M, 50 * M), (PAIR.format(d=0), 3.0, 200 * M, 100 * M),
+           (PAIR.format(d=2), 1.0, 10 * M, 10 * M), (PAIR.format(d=-1), 2.0, 300 * M, 30 * M),
+           (PAIR.format(d=-2), 1.0, 40 * M, 20 * M), (CELL, 4.0, 1000 * M, 500 * M)]
+    with open(p, "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=["Dispatch_Id", "Kernel_Name", "Counter_Name", "Counter_Value",
+                                          "Start_Timestamp", "End_Timestamp"])
+        w.writeheader()
+        t = 0
+        for i, (name, ms, rd, wr) in enumerate(seq):
+            for cname, v in (("TCC_EA0_RDREQ_DRAM_32B_sum", rd), ("TCC_EA0_WRREQ_WRITE_DRAM_32B_sum", wr)):
+                w.writerow({"Dispatch_Id": i + 1, "Kernel_Name": name, "Counter_Name": cname, "Counter_Value": v,
+                            "Start_Timestamp": t, "End_Timestamp": t + int(ms * 1e6)})
+            t += int(ms * 1e6) + 1000
+    rs.launches(str(p))
+    lines = capsys.readouterr().out.splitlines()
+    totals = [ln for ln in lines if "pass total" in ln]
+    assert len(totals) == 2
+    # first pass: 6 ms, (100 + 200 + 10) M x 32 B = 99.2 GB read, 160 M x 32 B = 51.2 GB written
+    assert totals[0].split()[-4:-1] == ["6.00", "99.2", "51.2"]
+    assert totals[1].split()[-4] == "3.00"
+    assert any(ln.split()[1] == "cell_kernel<14>" and ln.split()[2] == "4.00" for ln in lines if ln.strip())
