@@ -165,8 +165,12 @@ ofp_status ofp_time_cells(ofp_ctx* ctx, float* ms_out, int* kind_out, int capaci
  * the autotuner sets it per size).  Bits 8..9
  * of mode select a kernel variant for 14 or 16 moduli per launch (tuning: bit 8
  * = closure sums gathered before the class's words are loaded, bit 9 = a
- * register budget for three workgroups per CU); results do not depend on it.  Reference analogue: the reference's fused two-cell group kernels
- * (row6crt20.cu:1033-1154), which run in group scratch. */
+ * register budget for three workgroups per CU); results do not depend on it.
+ * Bit 10 of mode: group sweeps (ofp_group_sweep) run pair passes too, per group
+ * with its own class tables (off by default: at 4 moduli, n=27 windows, they
+ * are slower than one cell per launch).  Reference analogue: the reference's
+ * fused two-cell group kernels (row6crt20.cu:1033-1154), which run in group
+ * scratch. */
 ofp_status ofp_set_pair_passes(ofp_ctx* ctx, int mode, uint32_t cell_mask);
 
 /* Index width of the cell kernels: 0 = by n (32-bit ranks for n <= 22,
@@ -315,7 +319,11 @@ ofp_status ofp_shard_sweep(ofp_ctx* ctx, const ofp_shard_spec* spec, ofp_shard_i
  * group is stored contiguously with its words in global rank order (32-bit
  * ranks local to the group: every group of the window must hold fewer than
  * 2^32 words); the window's main array is the groups' main words one group
- * after the other, its blocked array likewise.  The cells run in place.
+ * after the other, its blocked array likewise.  The cells run in place, one
+ * per launch, or two per pass where group pair passes are on
+ * (ofp_set_pair_passes, bit 10 of mode: the pair pass over each group with the
+ * group's own class tables; cell_ms then holds a pass's time at its first cell
+ * and 0 at its second).
  * Reference analogue: one GPU's owner-local LOW and HIGH windows of the 8-GPU
  * bucket layout (docs/b300-bucket-snake.md:7-20; docs/b300-hbm32.md:251-270:
  * W = 28, LOW14 / HIGH13, 16,384 and 8,192 groups). */

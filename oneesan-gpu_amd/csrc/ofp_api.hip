@@ -71,6 +71,7 @@ struct ofp_ctx {
     int pair_kc = kMaxKC;  // moduli per pair-pass launch (even, <= 16)
     uint32_t pair_mask = 0;  // bit p: cells p and p-1 run as one pair pass; 0: the library's default
     int pair_variant = 0;    // pair-kernel variant (PairLaunch::variant)
+    int group_pairs = 0;     // group sweeps run pair passes too (off: at 4 moduli they are slower)
     int cap_percent = -1;  // height caps: cap a cell when it then sweeps <= this % of the blocked words
                            // (-1: 75 at one modulus, else 85)
     int xcd_shift = 5;     // chunk pieces per XCD: 2^xcd_shift chunks, < 0 = one eighth each
@@ -1009,8 +1010,9 @@ ofp_status ofp_time_cells(ofp_ctx* c, float* ms_out, int* kind_out, int capacity
 
 ofp_status ofp_set_pair_passes(ofp_ctx* c, int mode, uint32_t cell_mask) {
     if (!c) return OFP_E_ARG;
-    // mode bits 8..9 select the pair-kernel variant (tuning)
+    // mode bits 8..9 select the pair-kernel variant (tuning), bit 10 pair passes in group sweeps
     const int variant = (mode >> 8) & 3;
+    c->group_pairs = (mode >> 10) & 1;
     mode &= 0xff;
     if (mode < 0 || mode > kMaxKC || (mode > 1 && (mode & 1)))
         return fail(c, OFP_E_ARG, "pair passes: 0 (off), 1 (on) or an even moduli count per launch up to 16");
@@ -1366,14 +1368,60 @@ ofp_status ofp_group_sweep(ofp_ctx* c, const ofp_group_spec* sp, ofp_group_info*
     OFP_HIP(c, hipSetDevice(c->device), "hipSetDevice");
     const size_t mbytes = std::max<uint64_t>(M, 1) * Kp * 4, bbytes = std::max<uint64_t>(B, 1) * Kp * 4;
     const size_t tbytes = (size_t)G * kGroupTableBytes;
+    // The half's steps: cells p and p-1 as one pair pass where group pair passes
+    // are on and the context's pair settings pair them (the full-state rows'
+    // mask), else one cell.
+    const int p_hi = sp->high ? L : W - 1, p_lo = sp->high ? 1 : L + 1;
+    const uint32_t pmask = c->pair_mask ? c->pair_mask : default_pair_mask(k);
+    std::vector<int> steps;  // p of each step; a pair step is marked by p | 0x100
+    for (int p = p_hi; p >= p_lo;) {
+        if (c->group_pairs && c->pair_rows && p - 1 >= p_lo && p >= 2 && ((pmask >> p) & 1u)) {
+            steps.push_back(p | 0x100);
+            p -= 2;
+        } else {
+            steps.push_back(p);
+            --p;
+        }
+    }
+    int npairs = 0;
+    for (int x : steps) npairs += (x & 0x100) ? 1 : 0;
+    // Per group, pair step and window change d (-1, 0, 1, -2, 2): the group's
+    // class table and class count.
+    static const int kPairD[5] = {-1, 0, 1, -2, 2};
+    const size_t ctb = class_table_bytes(false);
+    const size_t ctbytes = (size_t)G * npairs * 5 * ctb;
+    std::vector<char> cts(std::max<size_t>(ctbytes, 1));
+    std::vector<uint64_t> ccount((size_t)G * npairs * 5, 0);
+    if (npairs) {
+        static uint64_t fm[kDpRows][kDpCols];
+        static cell::Pair<uint32_t> t[kDpRows][kDpCols];
+        for (uint32_t g = 0; g < G; ++g) {
+            cell::Occupancy om, ob;
+            cell::group_occupancy(sp->high != 0, W, L, sp->first_group + g, om, ob);
+            cell::build_group_table(om, fm);
+            int j = 0;
+            for (int x : steps) {
+                if (!(x & 0x100)) continue;
+                const int p = x & 0xff;
+                for (int di = 0; di < 5; ++di) {
+                    const size_t slot = ((size_t)g * npairs + j) * 5 + di;
+                    pair::pack_group_class_table<uint32_t, kDpCols>(om, fm, p, kPairD[di], t);
+                    std::memcpy(cts.data() + slot * ctb, t, ctb);
+                    ccount[slot] = pair::group_class_count(om, fm, W, p, kPairD[di]);
+                }
+                ++j;
+            }
+        }
+    }
     char* wbuf = nullptr;
     c->last_low.valid = false;
-    st = window_buffer(c, (mbytes + 255) / 256 * 256 + (bbytes + 255) / 256 * 256 + tbytes + 256, &wbuf);
+    st = window_buffer(c, (mbytes + 255) / 256 * 256 + (bbytes + 255) / 256 * 256 + tbytes + ctbytes + 512, &wbuf);
     if (st) return st;
     Carve cv{wbuf};
     uint32_t* dm = cv.take<uint32_t>(mbytes);
     uint32_t* db = cv.take<uint32_t>(bbytes);
     char* dtab = cv.take<char>(tbytes);
+    char* dct = cv.take<char>(ctbytes);
     uint32_t* dmods = cv.take<uint32_t>(kMaxKC * 4);
     std::vector<hipEvent_t> ev(info->cells + 1, nullptr);
     auto release = [&]() {
@@ -1395,6 +1443,7 @@ ofp_status ofp_group_sweep(ofp_ctx* c, const ofp_group_spec* sp, ofp_group_info*
                 return fail(c, OFP_E_RUNTIME, "group table packing");
             }
         OFP_HIP_R(hipMemcpy(dtab, tabs.data(), tabs.size(), hipMemcpyHostToDevice), "upload group tables");
+        if (ctbytes) OFP_HIP_R(hipMemcpy(dct, cts.data(), ctbytes, hipMemcpyHostToDevice), "upload class tables");
     }
     OFP_HIP_R(hipMemcpy(dmods, sp->moduli, k * 4, hipMemcpyHostToDevice), "copy moduli");
     const bool blk_in = sp->high && blk_io;
@@ -1419,10 +1468,46 @@ ofp_status ofp_group_sweep(ofp_ctx* c, const ofp_group_spec* sp, ofp_group_info*
         OFP_HIP_R(launch_fill_remote(dm, dom, fillM, db, dom, fillB, Kp, k, dmods, sp->seed, c->stream),
                   "fill group window");
     const int sf = c->sorted_from >= 0 ? c->sorted_from : (k >= 8 ? 6 : 0);
-    const int p_hi = sp->high ? L : W - 1, p_lo = sp->high ? 1 : L + 1;
     int launches = 0;
     OFP_HIP_R(hipEventRecord(ev[0], c->stream), "event");
-    for (int p = p_hi, i = 0; p >= p_lo; --p, ++i) {
+    int i = 0, pj = 0;
+    for (int x : steps) {
+        const int p = x & 0xff;
+        if (x & 0x100) {
+            // pair pass: per window change d, every group (groups are independent;
+            // inside a group the launches keep the order d = -1, 0, 1, -2, 2)
+            for (int di = 0; di < 5; ++di)
+                for (uint32_t g = 0; g < G; ++g) {
+                    const size_t slot = ((size_t)g * npairs + pj) * 5 + di;
+                    const uint64_t cnt = ccount[slot];
+                    if (!bc[g] || !cnt) continue;
+                    pair::PairLaunch a{};
+                    a.main_in = dm + moff[g] * Kp;
+                    a.blk_in = db + boff[g] * Kp;
+                    a.main_out = dm + moff[g] * Kp;
+                    a.blk_out = db + boff[g] * Kp;
+                    a.dp = dtab + (size_t)g * kGroupTableBytes;
+                    a.ct = dct + slot * ctb;
+                    a.classes = cnt;
+                    a.W = W;
+                    a.p = p;
+                    a.d = kPairD[di];
+                    a.read_blk = (p == W - 1) ? 0 : 1;
+                    a.K = Kp;
+                    a.off = 0;
+                    a.kc = Kp;
+                    for (int jj = 0; jj < Kp; ++jj) a.mod[jj] = sp->moduli[jj < k ? jj : 0];
+                    a.xcd_shift = piece_shift(c->xcd_shift, cnt, c->threads);
+                    a.variant = c->pair_variant;
+                    OFP_HIP_R(launch_pair(a, false, c->threads, grid_blocks(c, cnt), c->stream), "group pair launch");
+                    ++launches;
+                }
+            OFP_HIP_R(hipEventRecord(ev[i + 1], c->stream), "event");
+            OFP_HIP_R(hipEventRecord(ev[i + 2], c->stream), "event");
+            i += 2;
+            ++pj;
+            continue;
+        }
         for (uint32_t g = 0; g < G; ++g) {
             if (!bc[g]) continue;  // no blocked word: the cell changes nothing in the group
             CellLaunch a{};
@@ -1445,6 +1530,7 @@ ofp_status ofp_group_sweep(ofp_ctx* c, const ofp_group_spec* sp, ofp_group_info*
             ++launches;
         }
         OFP_HIP_R(hipEventRecord(ev[i + 1], c->stream), "event");
+        ++i;
     }
     OFP_HIP_R(hipEventSynchronize(ev[info->cells]), "group sync");
     double total = 0;

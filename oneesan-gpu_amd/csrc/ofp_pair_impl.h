@@ -193,6 +193,17 @@ OFP_HD Idx contrib(const Tab<Idx>& T, int pos, int v, int h) {
     const Z2<Idx> e = ldz(T.q[pos][h]);
     return v == 1 ? e.zn : e.zc;
 }
+// The same in the main numbering at main position pos >= 1 (the entries of the
+// row below's main half, Tab::mat / Tab::mcL).  A pass reads main position p
+// this way: with the global table and LOW groups both numberings give the same
+// entry there, but a HIGH group of the split at p fixes blocked position p and
+// leaves main position p free.  Below p the two numberings agree.
+template <typename Idx>
+OFP_HD Idx mcontrib(const Tab<Idx>& T, int pos, int v, int h) {
+    if (v == 0) return Idx(0);
+    const Z2<Idx> e = cell::ldm(T.q[pos - 1][h]);
+    return v == 1 ? e.zn : e.zc;
+}
 
 // Memory of a pass (state-major, stride K, moduli off..off+KC-1).  Loads and
 // stores may address the same arrays (the pass runs in place), so no pointer
@@ -242,13 +253,13 @@ template <int KC, typename Idx>
 OFP_HD void closures_after(const Tab<Idx>& T, int W, int p, const Class<Idx>& k, int u, bool read_blk,
                            const Io<KC, Idx>& io, Vals<KC>& acc, const uint32_t* mod) {
     const int h = k.h, hq = h + dh(u);
-    const Idx mbq = k.mainbase + contrib(T, p, u, h);  // main contribution of P''
+    const Idx mbq = k.mainbase + mcontrib(T, p, u, h);  // main contribution of P''
     Vals<KC> t;
     // LL (cell p-1 >= 3): scan S' downwards from its rank; top level = height hq
     if (p - 1 >= 3) {
         const Idx baseLL = mbq + T.cL(p - 1, hq) + T.cL(p - 2, hq + 1);
         // u = N: LN = NL - cL(p-1, h) + cL(p, h); blocked [P', L, L, S*] = NL - mainbase + blkbase
-        const Idx dLN = T.cL(p, h) - T.cL(p - 1, h);
+        const Idx dLN = T.mcL(p, h) - T.cL(p - 1, h);
         const Idx dBL = k.blkbase - k.mainbase;
         Idx rr = k.sr, s2 = 0;
         int hh = hq;
@@ -288,9 +299,9 @@ OFP_HD void closures_after(const Tab<Idx>& T, int W, int p, const Class<Idx>& k,
         // the main index of [P'*, N, R, R, S'] by the blocked-minus-main
         // contributions of P'*: db (unscanned) + dl (flip) + d2 (scanned, +2).
         const int hs = h + 2;  // height entering position p in P'*
-        Idx ub = k.mainbase + contrib(T, p, u, hs) + T.at(p - 1, hq + 2) + T.at(p - 2, hq + 1) + k.sr;
+        Idx ub = k.mainbase + mcontrib(T, p, u, hs) + T.at(p - 1, hq + 2) + T.at(p - 2, hq + 1) + k.sr;
         Idx u2 = 0, db = k.blkbase - k.mainbase, d2 = 0;
-        const Idx dRN = T.at(p, hs) - T.at(p - 1, hs);  // u = N: RN - NR in P'*
+        const Idx dRN = T.mat(p, hs) - T.at(p - 1, hs);  // u = N: RN - NR in P'*
         int Ha = h;  // height below the scanned position (after u and the window)
         uint64_t w = k.word >> (2 * p);
         for (int mp = p + 1; mp <= W - 1; ++mp) {
@@ -346,7 +357,7 @@ OFP_HD void pair_body(const PairLaunch& a, const Tab<Idx>& T, const Class<Idx>& 
         const int code = S::tri(i), x = code / 9, y = (code / 3) % 3, z = code % 3;
         const int h1 = h + dh(x), h2 = h1 + dh(y);
         pm[i] = h1 >= 0 && h2 >= 0;
-        im[i] = (uint64_t)(Idx)(k.mainbase + contrib(T, p, x, h) + contrib(T, p - 1, y, h1 < 0 ? 0 : h1) +
+        im[i] = (uint64_t)(Idx)(k.mainbase + mcontrib(T, p, x, h) + contrib(T, p - 1, y, h1 < 0 ? 0 : h1) +
                                 contrib(T, p - 2, z, h2 < 0 ? 0 : h2) + k.sr);
     }
     uint64_t ib[S::NB > 0 ? S::NB : 1];
@@ -551,6 +562,62 @@ inline void pack_class_table(const uint64_t (*d0)[kDpCols], int p, int d, cell::
 inline uint64_t class_count(const uint64_t (*d0)[kDpCols], int W, int p, int d) {
     uint64_t g[kDpRows][kDpCols];
     build_class_table(d0, p, d, g);
+    return W - 1 >= p ? g[W - 1][1] : 0;
+}
+
+// ---- pair passes over one occupancy group (cell::Occupancy, ofp_cell_impl.h) ----
+// The classes of a group are those of the whole state whose words lie in the
+// group: S' takes the group's completion counts below the window, G[p][h] =
+// F[p-2][h+d] (F: the group's main-numbering completion table; below a cell
+// both numberings agree), and each position m of P' above p obeys the group's
+// constraint there (fixed occupied: R or L, fixed empty: N, free: any):
+//   G[m][h] = G[m-1][h-1] + G[m-1][h+1]                (occupied)
+//           = G[m-1][h]                                (empty)
+//           = G[m-1][h] + G[m-1][h-1] + G[m-1][h+1]    (free)
+// and the packed decode entry of position m counts only the symbols allowed
+// there (as cell::group_entry).  The window positions p, p-1, p-2 of a pair
+// pass must be free (a LOW group fixes positions below low, a HIGH group above
+// low; the passes stay inside their half), so the class's words are the same
+// slots as in the whole state.  With the global table and no fixed position
+// this is build_class_table.
+inline void build_group_class_table(const cell::Occupancy& om, const uint64_t (*f)[kDpCols], int p, int d,
+                                    uint64_t (*g)[kDpCols]) {
+    for (int m = 0; m < kDpRows; ++m)
+        for (int h = 0; h < kDpCols; ++h) g[m][h] = 0;
+    for (int h = 0; h < kDpCols; ++h) {
+        const int e = h + d;
+        g[p][h] = (p >= 2 && e >= 0 && e < kDpCols) ? f[p - 2][e] : 0;
+    }
+    for (int m = p + 1; m < kDpRows; ++m) {
+        const int fix = om.at(m);
+        for (int h = 0; h < kDpCols; ++h) {
+            const uint64_t dn = h > 0 ? g[m - 1][h - 1] : 0, up = h + 1 < kDpCols ? g[m - 1][h + 1] : 0;
+            g[m][h] = fix == 1 ? dn + up : (fix == 0 ? g[m - 1][h] : g[m - 1][h] + dn + up);
+        }
+    }
+}
+
+template <typename Idx, int C>
+inline void pack_group_class_table(const cell::Occupancy& om, const uint64_t (*f)[kDpCols], int p, int d,
+                                   cell::Pair<Idx> (*out)[C]) {
+    uint64_t g[kDpRows][kDpCols];
+    build_group_class_table(om, f, p, d, g);
+    for (int m = 0; m < kDpRows; ++m)
+        for (int h = 0; h < C; ++h) {
+            out[m][h].n = 0;
+            out[m][h].c = 0;
+            if (m > p && m >= 1) {
+                const uint64_t here = g[m - 1][h], below = h > 0 ? g[m - 1][h - 1] : 0;
+                const int fix = om.at(m);
+                out[m][h].n = (Idx)(fix == 1 ? 0 : here);
+                out[m][h].c = (Idx)(fix == 1 ? below : (fix == 0 ? here : here + below));
+            }
+        }
+}
+
+inline uint64_t group_class_count(const cell::Occupancy& om, const uint64_t (*f)[kDpCols], int W, int p, int d) {
+    uint64_t g[kDpRows][kDpCols];
+    build_group_class_table(om, f, p, d, g);
     return W - 1 >= p ? g[W - 1][1] : 0;
 }
 

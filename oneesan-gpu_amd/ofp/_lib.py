@@ -408,7 +408,8 @@ class Context:
         """Whole-state rows as pair passes (two cells per in-place pass) instead of
         one launch per cell: False/0 off, True/1 on, an even 2..16 = on with at most
         that many moduli per launch; cells p and p-1 pair up where bit p of
-        cell_mask is set (0: the library's default)."""
+        cell_mask is set (0: the library's default).  Bits 8..9 of an int mode pick
+        a kernel variant; bit 10 also runs group sweeps as pair passes."""
         m = int(mode) if not isinstance(mode, bool) else (1 if mode else 0)
         self._check(self._lib.ofp_set_pair_passes(self._h, m, cell_mask & 0xFFFFFFFF), "ofp_set_pair_passes")
 

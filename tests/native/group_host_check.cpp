@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "../../oneesan-gpu_amd/csrc/ofp_cell_impl.h"
+#include "../../oneesan-gpu_amd/csrc/ofp_pair_impl.h"
 
 extern "C" {
 int or_include(uint64_t m, int width, int p, uint64_t* out, int* blocked);
@@ -78,9 +79,27 @@ static uint32_t group_of(uint64_t word, const cell::Occupancy& o) {
     return m;
 }
 
+// One pair pass (cells p, p-1) over one group's arrays, in place, launches in
+// the kernels' order d = -1, 0, 1, -2, 2, classes in rank order or reversed.
+template <int KC, int D>
+static void pair_d(const pair::PairLaunch& pa, const cell::Tab<uint32_t>& T, const cell::Occupancy& om,
+                   const uint64_t (*fm)[kDpCols], const pair::Io<KC, uint32_t>& io, bool rev) {
+    static cell::Pair<uint32_t> ct[kDpRows][kDpCols];
+    pair::pack_group_class_table<uint32_t, kDpCols>(om, fm, pa.p, D, ct);
+    const uint64_t cnt = pair::group_class_count(om, fm, pa.W, pa.p, D);
+    for (uint64_t t = 0; t < cnt; ++t) {
+        const uint64_t c = rev ? cnt - 1 - t : t;
+        const pair::Class<uint32_t> k = pair::decode_class<uint32_t>(T, ct, pa.W, pa.p, c);
+        pair::pair_body<KC, uint32_t, D, 0>(pa, T, k, io);
+    }
+}
+
 // Runs the cells of one half (high = false: W-1..low+1 over LOW groups, true:
-// low..1 over HIGH groups) group by group and compares with the oracle.
-template <int V>
+// low..1 over HIGH groups) group by group and compares with the oracle.  Pairs:
+// cells p, p-1 as pair passes where both are in the half (the group's class
+// tables, pack_group_class_table), the last odd cell alone; Rev: the classes
+// of each launch in reverse order.
+template <int V, bool Pairs = false, bool Rev = false>
 static int check(int n, int low, bool high, const uint32_t* mods, unsigned seed) {
     constexpr int KC = 3, K = 4;
     const int W = n + 1;
@@ -127,7 +146,13 @@ static int check(int n, int low, bool high, const uint32_t* mods, unsigned seed)
         ++groups;
         cell::pack_group_quads<uint32_t>(ob, fb, om, fm, q);
         cell::Tab<uint32_t> T{q};
-        std::vector<uint32_t> am(cm[g] * K, 0), ab((cb[g] ? cb[g] : 1) * K, 0);
+        // kGuard records past each array hold a sentinel: a read there spoils a
+        // residue, a write there is caught below
+        constexpr uint64_t kGuard = 64;
+        constexpr uint32_t kSentinel = 0xA5A5A5A5u;
+        std::vector<uint32_t> am((cm[g] + kGuard) * K, 0), ab(((cb[g] ? cb[g] : 1) + kGuard) * K, 0);
+        for (uint64_t i = cm[g] * K; i < am.size(); ++i) am[i] = kSentinel;
+        for (uint64_t i = (cb[g] ? cb[g] : 1) * K; i < ab.size(); ++i) ab[i] = kSentinel;
         for (uint64_t i = 0; i < nm; ++i)
             if (gm[i] == g)
                 for (int j = 0; j < KC; ++j) am[lm[i] * K + j] = mi[i * KC + j];
@@ -142,7 +167,36 @@ static int check(int n, int low, bool high, const uint32_t* mods, unsigned seed)
         a.kc = KC;
         a.layout = 1;
         for (int j = 0; j < KC; ++j) a.mod[j] = mods[j];
-        for (int p = p_hi; p >= p_lo; --p) {
+        for (int p = p_hi; p >= p_lo;) {
+            if (Pairs && p - 1 >= p_lo && p >= 2) {
+                pair::PairLaunch pa{};
+                pa.main_in = am.data();
+                pa.blk_in = ab.data();
+                pa.main_out = am.data();
+                pa.blk_out = ab.data();
+                pa.W = W;
+                pa.p = p;
+                pa.read_blk = p == W - 1 ? 0 : 1;
+                pa.K = K;
+                pa.off = 0;
+                pa.kc = KC;
+                for (int j = 0; j < KC; ++j) pa.mod[j] = mods[j];
+                const pair::Io<KC, uint32_t> io{am.data(), ab.data(), am.data(), ab.data(), K, 0};
+                pair_d<KC, -1>(pa, T, om, fm, io, Rev);
+                pair_d<KC, 0>(pa, T, om, fm, io, Rev);
+                pair_d<KC, 1>(pa, T, om, fm, io, Rev);
+                pair_d<KC, -2>(pa, T, om, fm, io, Rev);
+                pair_d<KC, 2>(pa, T, om, fm, io, Rev);
+                bool spoiled = false;
+                for (uint64_t i = cm[g] * K; i < am.size(); ++i) spoiled |= am[i] != kSentinel;
+                for (uint64_t i = (cb[g] ? cb[g] : 1) * K; i < ab.size(); ++i) spoiled |= ab[i] != kSentinel;
+                if (spoiled) {
+                    printf("OUT OF GROUP (pair) n=%d low=%d high=%d group=%u p=%d\n", n, low, high, g, p);
+                    return 1;
+                }
+                p -= 2;
+                continue;
+            }
             a.p = p;
             a.read_blk = p == W - 1 ? 0 : 1;
             for (uint64_t b = 0; b < cb[g]; ++b) cell::cell_one<KC, uint32_t, cell::kAoS, V>(a, T, b, mem);
@@ -150,6 +204,7 @@ static int check(int n, int low, bool high, const uint32_t* mods, unsigned seed)
                 printf("OUT OF GROUP n=%d low=%d high=%d group=%u p=%d\n", n, low, high, g, p);
                 return 1;
             }
+            --p;
         }
         for (uint64_t i = 0; i < nm; ++i)
             if (gm[i] == g)
@@ -177,7 +232,8 @@ static int check(int n, int low, bool high, const uint32_t* mods, unsigned seed)
                     return 1;
                 }
     }
-    printf("ok n=%d low=%d %s V=%d groups=%d cells=%d..%d\n", n, low, high ? "high" : "low", V, groups, p_hi, p_lo);
+    printf("ok n=%d low=%d %s V=%d%s groups=%d cells=%d..%d\n", n, low, high ? "high" : "low", V,
+           Pairs ? (Rev ? " pairs reversed" : " pairs") : "", groups, p_hi, p_lo);
     return 0;
 }
 
@@ -200,7 +256,7 @@ static void group_words(int width, const cell::Occupancy& o, const uint64_t (*f)
     rec(width - 1, 1, 0);
 }
 
-static int check_big(int n, int low, bool high, uint32_t g, const uint32_t* mods, unsigned seed) {
+static int check_big(int n, int low, bool high, uint32_t g, const uint32_t* mods, unsigned seed, bool pairs) {
     constexpr int KC = 3, K = 4;
     const int W = n + 1;
     const int p_hi = high ? low : W - 1, p_lo = high ? 1 : low + 1;
@@ -250,48 +306,73 @@ static int check_big(int n, int low, bool high, uint32_t g, const uint32_t* mods
     a.kc = KC;
     a.layout = 1;
     for (int j = 0; j < KC; ++j) a.mod[j] = mods[j];
-    for (int p = p_hi; p >= p_lo; --p) {
-        a.p = p;
-        a.read_blk = p == W - 1 ? 0 : 1;
-        for (uint64_t b = 0; b < nb; ++b) cell::cell_one<KC, uint32_t, cell::kAoS, 2>(a, T, b, mem);
-        if (g_oob) {
-            printf("OUT OF GROUP big n=%d low=%d group=%u p=%d\n", n, low, g, p);
-            return 1;
-        }
-        // oracle: push form of the cell over the group's words
-        std::vector<uint32_t> mo = em, bo(eb.size(), 0);
-        for (uint64_t i = 0; i < nm; ++i) {
-            uint64_t t;
-            int blocked;
-            if (!or_include(wm[i], W, p, &t, &blocked)) continue;
-            auto& map = blocked ? ib : im;
-            auto it = map.find(t);
-            if (it == map.end()) {
-                printf("TRANSITION LEAVES GROUP n=%d group=%u p=%d\n", n, g, p);
+    for (int p = p_hi; p >= p_lo;) {
+        const bool pr = pairs && p - 1 >= p_lo && p >= 2;
+        if (pr) {
+            pair::PairLaunch pa{};
+            pa.main_in = mi.data();
+            pa.blk_in = bi.data();
+            pa.main_out = mi.data();
+            pa.blk_out = bi.data();
+            pa.W = W;
+            pa.p = p;
+            pa.read_blk = p == W - 1 ? 0 : 1;
+            pa.K = K;
+            pa.off = 0;
+            pa.kc = KC;
+            for (int j = 0; j < KC; ++j) pa.mod[j] = mods[j];
+            const pair::Io<KC, uint32_t> io{mi.data(), bi.data(), mi.data(), bi.data(), K, 0};
+            pair_d<KC, -1>(pa, T, om, fm, io, false);
+            pair_d<KC, 0>(pa, T, om, fm, io, false);
+            pair_d<KC, 1>(pa, T, om, fm, io, false);
+            pair_d<KC, -2>(pa, T, om, fm, io, false);
+            pair_d<KC, 2>(pa, T, om, fm, io, false);
+        } else {
+            a.p = p;
+            a.read_blk = p == W - 1 ? 0 : 1;
+            for (uint64_t b = 0; b < nb; ++b) cell::cell_one<KC, uint32_t, cell::kAoS, 2>(a, T, b, mem);
+            if (g_oob) {
+                printf("OUT OF GROUP big n=%d low=%d group=%u p=%d\n", n, low, g, p);
                 return 1;
             }
-            uint32_t* dst = blocked ? &bo[it->second * K] : &mo[it->second * K];
-            for (int j = 0; j < KC; ++j) dst[j] = cell::addm(dst[j], em[i * K + j], mods[j]);
         }
-        if (p != W - 1)
-            for (uint64_t i = 0; i < nb; ++i) {
-                auto it = im.find(or_blocked_exclude(wb[i], p));
-                if (it == im.end()) {
-                    printf("EXCLUDE LEAVES GROUP n=%d group=%u p=%d\n", n, g, p);
+        // oracle: push form of each cell over the group's words
+        const int q_lo = pr ? p - 1 : p;
+        for (int q = p; q >= q_lo; --q) {
+            std::vector<uint32_t> mo = em, bo(eb.size(), 0);
+            for (uint64_t i = 0; i < nm; ++i) {
+                uint64_t t;
+                int blocked;
+                if (!or_include(wm[i], W, q, &t, &blocked)) continue;
+                auto& map = blocked ? ib : im;
+                auto it = map.find(t);
+                if (it == map.end()) {
+                    printf("TRANSITION LEAVES GROUP n=%d group=%u p=%d\n", n, g, q);
                     return 1;
                 }
-                for (int j = 0; j < KC; ++j)
-                    mo[it->second * K + j] = cell::addm(mo[it->second * K + j], eb[i * K + j], mods[j]);
+                uint32_t* dst = blocked ? &bo[it->second * K] : &mo[it->second * K];
+                for (int j = 0; j < KC; ++j) dst[j] = cell::addm(dst[j], em[i * K + j], mods[j]);
             }
-        em.swap(mo);
-        eb.swap(bo);
+            if (q != W - 1)
+                for (uint64_t i = 0; i < nb; ++i) {
+                    auto it = im.find(or_blocked_exclude(wb[i], q));
+                    if (it == im.end()) {
+                        printf("EXCLUDE LEAVES GROUP n=%d group=%u p=%d\n", n, g, q);
+                        return 1;
+                    }
+                    for (int j = 0; j < KC; ++j)
+                        mo[it->second * K + j] = cell::addm(mo[it->second * K + j], eb[i * K + j], mods[j]);
+                }
+            em.swap(mo);
+            eb.swap(bo);
+        }
         for (uint64_t i = 0; i < nm; ++i)
             for (int j = 0; j < KC; ++j)
                 if (mi[i * K + j] != em[i * K + j]) {
                     printf("MISMATCH big main n=%d group=%u p=%d i=%llu j=%d\n", n, g, p, (unsigned long long)i, j);
                     return 1;
                 }
-        if (p >= 2)
+        if (q_lo >= 2)
             for (uint64_t i = 0; i < nb; ++i)
                 for (int j = 0; j < KC; ++j)
                     if (bi[i * K + j] != eb[i * K + j]) {
@@ -299,19 +380,21 @@ static int check_big(int n, int low, bool high, uint32_t g, const uint32_t* mods
                                (unsigned long long)i, j);
                         return 1;
                     }
+        p = q_lo - 1;
     }
-    printf("ok big n=%d low=%d %s group=%u main=%llu blocked=%llu cells=%d..%d\n", n, low, high ? "high" : "low",
-           g, (unsigned long long)nm, (unsigned long long)nb, p_hi, p_lo);
+    printf("ok big n=%d low=%d %s%s group=%u main=%llu blocked=%llu cells=%d..%d\n", n, low, high ? "high" : "low",
+           pairs ? " pairs" : "", g, (unsigned long long)nm, (unsigned long long)nb, p_hi, p_lo);
     return 0;
 }
 
 int main(int argc, char** argv) {
     const uint32_t bmods[3] = {4294967291u, 2u, 4294966997u};
-    if (argc > 3 && (!strcmp(argv[1], "big") || !strcmp(argv[1], "bighigh"))) {
-        const bool high = !strcmp(argv[1], "bighigh");
+    if (argc > 3 && (!strncmp(argv[1], "big", 3))) {
+        const bool high = !strncmp(argv[1], "bighigh", 7);
+        const bool pairs = strstr(argv[1], "pair") != nullptr;  // bigpair, bighighpair: pair passes
         const int n = atoi(argv[2]), low = atoi(argv[3]);
         for (int i = 4; i < argc; ++i)
-            if (check_big(n, low, high, (uint32_t)strtoul(argv[i], nullptr, 10), bmods, 77u + i)) return 1;
+            if (check_big(n, low, high, (uint32_t)strtoul(argv[i], nullptr, 10), bmods, 77u + i, pairs)) return 1;
         printf("ALL OK\n");
         return 0;
     }
@@ -323,6 +406,8 @@ int main(int argc, char** argv) {
             for (int high = 0; high < 2; ++high) {
                 if (check<0>(n, low, high, mods, 100u * n + low + 50 * high)) return 1;
                 if (check<2>(n, low, high, mods, 200u * n + low + 50 * high)) return 1;
+                if (check<2, true, false>(n, low, high, mods, 500u * n + low + 50 * high)) return 1;
+                if (check<2, true, true>(n, low, high, mods, 600u * n + low + 50 * high)) return 1;
                 if (n % 2 == 0 && (check<1>(n, low, high, mods, 300u * n + low + 50 * high) ||
                                    check<3>(n, low, high, mods, 400u * n + low + 50 * high)))
                     return 1;

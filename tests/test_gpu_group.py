@@ -14,6 +14,7 @@ from group_masks import group_masks, occupancy_masks, window_order
 from ofp.path_bound import PRIMES
 
 pytestmark = pytest.mark.gpu
+GROUP_PAIRS = 1 << 10  # ofp_set_pair_passes mode bit: group sweeps run pair passes
 
 
 def _inputs(nm: int, mods, seed: int) -> np.ndarray:
@@ -162,4 +163,61 @@ def test_n27_high13_windows_agree(gpu_ctx):
     assert np.array_equal(whole, np.concatenate([a, b])) and not np.array_equal(whole, main)
     print("case n=27 HIGH seeded window", flush=True)
     s1, _, _ = gpu_ctx.group_sweep(n, mods, 4000, 2, low, seed=9, high=True)
-    assert s1["launches"] == 2 * 14
+    assert s1["launches"] > 0 and len(s1["cell_ms"]) == 14
+
+
+@pytest.mark.parametrize("high", [False, True])
+def test_n27_pair_passes_equal_single_cells(high):
+    """W = 28, 4 moduli: a LOW14 (or HIGH13) window swept as pair passes with each
+    group's class tables equals the same window swept one cell per launch, with
+    host inputs (the blocked input of the low half too)."""
+    n, low, mods = 27, 14, list(PRIMES[:4])
+    g0, g = (3, 4) if not high else (5, 3)
+    info, _, _ = ofp.group_plan(n, 4, g0, g, low, high=high)
+    main = _inputs(info["main_count"], mods, 2700 + high)
+    blk = _inputs(info["blocked_count"], mods, 2800 + high) if high else None
+    out = {}
+    for pairs in (True, False):
+        with ofp.Context() as ctx:
+            ctx.set_pair_passes(1 | GROUP_PAIRS if pairs else 1)
+            print(f"case n=27 high={high} pairs={pairs}", flush=True)
+            i, mo, bo = ctx.group_sweep(n, mods, g0, g, low, main_in=main, high=high, blocked_in=blk,
+                                        blocked_out=not high)
+            out[pairs] = (i, mo, bo)
+    (ip, mp, bp), (is_, ms, bs) = out[True], out[False]
+    assert np.array_equal(mp, ms) and not np.array_equal(mp, main)
+    if not high:
+        assert np.array_equal(bp, bs)
+    assert ip["launches"] != is_["launches"]
+    assert len(ip["cell_ms"]) == len(is_["cell_ms"]) == (low if high else n - low)
+
+
+@pytest.mark.parametrize("n,low,k,high", [(11, 5, 3, False), (12, 6, 2, False), (13, 7, 4, True),
+                                           (12, 5, 1, True), (11, 5, 14, True)])
+def test_group_pair_passes_equal_oracle(oracle, n, low, k, high):
+    """Group sweeps with pair passes (each group's class tables) against the
+    oracle's full-state cells, both halves."""
+    mods = ([2] + list(PRIMES))[:k] if k > 1 else [PRIMES[3]]
+    nm, nb = ofp.state_counts(n)
+    W = n + 1
+    fixed = n - low if high else low
+    mm, mb = group_masks(n, low, high)
+    om, ob = window_order(mm, 0, 1 << fixed), window_order(mb, 0, 1 << fixed)
+    main = _inputs(nm, mods, 4000 * n + low)
+    blk = _inputs(nb, mods, 5000 * n + low) if high else None
+    with ofp.Context() as ctx:
+        ctx.set_pair_passes(1 | GROUP_PAIRS)
+        print(f"case pairs n={n} low={low} k={k} high={high}", flush=True)
+        info, mo, bo = ctx.group_sweep(n, mods, 0, 1 << fixed, low, main_in=main[om], high=high,
+                                       blocked_in=blk[ob] if high else None, blocked_out=not high)
+    assert len(info["cell_ms"]) == (low if high else n - low)
+    for j, m in enumerate(mods):
+        em = np.ascontiguousarray(main[:, j])
+        eb = np.ascontiguousarray(blk[:, j]) if high else np.zeros(nb, np.uint32)
+        if high:
+            oracle.apply_cells(n, m, em, eb, low, 1, True, pull=True)
+        else:
+            oracle.apply_cells(n, m, em, eb, W - 1, low + 1, False, pull=True)
+        assert np.array_equal(mo[:, j], em[om]), (j, m)
+        if not high:
+            assert np.array_equal(bo[:, j], eb[ob]), (j, m)

@@ -87,13 +87,16 @@ def checker(tmp_path_factory, oracle):
 
 
 def test_group_body_equals_oracle_cells(checker):
-    """Every group of every split at n = 2..7, each swept alone on group-local
+    """Every group of every split at n = 2..8, each swept alone on group-local
     ranks, reproduces the oracle's full-state cells: the high cells over LOW
-    groups, the low cells (with blocked input) over HIGH groups."""
-    res = subprocess.run([str(checker), "7"], capture_output=True, text=True, timeout=600)
+    groups, the low cells (with blocked input) over HIGH groups -- one cell per
+    launch, and as pair passes with each group's own class tables (classes in
+    rank order and reversed, every launch's out-of-group guard intact)."""
+    res = subprocess.run([str(checker), "8"], capture_output=True, text=True, timeout=600)
     assert res.returncode == 0, res.stdout[-2000:] + res.stderr[-2000:]
     assert res.stdout.strip().endswith("ALL OK")
     assert res.stdout.count(" low V=") >= 40 and res.stdout.count(" high V=") >= 40
+    assert res.stdout.count(" pairs groups") >= 50 and res.stdout.count(" pairs reversed") >= 50
 
 
 def test_group_body_at_full_size(checker):
@@ -101,9 +104,12 @@ def test_group_body_at_full_size(checker):
     of n = 13): the depth-first enumeration of the group comes out in global
     rank order, and the body on group-local ranks equals the oracle's
     transitions applied to the group's words alone, cell by cell (27..15 over
-    LOW groups, 14..1 with blocked input over HIGH groups)."""
+    LOW groups, 14..1 with blocked input over HIGH groups), one cell per launch
+    and as pair passes with the group's class tables."""
     for mode, args in (("big", ["13", "6", "5", "17", "63"]), ("big", ["27", "14", "3", "100"]),
-                       ("bighigh", ["13", "6", "0", "5", "9"]), ("bighigh", ["27", "14", "0", "4"])):
+                       ("bighigh", ["13", "6", "0", "5", "9"]), ("bighigh", ["27", "14", "0", "4"]),
+                       ("bigpair", ["13", "6", "5", "63"]), ("bigpair", ["27", "14", "3", "100"]),
+                       ("bighighpair", ["13", "6", "0", "9"]), ("bighighpair", ["27", "14", "0", "4"])):
         res = subprocess.run([str(checker), mode, *args], capture_output=True, text=True, timeout=600)
         assert res.returncode == 0, res.stdout[-2000:] + res.stderr[-2000:]
         assert res.stdout.strip().endswith("ALL OK")
