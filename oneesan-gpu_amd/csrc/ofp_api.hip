@@ -583,6 +583,30 @@ bool row_capped(const ofp_ctx* c, const Sweep& s, int r) {
     return false;
 }
 
+// A row with capped cells (row r of a sweep from the initial state): each capped
+// cell alone, each run of uncapped cells as full-state cells, paired as in a
+// full row where pair passes apply.  (A pair pass is two consecutive full-state
+// cells, so the row computes what one launch per cell computes.)
+ofp_status enqueue_capped_row(ofp_ctx* c, Sweep& s, int r, int* launches) {
+    if (!pair_ready(c, s)) return enqueue_cells(c, s, s.W - 1, 1, false, launches, r);
+    const int pct = cap_pct(c, s);
+    for (int p = s.W - 1; p >= 1;) {
+        const bool reads_blk = p != s.W - 1;
+        if (effective_cap(s.W, r, p, s.nblk, nullptr, pct)) {
+            ofp_status st = enqueue_cells(c, s, p, p, reads_blk, launches, r);
+            if (st) return st;
+            --p;
+            continue;
+        }
+        int q = p;
+        while (q - 1 >= 1 && !effective_cap(s.W, r, q - 1, s.nblk, nullptr, pct)) --q;
+        ofp_status st = enqueue_pairs(c, s, p, q, reads_blk, launches);
+        if (st) return st;
+        p = q - 1;
+    }
+    return OFP_OK;
+}
+
 // Kernel time split between rows that run only uncapped cells (the full-state
 // kernel, replayed from the row graph) and rows with capped cells.
 struct RowTimes {
@@ -620,7 +644,7 @@ ofp_status run_rows(ofp_ctx* c, Sweep& s, int rows, int* launches, RowTimes* tim
             OFP_HIP(c, hipEventRecord(e, s.stream), "event record");
         }
         int l = 0;
-        ofp_status st = capped ? enqueue_cells(c, s, s.W - 1, 1, false, &l, r) : full_rows(c, s, n, &l);
+        ofp_status st = capped ? enqueue_capped_row(c, s, r, &l) : full_rows(c, s, n, &l);
         if (st) return st;
         if (times) {
             hipEvent_t e = event(2 * segs.size() + 1);
