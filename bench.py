@@ -168,17 +168,20 @@ def cpu_baseline(n: int, seconds: float) -> dict:
             "residues_per_hour_extrapolated": (3600.0 * rate / (W * (W - 1) * (nm + nb))) if rate else None}
 
 
-def cpu_solve(n: int, mod: int, repeats: int) -> dict:
-    """The CPU oracle's whole solve (pull form, one modulus): one warm-up and the
-    median of `repeats` when repeats > 1, else one timed solve."""
+def cpu_solve(n: int, mod: int, repeats: int, warmup: bool = True, progress: bool = False) -> dict:
+    """The CPU oracle's whole solve (pull form, one modulus): one warm-up (if
+    warmup) and the median of `repeats` when repeats > 1, else one timed solve;
+    progress: a log line after each solve."""
     from oracle import binding
-    if repeats > 1:
+    if repeats > 1 and warmup:
         binding.solve(n, mod)
     times, res = [], None
-    for _ in range(repeats):
+    for i in range(repeats):
         t0 = time.perf_counter()
         res = binding.solve(n, mod)
         times.append(time.perf_counter() - t0)
+        if progress:
+            log(f"bench: cpu n={n} solve {i + 1}/{repeats}: {times[-1]:.1f} s")
     med = sorted(times)[len(times) // 2]
     W = n + 1
     lib = binding.load()
@@ -320,6 +323,8 @@ def main() -> int:
     ap.add_argument("--k", type=int, default=0, help="moduli (0 = as many as the CRT bound needs)")
     ap.add_argument("--no-plain", action="store_true", help="skip the k=1 plain-form solve")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-n18", action="store_true",
+                    help="also time the CPU oracle's whole n=18 solve (median of three; a few minutes)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-shard", action="store_true", help="skip the n=27 one-GPU-share measurement")
     ap.add_argument("--no-small", action="store_true",
@@ -493,8 +498,13 @@ def main() -> int:
         log(f"bench: cpu baseline {cpu['value']:.3e} transitions/s on {cpu['cores']} threads")
         cpu["n12"] = cpu_solve(12, PRIMES[0], 3)
         cpu["n16"] = cpu_solve(16, PRIMES[0], 1)
+        if args.cpu_n18:
+            cpu["n18"] = cpu_solve(18, PRIMES[0], 3, warmup=False, progress=True)
+            log(f"bench: cpu n=18 solve {cpu['n18']['solve_s']:.1f} s (median of 3), residue {cpu['n18']['residue']}")
         if small:
             cpu["n12"]["residue_matches_gpu"] = cpu["n12"]["residue"] == small["n12"]["residue"]
+            if "n18" in cpu:
+                cpu["n18"]["residue_matches_gpu"] = cpu["n18"]["residue"] == small["n18"]["residue"]
 
     out = {
         "metric": "frontier transitions/s & residues/hour, batched vs 1-residue, n=22, bit-exact",
