@@ -1697,6 +1697,27 @@ ofp_status ofp_group_exchange(ofp_ctx* c, const ofp_group_spec* sp, int shares, 
     std::vector<XTile> tm, tb;
     window_tiles(lw.mc, tm);
     window_tiles(lw.bc, tb);
+    // the window's groups' bottom tables: main words below position low+1,
+    // blocked words below position low
+    std::vector<uint32_t> lut_m, lut_b, off_m((size_t)lw.groups * (kDpCols + 1)),
+        off_b((size_t)lw.groups * (kDpCols + 1));
+    {
+        std::vector<uint32_t> part;
+        uint64_t f[kDpRows][kDpCols];
+        for (uint32_t g = 0; g < lw.groups; ++g) {
+            cell::Occupancy om, ob;
+            cell::group_occupancy(false, W, L, lw.first + g, om, ob);
+            cell::build_group_table(om, f);
+            for (int kind = 0; kind < 2; ++kind) {
+                std::vector<uint32_t>& lut = kind == 0 ? lut_m : lut_b;
+                uint32_t* off = (kind == 0 ? off_m : off_b).data() + (size_t)g * (kDpCols + 1);
+                xchg::build_bottom_lut(om, f, d, kind == 0 ? L + 1 : L, part, off);
+                const uint32_t at = (uint32_t)lut.size();
+                for (int h = 0; h <= kDpCols; ++h) off[h] += at;
+                lut.insert(lut.end(), part.begin(), part.end());
+            }
+        }
+    }
     uint64_t M = 0, B = 0;
     for (uint64_t x : lw.mc) M += x;
     for (uint64_t x : lw.bc) B += x;
@@ -1711,10 +1732,12 @@ ofp_status ofp_group_exchange(ofp_ctx* c, const ofp_group_spec* sp, int shares, 
     const size_t b_om = al(std::max<size_t>(tm.size(), 1) * shares * 8),
                  b_ob = al(std::max<size_t>(tb.size(), 1) * shares * 8);
     const size_t b_tot = al(2 * kMaxShares * 8);
+    const size_t b_lm = al(std::max<size_t>(lut_m.size(), 1) * 4), b_lb = al(std::max<size_t>(lut_b.size(), 1) * 4);
+    const size_t b_off = al(off_m.size() * 4);
     const size_t b_im = al(std::max<uint64_t>(M, 1) * 8), b_ib = al(std::max<uint64_t>(B, 1) * 8);
     const size_t b_rm = al(std::max<uint64_t>(M, 1) * Kp * 4), b_rb = al(std::max<uint64_t>(B, 1) * Kp * 4);
-    const size_t need = 2 * b_ht + 2 * b_base + b_tm + b_tb + b_cm + b_cb + b_om + b_ob + b_tot + b_im + b_ib +
-                        b_rm + b_rb;
+    const size_t need = 2 * b_ht + 2 * b_base + b_tm + b_tb + b_cm + b_cb + b_om + b_ob + b_tot + b_lm + b_lb +
+                        2 * b_off + b_im + b_ib + b_rm + b_rb;
     char* buf = nullptr;
     st = stage_buffer(c, need, &buf);
     if (st) return st;
@@ -1730,6 +1753,10 @@ ofp_status ofp_group_exchange(ofp_ctx* c, const ofp_group_spec* sp, int shares, 
     uint64_t* d_om = cv.take<uint64_t>(b_om);
     uint64_t* d_ob = cv.take<uint64_t>(b_ob);
     uint64_t* d_tot = cv.take<uint64_t>(b_tot);
+    uint32_t* d_lm = cv.take<uint32_t>(b_lm);
+    uint32_t* d_lb = cv.take<uint32_t>(b_lb);
+    uint32_t* d_offm = cv.take<uint32_t>(b_off);
+    uint32_t* d_offb = cv.take<uint32_t>(b_off);
     uint64_t* d_im = cv.take<uint64_t>(b_im);
     uint64_t* d_ib = cv.take<uint64_t>(b_ib);
     uint32_t* d_rm = cv.take<uint32_t>(b_rm);
@@ -1738,6 +1765,10 @@ ofp_status ofp_group_exchange(ofp_ctx* c, const ofp_group_spec* sp, int shares, 
     OFP_HIP(c, hipMemcpy(d_htb, ht_b.data(), ht_b.size() * 4, hipMemcpyHostToDevice), "upload group rows");
     OFP_HIP(c, hipMemcpy(d_bm, base_m.data(), (size_t)NM * 8, hipMemcpyHostToDevice), "upload bases");
     OFP_HIP(c, hipMemcpy(d_bb, base_b.data(), (size_t)NM * 8, hipMemcpyHostToDevice), "upload bases");
+    if (!lut_m.empty()) OFP_HIP(c, hipMemcpy(d_lm, lut_m.data(), lut_m.size() * 4, hipMemcpyHostToDevice), "upload bottoms");
+    if (!lut_b.empty()) OFP_HIP(c, hipMemcpy(d_lb, lut_b.data(), lut_b.size() * 4, hipMemcpyHostToDevice), "upload bottoms");
+    OFP_HIP(c, hipMemcpy(d_offm, off_m.data(), off_m.size() * 4, hipMemcpyHostToDevice), "upload bottom offsets");
+    OFP_HIP(c, hipMemcpy(d_offb, off_b.data(), off_b.size() * 4, hipMemcpyHostToDevice), "upload bottom offsets");
     if (!tm.empty())
         OFP_HIP(c, hipMemcpy(d_tm, tm.data(), tm.size() * sizeof(XTile), hipMemcpyHostToDevice), "upload tiles");
     if (!tb.empty())
@@ -1757,6 +1788,8 @@ ofp_status ofp_group_exchange(ofp_ctx* c, const ofp_group_spec* sp, int shares, 
         a.ht = main_kind ? d_htm : d_htb;
         a.dplain = c->dp32;
         a.base_of = main_kind ? d_bm : d_bb;
+        a.lut = main_kind ? d_lm : d_lb;
+        a.lut_off = main_kind ? d_offm : d_offb;
         for (int s = 0; s <= shares; ++s) a.first[s] = first[s];
         a.shares = shares;
         a.counts = main_kind ? d_cm : d_cb;

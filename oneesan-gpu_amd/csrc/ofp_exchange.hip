@@ -10,16 +10,21 @@
 // L-major placements, docs/b300-bucket-snake.md:7-20).
 //
 // Three launches per kind of word (main, blocked), one workgroup per tile of
-// at most kXTileWords words of one group:
-//   count  -- decode the fixed positions of each word and count its
-//             destination share (one wave64 ballot per share and round);
+// at most kXTileWords words of one group, each round of a wave taking 64
+// consecutive local ranks.  Words with one top part (the fixed HIGH positions)
+// are a run of consecutive ranks -- a block, about a hundred words at W = 28 --
+// with one destination share and one top-part HIGH-rank contribution, so a
+// wave decodes top parts only in rounds that leave its current block
+// (xchg::decode_top) and takes each word's bottom contribution from the
+// group's bottom table (xchg::build_bottom_lut):
+//   count  -- count each word's destination share (one wave64 ballot per share
+//             and round);
 //   scan   -- one workgroup per share: exclusive prefix of the tile counts;
-//   write  -- decode each word whole, ranking it in its HIGH group on the way
-//             (xchg::decode_rank), and write its record.  Records keep the
-//             window order inside a bucket: each round of 256 words is
-//             compacted with one wave64 ballot per share (a lane's slot = the
-//             tile's cursor + the set lanes below it + the counts of the lower
-//             waves), so the buckets are deterministic.
+//   write  -- write each word's record.  Records keep the window order inside a
+//             bucket: each round of 256 words is compacted with one wave64
+//             ballot per share (a lane's slot = the tile's cursor + the set
+//             lanes below it + the counts of the lower waves), so the buckets
+//             are deterministic.
 // The residues are read once in window order (coalesced) and written once as
 // records; records of one share in one round are contiguous.
 #include "ofp_exchange_impl.h"
@@ -41,6 +46,50 @@ __device__ const Quad<uint32_t>* group_table(const XchgLaunch& a, uint32_t g) {
     return reinterpret_cast<const Quad<uint32_t>*>(a.gtab + (size_t)g * sizeof(Quad<uint32_t>) * kDpRows * kDpCols);
 }
 
+// A block: the consecutive local ranks whose words share one top part (the
+// fixed HIGH positions), hence one HIGH mask, destination share and top-part
+// HIGH-rank contribution; a word's bottom contributes lut[lut + r - start].
+struct Blk {
+    uint32_t start, len, mask, top, lut;
+    int share;
+};
+
+template <bool Main>
+__device__ Blk find_block(const XchgLaunch& a, const Quad<uint32_t> (*tq)[kDpCols], const uint32_t* off, uint32_t r) {
+    const xchg::Top t = xchg::decode_top<Main>(tq, a.top, a.base, r);
+    Blk b;
+    b.start = r - t.r_rest;
+    b.lut = off[t.h];
+    b.len = off[t.h + 1] - b.lut;
+    b.mask = t.mask;
+    b.top = xchg::top_high(t, a.ht);
+    b.share = xchg::share_of(t.mask, a.first, a.shares);
+    return b;
+}
+
+// Give every valid lane of the wave the block of its rank r.  `cb` is the
+// wave's current block (the same in every lane), kept from round to round:
+// rounds walk consecutive ranks, so in a large block every lane falls in it and
+// nothing is decoded.  Otherwise the lanes outside it decode their own top parts
+// side by side (one decode's time; blocks of a few words are common where the
+// group's occupied positions are few), and the block of the last valid lane,
+// which the next round continues, becomes the wave's current block.
+__device__ void assign_blocks(const XchgLaunch& a, const Quad<uint32_t> (*tq)[kDpCols], const uint32_t* off,
+                              bool valid, uint32_t r, Blk& cb, Blk& mine) {
+    const bool in = valid && (r - cb.start < cb.len);
+    mine = cb;
+    if (__ballot(valid && !in) == 0) return;
+    if (valid && !in) mine = a.main ? find_block<true>(a, tq, off, r) : find_block<false>(a, tq, off, r);
+    const uint64_t vm = __ballot(valid);
+    const int last = 63 - __clzll((long long)vm);
+    cb.start = (uint32_t)__shfl((int)mine.start, last);
+    cb.len = (uint32_t)__shfl((int)mine.len, last);
+    cb.mask = (uint32_t)__shfl((int)mine.mask, last);
+    cb.top = (uint32_t)__shfl((int)mine.top, last);
+    cb.lut = (uint32_t)__shfl((int)mine.lut, last);
+    cb.share = __shfl(mine.share, last);
+}
+
 __global__ __launch_bounds__(kXThreads) void xchg_count_kernel(XchgLaunch a) {
     __shared__ Quad<uint32_t> tq[kDpRows][kDpCols];
     __shared__ uint32_t cnt[kMaxShares];
@@ -48,23 +97,23 @@ __global__ __launch_bounds__(kXThreads) void xchg_count_kernel(XchgLaunch a) {
     load_quads(tq, group_table(a, t.group));
     if (threadIdx.x < kMaxShares) cnt[threadIdx.x] = 0;
     __syncthreads();
+    const uint32_t* off = a.lut_off + (size_t)t.group * (kDpCols + 1);
     // lane q of each wave counts share q
     const int lane = threadIdx.x & 63;
-    uint32_t mine = 0;
+    uint32_t mine_cnt = 0;
+    Blk cb{0u, 0u, 0u, 0u, 0u, 0};
     for (uint32_t r0 = 0; r0 < t.len; r0 += kXThreads) {
         const uint32_t i = r0 + threadIdx.x;
-        int s = -1;
-        if (i < t.len) {
-            const uint64_t w = a.main ? xchg::decode_group_word<true>(tq, a.top, a.base, t.rank0 + i)
-                                      : xchg::decode_group_word<false>(tq, a.top, a.base, t.rank0 + i);
-            s = xchg::share_of(xchg::high_mask(w, a.base, a.nf), a.first, a.shares);
-        }
+        const bool valid = i < t.len;
+        Blk mine;
+        assign_blocks(a, tq, off, valid, t.rank0 + i, cb, mine);
+        const int s = valid ? mine.share : -1;
         for (int q = 0; q < a.shares; ++q) {
             const uint32_t c = (uint32_t)__popcll(__ballot(s == q));
-            if (lane == q) mine += c;
+            if (lane == q) mine_cnt += c;
         }
     }
-    if (lane < a.shares) atomicAdd(&cnt[lane], mine);
+    if (lane < a.shares) atomicAdd(&cnt[lane], mine_cnt);
     __syncthreads();
     if ((int)threadIdx.x < a.shares) a.counts[(size_t)threadIdx.x * a.ntiles + blockIdx.x] = cnt[threadIdx.x];
 }
@@ -97,12 +146,12 @@ __global__ __launch_bounds__(1024) void xchg_scan_kernel(XchgLaunch a) {
 
 __global__ __launch_bounds__(kXThreads) void xchg_write_kernel(XchgLaunch a) {
     __shared__ Quad<uint32_t> tq[kDpRows][kDpCols];
-    __shared__ Quad<uint32_t> dq[kDpRows][kDpCols];
     __shared__ uint64_t cursor[kMaxShares];
     __shared__ uint32_t wcnt[kXWaves][kMaxShares];
     const XTile t = a.tiles[blockIdx.x];
     load_quads(tq, group_table(a, t.group));
-    load_quads(dq, reinterpret_cast<const Quad<uint32_t>*>(a.dplain));
+    const uint32_t* off = a.lut_off + (size_t)t.group * (kDpCols + 1);
+    Blk cb{0u, 0u, 0u, 0u, 0u, 0};
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (tid < a.shares) {
         uint64_t start = 0;
@@ -114,15 +163,14 @@ __global__ __launch_bounds__(kXThreads) void xchg_write_kernel(XchgLaunch a) {
     for (uint32_t r0 = 0; r0 < t.len; r0 += kXThreads) {
         const uint32_t i = r0 + tid;
         const bool valid = i < t.len;
+        const uint32_t r = t.rank0 + i;
+        Blk mine;
+        assign_blocks(a, tq, off, valid, r, cb, mine);
         int s = -1;
         uint64_t dst = 0;
         if (valid) {
-            uint64_t w;
-            uint32_t mask;
-            const uint32_t rank = a.main ? xchg::decode_rank<true>(tq, dq, a.ht, a.top, a.base, t.rank0 + i, w, mask)
-                                         : xchg::decode_rank<false>(tq, dq, a.ht, a.top, a.base, t.rank0 + i, w, mask);
-            s = xchg::share_of(mask, a.first, a.shares);
-            dst = a.base_of[mask] + rank;
+            s = mine.share;
+            dst = a.base_of[mine.mask] + (uint32_t)(mine.top + a.lut[mine.lut + (r - mine.start)]);
         }
         uint32_t pre = 0;
         for (int q = 0; q < a.shares; ++q) {

@@ -27,6 +27,7 @@
 // Ranks are summed mod 2^32: every HIGH group of the W = 28 split holds fewer
 // than 2^32 words, so the true rank comes out whatever the unused entries hold.
 #pragma once
+#include <functional>
 #include <vector>
 
 #include "ofp_cell_impl.h"
@@ -136,6 +137,85 @@ OFP_HD uint32_t decode_rank(const Quad<uint32_t> (*q)[kDpCols], const Quad<uint3
     word_out = word;
     mask_out = mask;
     return rank;
+}
+
+// The top part of a word alone: positions top..base decoded from local rank r
+// (the walk of decode_rank), giving the rank r_rest of the bottom part among the
+// group's bottoms entered at height h, the HIGH mask, and the fixed positions'
+// recorded L's (lmask, hpack).  Every word with the same top part lies in one
+// run of consecutive local ranks (a block: r - r_rest .. + bottoms(h) - 1) and
+// shares mask, share and the top part's HIGH-rank contribution (top_high).
+struct Top {
+    uint32_t r_rest, mask, lmask;
+    uint64_t hpack;
+    int h;
+};
+template <bool Main>
+OFP_HD Top decode_top(const Quad<uint32_t> (*q)[kDpCols], int top, int base, uint32_t r) {
+    Top t{};
+    int h = 1;
+    for (int pos = top; pos >= base; --pos) {
+        const Z2<uint32_t> e = (Main && pos >= 1) ? cell::ldm(q[pos - 1][h]) : cell::ldz(q[pos][h]);
+        const bool g0 = r >= e.zn, g1 = r >= e.zc;
+        r -= g1 ? e.zc : (g0 ? e.zn : 0u);
+        const int j = pos - base;
+        t.mask |= (uint32_t)g0 << j;
+        if (g1 && h >= 1) {
+            t.lmask |= 1u << j;
+            t.hpack |= (uint64_t)(h - 1) << (4 * j);
+        }
+        h += g1 ? 1 : (g0 ? -1 : 0);
+    }
+    t.r_rest = r;
+    t.h = h;
+    return t;
+}
+// The fixed positions' part of the HIGH rank (the recorded L's).
+OFP_HD uint32_t top_high(const Top& t, const uint32_t* ht) {
+    uint32_t rank = 0, lm = t.lmask;
+    while (lm) {
+        const int j = __builtin_ctz(lm);
+        lm &= lm - 1u;
+        rank += ht[(size_t)((1u << j) - 1u + (t.mask & ((1u << j) - 1u))) * kDpCols +
+                   (uint32_t)((t.hpack >> (4 * j)) & 15u)];
+    }
+    return rank;
+}
+
+// The bottoms of a LOW group (positions base-1..0, entered at height h, the
+// group's occupancy on its fixed positions) in the group's order, and for each
+// its rank contribution in the plain numbering (what a HIGH group, free below
+// its fixed block, counts for it): lut[off[h] + j] for the j-th bottom entered
+// at h; off has kDpCols + 1 entries, off[h+1] - off[h] = f[base][h] bottoms.
+inline void build_bottom_lut(const cell::Occupancy& o, const uint64_t (*f)[kDpCols], const uint64_t (*d)[kDpCols],
+                             int base, std::vector<uint32_t>& lut, uint32_t* off) {
+    lut.clear();
+    for (int h0 = 0; h0 < kDpCols; ++h0) {
+        off[h0] = (uint32_t)lut.size();
+        if (base == 0) {
+            if (h0 == 0) lut.push_back(0);
+            continue;
+        }
+        if (f[base][h0] == 0) continue;
+        // depth-first in order N < R < L at each position from base-1 down
+        std::function<void(int, int, uint32_t)> rec = [&](int pos, int h, uint32_t rank) {
+            if (pos < 0) {
+                if (h == 0) lut.push_back(rank);
+                return;
+            }
+            const int fix = o.at(pos);
+            for (int v = 0; v < 3; ++v) {
+                if (fix >= 0 && (v != 0) != (fix == 1)) continue;
+                const int nh = h + (v == 2) - (v == 1);
+                if (nh < 0 || nh >= kDpCols || f[pos][nh] == 0) continue;
+                const uint64_t here = d[pos][h], below = h > 0 ? d[pos][h - 1] : 0;
+                const uint32_t c = v == 0 ? 0u : (uint32_t)(v == 1 ? here : here + below);
+                rec(pos - 1, nh, rank + c);
+            }
+        };
+        rec(base - 1, h0, 0u);
+    }
+    off[kDpCols] = (uint32_t)lut.size();
 }
 
 // Share (destination shard) of a HIGH mask: shares own consecutive runs of

@@ -126,6 +126,8 @@ struct XchgLaunch {
     const uint32_t* ht;      // group-row table of this kind (ofp_exchange_impl.h)
     const void* dplain;      // plain packed 32-bit height table
     const uint64_t* base_of; // per HIGH mask: offset of its group in its share's placement
+    const uint32_t* lut;     // bottoms' plain-numbering rank contributions (xchg::build_bottom_lut), all groups
+    const uint32_t* lut_off; // per group of the window: kDpCols + 1 offsets into lut, by entry height
     uint32_t first[kMaxShares + 1];  // share s owns HIGH masks first[s]..first[s+1]-1
     int shares;
     uint32_t* counts;        // [shares][ntiles] words per destination share and tile

@@ -1,6 +1,7 @@
 // Host check of the exchange's index math (oneesan-gpu_amd/csrc/
 // ofp_exchange_impl.h): decode_group_word, high_mask, high_rank, the fused
-// decode_rank and build_high_rows, built for the CPU.
+// decode_rank, build_high_rows, and the block route (decode_top, top_high and
+// the group's bottom table from build_bottom_lut), built for the CPU.
 //
 // `exchange_host_check small NMAX`: for every n <= NMAX and every split low,
 // every main and blocked word is enumerated in global rank order (the oracle's
@@ -78,6 +79,18 @@ static void low_table(int W, int low, uint32_t m, cell::Quad<uint32_t> (*q)[kDpC
     cell::pack_group_quads<uint32_t>(ob, fb, om, fm, q);
 }
 
+// The block route (decode_top + top_high + the group's bottom table) must give
+// the rank decode_rank gives; the bottom table holds f[base][h] entries per h.
+static bool block_route_ok(const cell::Quad<uint32_t> (*q)[kDpCols], bool mainw, int top, int base, uint32_t lrank,
+                           const uint32_t* ht, const std::vector<uint32_t>& lut, const uint32_t* off,
+                           const uint64_t (*f)[kDpCols], uint32_t want_mask, uint32_t want_rank) {
+    const xchg::Top t = mainw ? xchg::decode_top<true>(q, top, base, lrank) : xchg::decode_top<false>(q, top, base, lrank);
+    if (t.h < 0 || t.h >= kDpCols) return false;
+    if ((uint64_t)(off[t.h + 1] - off[t.h]) != f[base][t.h] || t.r_rest >= off[t.h + 1] - off[t.h]) return false;
+    const uint32_t r = xchg::top_high(t, ht) + lut[off[t.h] + t.r_rest];
+    return t.mask == want_mask && r == want_rank;
+}
+
 static int small(int nmax) {
     int cases = 0;
     static cell::Quad<uint32_t> q[kDpRows][kDpCols];
@@ -102,11 +115,18 @@ static int small(int nmax) {
                     hrank[i] = highc[occ(words[i], base, nf)]++;
                 }
                 uint32_t cur = ~0u;
+                static uint64_t lf[kDpRows][kDpCols];
+                std::vector<uint32_t> lut;
+                uint32_t off[kDpCols + 1];
                 for (uint64_t i = 0; i < cnt; ++i) {
                     const uint64_t w = words[i];
                     const uint32_t lm = occ(w, 0, low);
                     if (lm != cur) {
                         low_table(W, low, lm, q);
+                        cell::Occupancy om, ob;
+                        cell::group_occupancy(false, W, low, lm, om, ob);
+                        cell::build_group_table(om, lf);
+                        xchg::build_bottom_lut(om, lf, g_d, base, lut, off);
                         cur = lm;
                     }
                     const uint64_t got = mainw ? xchg::decode_group_word<true>(q, top, 0, (uint32_t)lrank[i])
@@ -121,8 +141,10 @@ static int small(int nmax) {
                     const uint32_t fr =
                         mainw ? xchg::decode_rank<true>(q, g_plain, htm.data(), top, base, (uint32_t)lrank[i], fw, fm)
                               : xchg::decode_rank<false>(q, g_plain, htb.data(), top, base, (uint32_t)lrank[i], fw, fm);
+                    const bool blk_ok = block_route_ok(q, mainw, top, base, (uint32_t)lrank[i],
+                                                       mainw ? htm.data() : htb.data(), lut, off, lf, hm, hr);
                     if (got != w || xchg::high_mask(top_only, base, nf) != hm || hm != occ(w, base, nf) ||
-                        hr != hrank[i] || fw != w || fm != hm || fr != hr) {
+                        hr != hrank[i] || fw != w || fm != hm || fr != hr || !blk_ok) {
                         printf("MISMATCH n=%d low=%d kind=%s rank=%llu word=%llx decoded=%llx high mask %u/%u rank "
                                "%u/%llu\n",
                                n, low, mainw ? "main" : "blocked", (unsigned long long)i, (unsigned long long)w,
@@ -161,6 +183,11 @@ static int big(int n, int low, int samples, unsigned seed) {
             cell::build_group_table(mainw ? om : ob, f);
             const uint64_t lr = group_rank(w, top, mainw ? om : ob, f);
             low_table(W, low, lm, q);
+            static uint64_t lf[kDpRows][kDpCols];
+            std::vector<uint32_t> lut;
+            uint32_t off[kDpCols + 1];
+            cell::build_group_table(om, lf);
+            xchg::build_bottom_lut(om, lf, g_d, base, lut, off);
             const uint64_t got = mainw ? xchg::decode_group_word<true>(q, top, 0, (uint32_t)lr)
                                        : xchg::decode_group_word<false>(q, top, 0, (uint32_t)lr);
             // HIGH rank through the HIGH group's own table
@@ -173,7 +200,10 @@ static int big(int n, int low, int samples, unsigned seed) {
             uint32_t fm = 0;
             const uint32_t fr = mainw ? xchg::decode_rank<true>(q, g_plain, htm.data(), top, base, (uint32_t)lr, fw, fm)
                                       : xchg::decode_rank<false>(q, g_plain, htb.data(), top, base, (uint32_t)lr, fw, fm);
-            if (lr >> 32 || got != w || want >> 32 || hr != (uint32_t)want || fw != w || fm != hm || fr != hr) {
+            const bool blk_ok = block_route_ok(q, mainw, top, base, (uint32_t)lr, mainw ? htm.data() : htb.data(),
+                                               lut, off, lf, hm, hr);
+            if (lr >> 32 || got != w || want >> 32 || hr != (uint32_t)want || fw != w || fm != hm || fr != hr ||
+                !blk_ok) {
                 printf("MISMATCH big n=%d low=%d kind=%s word=%llx low rank %llu decoded %llx high rank %u/%llu\n",
                        n, low, mainw ? "main" : "blocked", (unsigned long long)w, (unsigned long long)lr,
                        (unsigned long long)got, hr, (unsigned long long)want);
