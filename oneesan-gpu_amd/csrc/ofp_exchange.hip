@@ -36,6 +36,7 @@ namespace {
 using cell::Quad;
 constexpr int kXThreads = 256;
 constexpr int kXWaves = kXThreads / 64;
+constexpr int kXSlices = 4;  // slices of 256 words per round of the write pass
 
 __device__ void load_quads(Quad<uint32_t> (*dst)[kDpCols], const Quad<uint32_t>* src) {
     Quad<uint32_t>* d = &dst[0][0];
@@ -54,7 +55,9 @@ struct Blk {
     int share;
 };
 
-template <bool Main>
+// Rank is false in the count pass, which needs a block's extent and share only
+// (not its top-part HIGH rank, whose group-row reads are global loads).
+template <bool Main, bool Rank>
 __device__ Blk find_block(const XchgLaunch& a, const Quad<uint32_t> (*tq)[kDpCols], const uint32_t* off, uint32_t r) {
     const xchg::Top t = xchg::decode_top<Main>(tq, a.top, a.base, r);
     Blk b;
@@ -62,7 +65,7 @@ __device__ Blk find_block(const XchgLaunch& a, const Quad<uint32_t> (*tq)[kDpCol
     b.lut = off[t.h];
     b.len = off[t.h + 1] - b.lut;
     b.mask = t.mask;
-    b.top = xchg::top_high(t, a.ht);
+    b.top = Rank ? xchg::top_high(t, a.ht) : 0u;
     b.share = xchg::share_of(t.mask, a.first, a.shares);
     return b;
 }
@@ -74,30 +77,38 @@ __device__ Blk find_block(const XchgLaunch& a, const Quad<uint32_t> (*tq)[kDpCol
 // side by side (one decode's time; blocks of a few words are common where the
 // group's occupied positions are few), and the block of the last valid lane,
 // which the next round continues, becomes the wave's current block.
+template <bool Rank>
 __device__ void assign_blocks(const XchgLaunch& a, const Quad<uint32_t> (*tq)[kDpCols], const uint32_t* off,
                               bool valid, uint32_t r, Blk& cb, Blk& mine) {
     const bool in = valid && (r - cb.start < cb.len);
     mine = cb;
     if (__ballot(valid && !in) == 0) return;
-    if (valid && !in) mine = a.main ? find_block<true>(a, tq, off, r) : find_block<false>(a, tq, off, r);
+    if (valid && !in) mine = a.main ? find_block<true, Rank>(a, tq, off, r) : find_block<false, Rank>(a, tq, off, r);
     const uint64_t vm = __ballot(valid);
     const int last = 63 - __clzll((long long)vm);
     cb.start = (uint32_t)__shfl((int)mine.start, last);
     cb.len = (uint32_t)__shfl((int)mine.len, last);
     cb.mask = (uint32_t)__shfl((int)mine.mask, last);
-    cb.top = (uint32_t)__shfl((int)mine.top, last);
+    if (Rank) cb.top = (uint32_t)__shfl((int)mine.top, last);
     cb.lut = (uint32_t)__shfl((int)mine.lut, last);
     cb.share = __shfl(mine.share, last);
+}
+
+// The group's bottom offsets (kDpCols + 1 entries) into LDS.
+__device__ void load_off(uint32_t* dst, const XchgLaunch& a, uint32_t g) {
+    const uint32_t* src = a.lut_off + (size_t)g * (kDpCols + 1);
+    for (int t = threadIdx.x; t < kDpCols + 1; t += blockDim.x) dst[t] = src[t];
 }
 
 __global__ __launch_bounds__(kXThreads) void xchg_count_kernel(XchgLaunch a) {
     __shared__ Quad<uint32_t> tq[kDpRows][kDpCols];
     __shared__ uint32_t cnt[kMaxShares];
+    __shared__ uint32_t off[kDpCols + 1];
     const XTile t = a.tiles[blockIdx.x];
     load_quads(tq, group_table(a, t.group));
+    load_off(off, a, t.group);
     if (threadIdx.x < kMaxShares) cnt[threadIdx.x] = 0;
     __syncthreads();
-    const uint32_t* off = a.lut_off + (size_t)t.group * (kDpCols + 1);
     // lane q of each wave counts share q
     const int lane = threadIdx.x & 63;
     uint32_t mine_cnt = 0;
@@ -106,8 +117,16 @@ __global__ __launch_bounds__(kXThreads) void xchg_count_kernel(XchgLaunch a) {
         const uint32_t i = r0 + threadIdx.x;
         const bool valid = i < t.len;
         Blk mine;
-        assign_blocks(a, tq, off, valid, t.rank0 + i, cb, mine);
+        assign_blocks<false>(a, tq, off, valid, t.rank0 + i, cb, mine);
         const int s = valid ? mine.share : -1;
+        const uint64_t vm = __ballot(valid);
+        if (vm == 0) continue;
+        // the usual case: one destination share for the whole round
+        const int s0 = __shfl(s, __ffsll((long long)vm) - 1);
+        if (__ballot(valid && s != s0) == 0) {
+            if (lane == s0) mine_cnt += (uint32_t)__popcll(vm);
+            continue;
+        }
         for (int q = 0; q < a.shares; ++q) {
             const uint32_t c = (uint32_t)__popcll(__ballot(s == q));
             if (lane == q) mine_cnt += c;
@@ -144,13 +163,22 @@ __global__ __launch_bounds__(1024) void xchg_scan_kernel(XchgLaunch a) {
     if (tid == 1023) a.totals[s] = part[1023];
 }
 
+// Rounds of kXSlices x 256 words: slice j of a round is the 256 consecutive
+// ranks r0 + 256 j ..; each thread loads its words' residues for every slice
+// first (so that a round keeps kXSlices loads per thread in flight), walks
+// the blocks and counts the slices' records per destination share and wave,
+// then one thread per share turns the counts of the round's (slice, wave)
+// pairs, in window order, into absolute bucket slots, and the words are stored.
 __global__ __launch_bounds__(kXThreads) void xchg_write_kernel(XchgLaunch a) {
+    constexpr int E = kXSlices * kXWaves;  // (slice, wave) pairs of a round, in window order
     __shared__ Quad<uint32_t> tq[kDpRows][kDpCols];
     __shared__ uint64_t cursor[kMaxShares];
-    __shared__ uint32_t wcnt[kXWaves][kMaxShares];
+    __shared__ uint32_t wcnt[E][kMaxShares];
+    __shared__ uint64_t sbase[E][kMaxShares];
+    __shared__ uint32_t off[kDpCols + 1];
     const XTile t = a.tiles[blockIdx.x];
     load_quads(tq, group_table(a, t.group));
-    const uint32_t* off = a.lut_off + (size_t)t.group * (kDpCols + 1);
+    load_off(off, a, t.group);
     Blk cb{0u, 0u, 0u, 0u, 0u, 0};
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (tid < a.shares) {
@@ -158,47 +186,83 @@ __global__ __launch_bounds__(kXThreads) void xchg_write_kernel(XchgLaunch a) {
         for (int q = 0; q < tid; ++q) start += a.totals[q];
         cursor[tid] = start + a.offs[(size_t)tid * a.ntiles + blockIdx.x];
     }
-    __syncthreads();
+    __syncthreads();  // the tables and cursors are in LDS before any wave decodes
     const uint64_t below = (1ull << lane) - 1ull;
-    for (uint32_t r0 = 0; r0 < t.len; r0 += kXThreads) {
-        const uint32_t i = r0 + tid;
-        const bool valid = i < t.len;
-        const uint32_t r = t.rank0 + i;
-        Blk mine;
-        assign_blocks(a, tq, off, valid, r, cb, mine);
-        int s = -1;
-        uint64_t dst = 0;
-        if (valid) {
-            s = mine.share;
-            dst = a.base_of[mine.mask] + (uint32_t)(mine.top + a.lut[mine.lut + (r - mine.start)]);
+    const bool even = (a.K & 1) == 0;
+    for (uint32_t r0 = 0; r0 < t.len; r0 += kXThreads * kXSlices) {
+        uint2 v01[kXSlices], v23[kXSlices];
+        int sh[kXSlices];
+        uint32_t pre[kXSlices];
+        uint64_t dst[kXSlices];
+#pragma unroll
+        for (int j = 0; j < kXSlices; ++j) {
+            const uint32_t i = r0 + (uint32_t)(j * kXThreads + tid);
+            v01[j] = make_uint2(0u, 0u);
+            v23[j] = make_uint2(0u, 0u);
+            if (i < t.len && even) {
+                const uint32_t* src = a.res + (t.word0 + i) * (uint64_t)a.K;
+                v01[j] = *reinterpret_cast<const uint2*>(src);
+                if (a.K >= 4) v23[j] = *reinterpret_cast<const uint2*>(src + 2);
+            }
         }
-        uint32_t pre = 0;
-        for (int q = 0; q < a.shares; ++q) {
-            const uint64_t m = __ballot(s == q);
-            if (s == q) pre = (uint32_t)__popcll(m & below);
-            if (lane == 0) wcnt[wave][q] = (uint32_t)__popcll(m);
-        }
-        __syncthreads();
-        if (valid) {
-            uint64_t slot = cursor[s] + pre;
-            for (int v = 0; v < wave; ++v) slot += wcnt[v][s];
-            a.out_idx[slot] = dst;
-            const uint32_t* src = a.res + (t.word0 + i) * (uint64_t)a.K;
-            uint32_t* out = a.out_res + slot * (uint64_t)a.K;
-            if ((a.K & 1) == 0) {
-                for (int j = 0; j < a.K; j += 2)
-                    *reinterpret_cast<uint2*>(out + j) = *reinterpret_cast<const uint2*>(src + j);
+#pragma unroll
+        for (int j = 0; j < kXSlices; ++j) {
+            const uint32_t i = r0 + (uint32_t)(j * kXThreads + tid);
+            const bool valid = i < t.len;
+            const uint32_t r = t.rank0 + i;
+            Blk mine;
+            assign_blocks<true>(a, tq, off, valid, r, cb, mine);
+            sh[j] = -1;
+            dst[j] = 0;
+            if (valid) {
+                sh[j] = mine.share;
+                dst[j] = a.base_of[mine.mask] + (uint32_t)(mine.top + a.lut[mine.lut + (r - mine.start)]);
+            }
+            const int e = j * kXWaves + wave;
+            const uint64_t vm = __ballot(valid);
+            const int s0 = vm ? __shfl(sh[j], __ffsll((long long)vm) - 1) : 0;
+            if (__ballot(valid && sh[j] != s0) == 0) {
+                // one destination share for the wave's words of this slice
+                pre[j] = (uint32_t)__popcll(vm & below);
+                if (lane < a.shares) wcnt[e][lane] = lane == s0 ? (uint32_t)__popcll(vm) : 0u;
             } else {
-                for (int j = 0; j < a.K; ++j) out[j] = src[j];
+                pre[j] = 0;
+                for (int q = 0; q < a.shares; ++q) {
+                    const uint64_t m = __ballot(sh[j] == q);
+                    if (sh[j] == q) pre[j] = (uint32_t)__popcll(m & below);
+                    if (lane == 0) wcnt[e][q] = (uint32_t)__popcll(m);
+                }
             }
         }
         __syncthreads();
         if (tid < a.shares) {
-            uint32_t add = 0;
-            for (int v = 0; v < kXWaves; ++v) add += wcnt[v][tid];
-            cursor[tid] += add;
+            uint64_t run = cursor[tid];
+            for (int e = 0; e < E; ++e) {
+                sbase[e][tid] = run;
+                run += wcnt[e][tid];
+            }
+            cursor[tid] = run;
         }
         __syncthreads();
+        // (the next round's first barrier orders these reads of sbase before
+        // its rewrite)
+#pragma unroll
+        for (int j = 0; j < kXSlices; ++j) {
+            if (sh[j] < 0) continue;
+            const uint32_t i = r0 + (uint32_t)(j * kXThreads + tid);
+            const uint64_t slot = sbase[j * kXWaves + wave][sh[j]] + pre[j];
+            a.out_idx[slot] = dst[j];
+            const uint32_t* src = a.res + (t.word0 + i) * (uint64_t)a.K;
+            uint32_t* out = a.out_res + slot * (uint64_t)a.K;
+            if (even) {
+                *reinterpret_cast<uint2*>(out) = v01[j];
+                if (a.K >= 4) *reinterpret_cast<uint2*>(out + 2) = v23[j];
+                for (int k = 4; k < a.K; k += 2)
+                    *reinterpret_cast<uint2*>(out + k) = *reinterpret_cast<const uint2*>(src + k);
+            } else {
+                for (int k = 0; k < a.K; ++k) out[k] = src[k];
+            }
+        }
     }
 }
 

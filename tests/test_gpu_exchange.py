@@ -5,8 +5,9 @@ buckets by the share that owns their HIGH group (ofp_group_exchange), the
 buckets are delivered, and every share sweeps the low half over its HIGH
 window -- must equal the full-state row (ofp_apply_cells), bit-exact.  The
 staging itself is checked against the high half's output permuted on the host.
-At the n = 27 shape, one LOW window is exchanged and its buckets checked for
-counts, ranges and the residues they carry."""
+At the n = 27 shape, LOW windows are exchanged and their buckets checked for
+counts, ranges, the residues they carry, window order inside each bucket and
+run-to-run identity."""
 from __future__ import annotations
 
 import numpy as np
@@ -122,6 +123,52 @@ def test_n27_window_buckets(gpu_ctx):
         # the records carry exactly the window's residues (as a multiset per modulus)
         for j in range(4):
             assert np.array_equal(np.sort(res[:, j]), np.sort(src[:, j]))
+
+
+def _keys(res: np.ndarray) -> np.ndarray:
+    """A 64-bit key per row: its first two residues."""
+    return (res[:, 0].astype(np.uint64) << np.uint64(32)) | res[:, 1].astype(np.uint64)
+
+
+def test_n27_large_window_order_and_identity(gpu_ctx):
+    """LOW14 groups 1000..1001 (25.4 M words, tiles of many rounds and blocks of
+    every length): two exchanges of the same window give identical buckets; each
+    record's residues are those of one word of the window (each word once), and
+    inside a bucket the records follow the window order; indices are distinct and
+    inside their share's placement."""
+    n, low, mods, shares = 27, 14, list(PRIMES[:4]), 8
+    ginfo, _, _ = ofp.group_plan(n, 4, 1000, 2, low)
+    main = _inputs(ginfo["main_count"], mods, 2728)
+    print("case n=27 LOW groups 1000..1001 sweep", flush=True)
+    _, mo, bo = gpu_ctx.group_sweep(n, mods, 1000, 2, low, main_in=main, blocked_out=True)
+    print("case n=27 LOW groups 1000..1001 exchange (twice)", flush=True)
+    info, b = gpu_ctx.group_exchange(n, mods, 1000, 2, low, shares=shares, download=True)
+    info2, b2 = gpu_ctx.group_exchange(n, mods, 1000, 2, low, shares=shares, download=True)
+    assert info["main_records"] == info2["main_records"] and info["blocked_records"] == info2["blocked_records"]
+    for key in b:
+        assert np.array_equal(b[key], b2[key]), key
+    _, hmc, hbc = ofp.group_plan(n, 4, 0, 1 << 13, low, high=True)
+    first = info["share_first"]
+    for kind, counts, src in (("main", hmc, mo), ("blocked", hbc, bo)):
+        rec = info[f"{kind}_records"]
+        idx, res = b[f"{kind}_index"], b[f"{kind}_res"]
+        wk = _keys(src)
+        order = np.argsort(wk)
+        sk = wk[order]
+        assert np.all(sk[1:] != sk[:-1])  # the seeded residues tell the words apart
+        pos = np.searchsorted(sk, _keys(res))
+        assert np.all(pos < sk.size)
+        wpos = order[pos]
+        assert np.array_equal(src[wpos], res)  # every record is a word of the window
+        assert np.unique(wpos).size == wpos.size == src.shape[0]  # each word exactly once
+        lo = 0
+        for s in range(shares):
+            part, wp = idx[lo:lo + rec[s]], wpos[lo:lo + rec[s]]
+            size = int(counts[first[s]:first[s + 1]].sum())
+            assert part.size == 0 or int(part.max()) < size
+            assert np.unique(part).size == part.size
+            assert np.all(np.diff(wp) > 0), (kind, s)  # window order inside the bucket
+            lo += rec[s]
 
 
 def test_exchange_needs_the_swept_window(gpu_ctx):
