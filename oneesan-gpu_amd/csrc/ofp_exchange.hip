@@ -10,21 +10,21 @@
 // L-major placements, docs/b300-bucket-snake.md:7-20).
 //
 // Three launches per kind of word (main, blocked), one workgroup per tile of
-// at most kXTileWords words of one group, each round of a wave taking 64
-// consecutive local ranks.  Words with one top part (the fixed HIGH positions)
-// are a run of consecutive ranks -- a block, about a hundred words at W = 28 --
-// with one destination share and one top-part HIGH-rank contribution, so a
-// wave decodes top parts only in rounds that leave its current block
-// (xchg::decode_top) and takes each word's bottom contribution from the
-// group's bottom table (xchg::build_bottom_lut):
-//   count  -- count each word's destination share (one wave64 ballot per share
-//             and round);
+// at most kXTileWords words of one group.  Words with one top part (the fixed
+// HIGH positions) are a run of consecutive ranks -- a block, about a hundred
+// words at W = 28 -- with one destination share and one top-part HIGH-rank
+// contribution, so each thread walks a run of consecutive ranks block by
+// block, decoding one top part per block it meets (xchg::decode_top), and takes
+// each word's bottom contribution from the group's bottom table
+// (xchg::build_bottom_lut):
+//   count  -- words per destination share and tile (each thread walks a run
+//             of the tile);
 //   scan   -- one workgroup per share: exclusive prefix of the tile counts;
 //   write  -- write each word's record.  Records keep the window order inside a
-//             bucket: each round of 256 words is compacted with one wave64
-//             ballot per share (a lane's slot = the tile's cursor + the set
-//             lanes below it + the counts of the lower waves), so the buckets
-//             are deterministic.
+//             bucket: each round of 1024 words is compacted with wave64 ballots
+//             (a lane's slot = the tile's cursor + the set lanes below it + the
+//             counts of the lower waves and slices), so the buckets are
+//             deterministic.
 // The residues are read once in window order (coalesced) and written once as
 // records; records of one share in one round are contiguous.
 #include "ofp_exchange_impl.h"
@@ -47,57 +47,30 @@ __device__ const Quad<uint32_t>* group_table(const XchgLaunch& a, uint32_t g) {
     return reinterpret_cast<const Quad<uint32_t>*>(a.gtab + (size_t)g * sizeof(Quad<uint32_t>) * kDpRows * kDpCols);
 }
 
-// A block: the consecutive local ranks whose words share one top part (the
-// fixed HIGH positions), hence one HIGH mask, destination share and top-part
-// HIGH-rank contribution; a word's bottom contributes lut[lut + r - start].
-struct Blk {
-    uint32_t start, len, mask, top, lut;
-    int share;
-};
-
-// Rank is false in the count pass, which needs a block's extent and share only
-// (not its top-part HIGH rank, whose group-row reads are global loads).
-template <bool Main, bool Rank>
-__device__ Blk find_block(const XchgLaunch& a, const Quad<uint32_t> (*tq)[kDpCols], const uint32_t* off, uint32_t r) {
-    const xchg::Top t = xchg::decode_top<Main>(tq, a.top, a.base, r);
-    Blk b;
-    b.start = r - t.r_rest;
-    b.lut = off[t.h];
-    b.len = off[t.h + 1] - b.lut;
-    b.mask = t.mask;
-    b.top = Rank ? xchg::top_high(t, a.ht) : 0u;
-    b.share = xchg::share_of(t.mask, a.first, a.shares);
-    return b;
-}
-
-// Give every valid lane of the wave the block of its rank r.  `cb` is the
-// wave's current block (the same in every lane), kept from round to round:
-// rounds walk consecutive ranks, so in a large block every lane falls in it and
-// nothing is decoded.  Otherwise the lanes outside it decode their own top parts
-// side by side (one decode's time; blocks of a few words are common where the
-// group's occupied positions are few), and the block of the last valid lane,
-// which the next round continues, becomes the wave's current block.
-template <bool Rank>
-__device__ void assign_blocks(const XchgLaunch& a, const Quad<uint32_t> (*tq)[kDpCols], const uint32_t* off,
-                              bool valid, uint32_t r, Blk& cb, Blk& mine) {
-    const bool in = valid && (r - cb.start < cb.len);
-    mine = cb;
-    if (__ballot(valid && !in) == 0) return;
-    if (valid && !in) mine = a.main ? find_block<true, Rank>(a, tq, off, r) : find_block<false, Rank>(a, tq, off, r);
-    const uint64_t vm = __ballot(valid);
-    const int last = 63 - __clzll((long long)vm);
-    cb.start = (uint32_t)__shfl((int)mine.start, last);
-    cb.len = (uint32_t)__shfl((int)mine.len, last);
-    cb.mask = (uint32_t)__shfl((int)mine.mask, last);
-    if (Rank) cb.top = (uint32_t)__shfl((int)mine.top, last);
-    cb.lut = (uint32_t)__shfl((int)mine.lut, last);
-    cb.share = __shfl(mine.share, last);
-}
-
 // The group's bottom offsets (kDpCols + 1 entries) into LDS.
 __device__ void load_off(uint32_t* dst, const XchgLaunch& a, uint32_t g) {
     const uint32_t* src = a.lut_off + (size_t)g * (kDpCols + 1);
     for (int t = threadIdx.x; t < kDpCols + 1; t += blockDim.x) dst[t] = src[t];
+}
+
+// Count pass: thread t takes a contiguous run of the tile's ranks and walks it
+// block by block -- one top-part decode per block it meets (about one per
+// thread: blocks are longer than the runs), adding the block's words in the
+// run to the block's destination share.
+template <bool Main>
+__device__ void count_run(const XchgLaunch& a, const XTile& t, const Quad<uint32_t> (*tq)[kDpCols],
+                          const uint32_t* off, uint32_t* cnt) {
+    const uint32_t per = (t.len + kXThreads - 1) / kXThreads;
+    uint32_t i = min(threadIdx.x * per, t.len);
+    const uint32_t e = min(i + per, t.len);
+    while (i < e) {
+        const xchg::Top tp = xchg::decode_top<Main>(tq, a.top, a.base, t.rank0 + i);
+        // the block holds ranks r - r_rest .. r - r_rest + len - 1, r_rest < len
+        const uint32_t end = i + (off[tp.h + 1] - off[tp.h] - tp.r_rest);
+        const uint32_t stop = min(end, e);
+        atomicAdd(&cnt[xchg::share_of(tp.mask, a.first, a.shares)], stop - i);
+        i = stop;
+    }
 }
 
 __global__ __launch_bounds__(kXThreads) void xchg_count_kernel(XchgLaunch a) {
@@ -109,30 +82,8 @@ __global__ __launch_bounds__(kXThreads) void xchg_count_kernel(XchgLaunch a) {
     load_off(off, a, t.group);
     if (threadIdx.x < kMaxShares) cnt[threadIdx.x] = 0;
     __syncthreads();
-    // lane q of each wave counts share q
-    const int lane = threadIdx.x & 63;
-    uint32_t mine_cnt = 0;
-    Blk cb{0u, 0u, 0u, 0u, 0u, 0};
-    for (uint32_t r0 = 0; r0 < t.len; r0 += kXThreads) {
-        const uint32_t i = r0 + threadIdx.x;
-        const bool valid = i < t.len;
-        Blk mine;
-        assign_blocks<false>(a, tq, off, valid, t.rank0 + i, cb, mine);
-        const int s = valid ? mine.share : -1;
-        const uint64_t vm = __ballot(valid);
-        if (vm == 0) continue;
-        // the usual case: one destination share for the whole round
-        const int s0 = __shfl(s, __ffsll((long long)vm) - 1);
-        if (__ballot(valid && s != s0) == 0) {
-            if (lane == s0) mine_cnt += (uint32_t)__popcll(vm);
-            continue;
-        }
-        for (int q = 0; q < a.shares; ++q) {
-            const uint32_t c = (uint32_t)__popcll(__ballot(s == q));
-            if (lane == q) mine_cnt += c;
-        }
-    }
-    if (lane < a.shares) atomicAdd(&cnt[lane], mine_cnt);
+    if (a.main) count_run<true>(a, t, tq, off, cnt);
+    else count_run<false>(a, t, tq, off, cnt);
     __syncthreads();
     if ((int)threadIdx.x < a.shares) a.counts[(size_t)threadIdx.x * a.ntiles + blockIdx.x] = cnt[threadIdx.x];
 }
@@ -163,37 +114,62 @@ __global__ __launch_bounds__(1024) void xchg_scan_kernel(XchgLaunch a) {
     if (tid == 1023) a.totals[s] = part[1023];
 }
 
-// Rounds of kXSlices x 256 words: slice j of a round is the 256 consecutive
-// ranks r0 + 256 j ..; each thread loads its words' residues for every slice
-// first (so that a round keeps kXSlices loads per thread in flight), walks
-// the blocks and counts the slices' records per destination share and wave,
-// then one thread per share turns the counts of the round's (slice, wave)
-// pairs, in window order, into absolute bucket slots, and the words are stored.
+// The records of a run of kXSlices consecutive words of a round, by a block
+// walk (one top-part decode per block met, about one per run): destination
+// share (-1 past the tile) and index, into the round's LDS arrays.
+template <bool Main>
+__device__ void walk_run(const XchgLaunch& a, const XTile& t, const Quad<uint32_t> (*tq)[kDpCols],
+                         const uint32_t* off, uint32_t r0, uint64_t* wdst, int8_t* wsh) {
+    const uint32_t first = r0 + (uint32_t)(threadIdx.x * kXSlices);
+    const uint32_t e = min(first + (uint32_t)kXSlices, t.len);
+    for (uint32_t x = e > first ? e : first; x < first + (uint32_t)kXSlices; ++x) wsh[x - r0] = -1;
+    uint32_t i = first;
+    while (i < e) {
+        const xchg::Top tp = xchg::decode_top<Main>(tq, a.top, a.base, t.rank0 + i);
+        const uint32_t lut0 = off[tp.h] + tp.r_rest;  // bottom table entry of rank i
+        const uint32_t stop = min(i + (off[tp.h + 1] - off[tp.h] - tp.r_rest), e);
+        const uint64_t base = a.base_of[tp.mask];
+        const uint32_t top = xchg::top_high(tp, a.ht);
+        const int8_t sh = (int8_t)xchg::share_of(tp.mask, a.first, a.shares);
+        for (uint32_t x = i; x < stop; ++x) {
+            wdst[x - r0] = base + (uint32_t)(top + a.lut[lut0 + (x - i)]);
+            wsh[x - r0] = sh;
+        }
+        i = stop;
+    }
+}
+
+// Rounds of R = kXSlices x 256 words.  Each thread loads the residues of its
+// words of the round's slices (slice j = the 256 consecutive ranks r0 + 256 j
+// ..; kXSlices loads per thread in flight), walks the blocks of its run of
+// kXSlices consecutive words (walk_run) into LDS, then, slice by slice, counts
+// the records per destination share and wave; one thread per share turns the
+// counts of the round's (slice, wave) pairs, in window order, into absolute
+// bucket slots, and the words are stored.
 __global__ __launch_bounds__(kXThreads) void xchg_write_kernel(XchgLaunch a) {
     constexpr int E = kXSlices * kXWaves;  // (slice, wave) pairs of a round, in window order
+    constexpr int R = kXSlices * kXThreads;
     __shared__ Quad<uint32_t> tq[kDpRows][kDpCols];
     __shared__ uint64_t cursor[kMaxShares];
     __shared__ uint32_t wcnt[E][kMaxShares];
     __shared__ uint64_t sbase[E][kMaxShares];
     __shared__ uint32_t off[kDpCols + 1];
+    __shared__ uint64_t wdst[R];
+    __shared__ int8_t wsh[R];
     const XTile t = a.tiles[blockIdx.x];
     load_quads(tq, group_table(a, t.group));
     load_off(off, a, t.group);
-    Blk cb{0u, 0u, 0u, 0u, 0u, 0};
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (tid < a.shares) {
         uint64_t start = 0;
         for (int q = 0; q < tid; ++q) start += a.totals[q];
         cursor[tid] = start + a.offs[(size_t)tid * a.ntiles + blockIdx.x];
     }
-    __syncthreads();  // the tables and cursors are in LDS before any wave decodes
+    __syncthreads();  // the tables and cursors are in LDS before any thread decodes
     const uint64_t below = (1ull << lane) - 1ull;
     const bool even = (a.K & 1) == 0;
-    for (uint32_t r0 = 0; r0 < t.len; r0 += kXThreads * kXSlices) {
+    for (uint32_t r0 = 0; r0 < t.len; r0 += R) {
         uint2 v01[kXSlices], v23[kXSlices];
-        int sh[kXSlices];
-        uint32_t pre[kXSlices];
-        uint64_t dst[kXSlices];
 #pragma unroll
         for (int j = 0; j < kXSlices; ++j) {
             const uint32_t i = r0 + (uint32_t)(j * kXThreads + tid);
@@ -205,19 +181,15 @@ __global__ __launch_bounds__(kXThreads) void xchg_write_kernel(XchgLaunch a) {
                 if (a.K >= 4) v23[j] = *reinterpret_cast<const uint2*>(src + 2);
             }
         }
+        if (a.main) walk_run<true>(a, t, tq, off, r0, wdst, wsh);
+        else walk_run<false>(a, t, tq, off, r0, wdst, wsh);
+        __syncthreads();
+        int sh[kXSlices];
+        uint32_t pre[kXSlices];
 #pragma unroll
         for (int j = 0; j < kXSlices; ++j) {
-            const uint32_t i = r0 + (uint32_t)(j * kXThreads + tid);
-            const bool valid = i < t.len;
-            const uint32_t r = t.rank0 + i;
-            Blk mine;
-            assign_blocks<true>(a, tq, off, valid, r, cb, mine);
-            sh[j] = -1;
-            dst[j] = 0;
-            if (valid) {
-                sh[j] = mine.share;
-                dst[j] = a.base_of[mine.mask] + (uint32_t)(mine.top + a.lut[mine.lut + (r - mine.start)]);
-            }
+            sh[j] = wsh[j * kXThreads + tid];
+            const bool valid = sh[j] >= 0;
             const int e = j * kXWaves + wave;
             const uint64_t vm = __ballot(valid);
             const int s0 = vm ? __shfl(sh[j], __ffsll((long long)vm) - 1) : 0;
@@ -244,14 +216,12 @@ __global__ __launch_bounds__(kXThreads) void xchg_write_kernel(XchgLaunch a) {
             cursor[tid] = run;
         }
         __syncthreads();
-        // (the next round's first barrier orders these reads of sbase before
-        // its rewrite)
 #pragma unroll
         for (int j = 0; j < kXSlices; ++j) {
             if (sh[j] < 0) continue;
             const uint32_t i = r0 + (uint32_t)(j * kXThreads + tid);
             const uint64_t slot = sbase[j * kXWaves + wave][sh[j]] + pre[j];
-            a.out_idx[slot] = dst[j];
+            a.out_idx[slot] = wdst[j * kXThreads + tid];
             const uint32_t* src = a.res + (t.word0 + i) * (uint64_t)a.K;
             uint32_t* out = a.out_res + slot * (uint64_t)a.K;
             if (even) {
@@ -263,6 +233,7 @@ __global__ __launch_bounds__(kXThreads) void xchg_write_kernel(XchgLaunch a) {
                 for (int k = 0; k < a.K; ++k) out[k] = src[k];
             }
         }
+        __syncthreads();  // every read of this round's wdst and sbase is done before the next walk
     }
 }
 
