@@ -21,7 +21,7 @@
 //             of the tile);
 //   scan   -- one workgroup per share: exclusive prefix of the tile counts;
 //   write  -- write each word's record.  Records keep the window order inside a
-//             bucket: each round of 1024 words is compacted with wave64 ballots
+//             bucket: each round of 2048 words is compacted with wave64 ballots
 //             (a lane's slot = the tile's cursor + the set lanes below it + the
 //             counts of the lower waves and slices), so the buckets are
 //             deterministic.
@@ -36,7 +36,7 @@ namespace {
 using cell::Quad;
 constexpr int kXThreads = 256;
 constexpr int kXWaves = kXThreads / 64;
-constexpr int kXSlices = 4;  // slices of 256 words per round of the write pass
+constexpr int kXSlices = 8;  // slices of 256 words per round of the write pass
 
 __device__ void load_quads(Quad<uint32_t> (*dst)[kDpCols], const Quad<uint32_t>* src) {
     Quad<uint32_t>* d = &dst[0][0];
