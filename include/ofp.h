@@ -173,6 +173,13 @@ ofp_status ofp_time_cells(ofp_ctx* ctx, float* ms_out, int* kind_out, int capaci
  * scratch. */
 ofp_status ofp_set_pair_passes(ofp_ctx* ctx, int mode, uint32_t cell_mask);
 
+/* Streams a group sweep (ofp_group_sweep) deals each cell's group launches to,
+ * 1..4 (0 = the default, 2).  The groups of one cell are independent, so their
+ * launches run side by side; every cell still starts after the previous cell
+ * of every group has finished.  Results do not depend on it.  No reference
+ * counterpart (the reference runs one group kernel at a time per GPU). */
+ofp_status ofp_set_group_streams(ofp_ctx* ctx, int streams);
+
 /* Index width of the cell kernels: 0 = by n (32-bit ranks for n <= 22,
  * 64-bit above), 64 = 64-bit ranks at every n.  Results do not depend on it;
  * lets tests run the 64-bit kernels at sizes the CPU oracle checks.  No

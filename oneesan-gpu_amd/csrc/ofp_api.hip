@@ -72,6 +72,9 @@ struct ofp_ctx {
     uint32_t pair_mask = 0;  // bit p: cells p and p-1 run as one pair pass; 0: the library's default
     int pair_variant = 0;    // pair-kernel variant (PairLaunch::variant)
     int group_pairs = 0;     // group sweeps run pair passes too (off: at 4 moduli they are slower)
+    int group_streams = 2;   // streams a group sweep deals each cell's group launches to (1..4)
+    hipStream_t gstream[3] = {};             // the extra streams (created on first use)
+    hipEvent_t gfork = nullptr, gjoin[3] = {};
     int cap_percent = -1;  // height caps: cap a cell when it then sweeps <= this % of the blocked words
                            // (-1: 75 at one modulus, else 85)
     int xcd_shift = 5;     // chunk pieces per XCD: 2^xcd_shift chunks, < 0 = one eighth each
@@ -1050,6 +1053,14 @@ ofp_status ofp_set_pair_passes(ofp_ctx* c, int mode, uint32_t cell_mask) {
     return OFP_OK;
 }
 
+ofp_status ofp_set_group_streams(ofp_ctx* c, int streams) {
+    if (!c) return OFP_E_ARG;
+    if (streams == 0) streams = 2;
+    if (streams < 1 || streams > 4) return fail(c, OFP_E_ARG, "group streams: 1..4 (0 = the default, 2)");
+    c->group_streams = streams;
+    return OFP_OK;
+}
+
 ofp_status ofp_set_index_width(ofp_ctx* c, int bits) {
     if (!c) return OFP_E_ARG;
     if (bits != 0 && bits != 64) return fail(c, OFP_E_ARG, "index width must be 0 (by n) or 64");
@@ -1492,6 +1503,13 @@ ofp_status ofp_group_sweep(ofp_ctx* c, const ofp_group_spec* sp, ofp_group_info*
         OFP_HIP_R(launch_fill_remote(dm, dom, fillM, db, dom, fillB, Kp, k, dmods, sp->seed, c->stream),
                   "fill group window");
     const int sf = c->sorted_from >= 0 ? c->sorted_from : (k >= 8 ? 6 : 0);
+    if (c->group_streams > 1 && !c->gfork) {
+        OFP_HIP_R(hipEventCreateWithFlags(&c->gfork, hipEventDisableTiming), "group event");
+        for (int q = 0; q < 3; ++q) {
+            OFP_HIP_R(hipStreamCreateWithFlags(&c->gstream[q], hipStreamNonBlocking), "group stream");
+            OFP_HIP_R(hipEventCreateWithFlags(&c->gjoin[q], hipEventDisableTiming), "group event");
+        }
+    }
     int launches = 0;
     OFP_HIP_R(hipEventRecord(ev[0], c->stream), "event");
     int i = 0, pj = 0;
@@ -1532,8 +1550,21 @@ ofp_status ofp_group_sweep(ofp_ctx* c, const ofp_group_spec* sp, ofp_group_info*
             ++pj;
             continue;
         }
+        // The groups of a cell are independent (each has its own words), so their
+        // launches are dealt round-robin to ns streams that run side by side: the
+        // many small groups' launches fill each other's ramp-up and tail.  The
+        // streams fork from and join the sweep's stream at every cell, which keeps
+        // the cell boundaries (and per-cell times) of the single-stream order.
+        const int ns = c->group_streams;
+        if (ns > 1) {
+            OFP_HIP_R(hipEventRecord(c->gfork, c->stream), "group fork");
+            for (int q = 0; q < ns - 1; ++q) OFP_HIP_R(hipStreamWaitEvent(c->gstream[q], c->gfork, 0), "group fork");
+        }
+        uint32_t dealt = 0;
         for (uint32_t g = 0; g < G; ++g) {
             if (!bc[g]) continue;  // no blocked word: the cell changes nothing in the group
+            const uint32_t q = dealt++ % (uint32_t)ns;
+            hipStream_t gs = q == 0 ? c->stream : c->gstream[q - 1];
             CellLaunch a{};
             a.main = dm + moff[g] * Kp;
             a.blk = db + boff[g] * Kp;
@@ -1550,9 +1581,14 @@ ofp_status ofp_group_sweep(ofp_ctx* c, const ofp_group_spec* sp, ofp_group_info*
             a.off = 0;
             a.kc = k;
             for (int j = 0; j < k; ++j) a.mod[j] = sp->moduli[j];
-            OFP_HIP_R(launch_cell(a, false, c->threads, grid_blocks(c, bc[g]), c->stream), "group cell launch");
+            OFP_HIP_R(launch_cell(a, false, c->threads, grid_blocks(c, bc[g]), gs), "group cell launch");
             ++launches;
         }
+        if (ns > 1)
+            for (int q = 0; q < ns - 1; ++q) {
+                OFP_HIP_R(hipEventRecord(c->gjoin[q], c->gstream[q]), "group join");
+                OFP_HIP_R(hipStreamWaitEvent(c->stream, c->gjoin[q], 0), "group join");
+            }
         OFP_HIP_R(hipEventRecord(ev[i + 1], c->stream), "event");
         ++i;
     }
@@ -1931,6 +1967,11 @@ void ofp_destroy(ofp_ctx* c) {
     if (c->dpc32) (void)hipFree(c->dpc32);
     if (c->dpc64) (void)hipFree(c->dpc64);
     if (c->ct32) (void)hipFree(c->ct32);
+    for (int i = 0; i < 3; ++i) {
+        if (c->gstream[i]) (void)hipStreamDestroy(c->gstream[i]);
+        if (c->gjoin[i]) (void)hipEventDestroy(c->gjoin[i]);
+    }
+    if (c->gfork) (void)hipEventDestroy(c->gfork);
     if (c->win) (void)hipFree(c->win);
     if (c->stage) (void)hipFree(c->stage);
     if (c->ev0) (void)hipEventDestroy(c->ev0);

@@ -31,6 +31,7 @@ EXPORTS = (
     "ofp_shard_plan", "ofp_shard_sweep", "ofp_group_plan", "ofp_group_sweep", "ofp_set_height_caps", "ofp_height_cap", "ofp_set_xcd_pieces",
     "ofp_set_sorted_cells", "ofp_time_cells", "ofp_set_index_width", "ofp_set_max_batch",
     "ofp_moduli_groups", "ofp_solve_static", "ofp_set_pair_passes", "ofp_group_exchange",
+    "ofp_set_group_streams",
 )
 
 
@@ -211,6 +212,7 @@ def load() -> ctypes.CDLL:
         "ofp_set_index_width": (ctypes.c_int32, [vp, ctypes.c_int]),
         "ofp_set_max_batch": (ctypes.c_int32, [vp, ctypes.c_int]),
         "ofp_set_pair_passes": (ctypes.c_int32, [vp, ctypes.c_int, ctypes.c_uint32]),
+        "ofp_set_group_streams": (ctypes.c_int32, [vp, ctypes.c_int]),
         "ofp_solve_static": (ctypes.c_int32, [vp, ctypes.c_int, u32p, ctypes.c_int, u32p]),
         "ofp_moduli_groups": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                              ctypes.c_int]),
@@ -412,6 +414,10 @@ class Context:
         a kernel variant; bit 10 also runs group sweeps as pair passes."""
         m = int(mode) if not isinstance(mode, bool) else (1 if mode else 0)
         self._check(self._lib.ofp_set_pair_passes(self._h, m, cell_mask & 0xFFFFFFFF), "ofp_set_pair_passes")
+
+    def set_group_streams(self, streams: int = 0) -> None:
+        """Streams a group sweep deals each cell's group launches to (1..4, 0 = 2)."""
+        self._check(self._lib.ofp_set_group_streams(self._h, streams), "ofp_set_group_streams")
 
     def set_max_batch(self, max_moduli: int) -> None:
         """At most max_moduli moduli per sweep of solve() (0 = as many as fit HBM)."""

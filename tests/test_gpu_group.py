@@ -221,3 +221,24 @@ def test_group_pair_passes_equal_oracle(oracle, n, low, k, high):
         assert np.array_equal(mo[:, j], em[om]), (j, m)
         if not high:
             assert np.array_equal(bo[:, j], eb[ob]), (j, m)
+
+
+@pytest.mark.parametrize("high", [False, True])
+def test_group_streams_same_results(high):
+    """A window of many small and a few large groups (n = 27 shape) swept with its
+    group launches on one, two and four streams: bit-identical outputs."""
+    n, low, mods = 27, 14, list(PRIMES[:4])
+    first, groups = (2484, 12) if not high else (97, 12)
+    ginfo, _, _ = ofp.group_plan(n, 4, first, groups, low, high=high)
+    main = _inputs(ginfo["main_count"], mods, 4242 + high)
+    outs = []
+    with ofp.Context() as ctx:
+        for ns in (1, 2, 4):
+            ctx.set_group_streams(ns)
+            print(f"case n=27 {'HIGH' if high else 'LOW'} groups {first}+{groups} streams={ns}", flush=True)
+            _, mo, bo = ctx.group_sweep(n, mods, first, groups, low, main_in=main, blocked_out=not high, high=high)
+            outs.append((mo, bo))
+    for mo, bo in outs[1:]:
+        assert np.array_equal(mo, outs[0][0])
+        if not high:
+            assert np.array_equal(bo, outs[0][1])
