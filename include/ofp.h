@@ -166,19 +166,23 @@ ofp_status ofp_time_cells(ofp_ctx* ctx, float* ms_out, int* kind_out, int capaci
  * of mode select a kernel variant for 14 or 16 moduli per launch (tuning: bit 8
  * = closure sums gathered before the class's words are loaded, bit 9 = a
  * register budget for three workgroups per CU); results do not depend on it.
- * Bit 10 of mode: group sweeps (ofp_group_sweep) run pair passes too, per group
- * with its own class tables (off by default: at 4 moduli, n=27 windows, they
- * are slower than one cell per launch).  Reference analogue: the reference's
- * fused two-cell group kernels (row6crt20.cu:1033-1154), which run in group
- * scratch. */
+ * Other mode bits are refused (group sweeps: ofp_set_group_launch).
+ * Reference analogue: the reference's fused two-cell group kernels
+ * (row6crt20.cu:1033-1154), which run in group scratch. */
 ofp_status ofp_set_pair_passes(ofp_ctx* ctx, int mode, uint32_t cell_mask);
 
-/* Streams a group sweep (ofp_group_sweep) deals each cell's group launches to,
- * 1..4 (0 = the default, 2).  The groups of one cell are independent, so their
- * launches run side by side; every cell still starts after the previous cell
- * of every group has finished.  Results do not depend on it.  No reference
- * counterpart (the reference runs one group kernel at a time per GPU). */
-ofp_status ofp_set_group_streams(ofp_ctx* ctx, int streams);
+/* How a group sweep (ofp_group_sweep) launches its steps.  streams: 1..4
+ * (0 = the default, 3) streams that each step's group launches are dealt to,
+ * by work; the groups of one step are independent, so their launches run side
+ * by side, and every step still starts after the previous step of every group
+ * has finished.  pair_mask: walking down the half, cells p and p-1 run as one
+ * pair pass per group (the group's own class tables) where bit p is set, else
+ * cell p runs alone; 0 = no pair passes, -1 = the library's default (pairs
+ * from cell 7 up, cells 6..1 alone).  Results depend on neither.  No reference
+ * counterpart for the streams (the reference runs one group kernel at a time
+ * per GPU); the pair passes are the analogue of its fused two-cell group
+ * kernels (row6crt20.cu:1033-1154). */
+ofp_status ofp_set_group_launch(ofp_ctx* ctx, int streams, long long pair_mask);
 
 /* Index width of the cell kernels: 0 = by n (32-bit ranks for n <= 22,
  * 64-bit above), 64 = 64-bit ranks at every n.  Results do not depend on it;
@@ -326,11 +330,10 @@ ofp_status ofp_shard_sweep(ofp_ctx* ctx, const ofp_shard_spec* spec, ofp_shard_i
  * group is stored contiguously with its words in global rank order (32-bit
  * ranks local to the group: every group of the window must hold fewer than
  * 2^32 words); the window's main array is the groups' main words one group
- * after the other, its blocked array likewise.  The cells run in place, one
- * per launch, or two per pass where group pair passes are on
- * (ofp_set_pair_passes, bit 10 of mode: the pair pass over each group with the
- * group's own class tables; cell_ms then holds a pass's time at its first cell
- * and 0 at its second).
+ * after the other, its blocked array likewise.  The cells run in place, two
+ * per pass where group pair passes are on (ofp_set_group_launch: the pair
+ * pass over each group with the group's own class tables; cell_ms then holds a
+ * pass's time at its first cell and 0 at its second), else one per launch.
  * Reference analogue: one GPU's owner-local LOW and HIGH windows of the 8-GPU
  * bucket layout (docs/b300-bucket-snake.md:7-20; docs/b300-hbm32.md:251-270:
  * W = 28, LOW14 / HIGH13, 16,384 and 8,192 groups). */
@@ -350,6 +353,7 @@ typedef struct ofp_group_info {
     int low, cells, launches;            /* cells in the half: W-1-low or low */
     double kernel_s;                     /* all cells, HIP events */
     float cell_ms[OFP_MAX_N];            /* cell_ms[i]: the i-th cell of the half, top down */
+    int steps;                           /* passes over the window: pair passes + single cells */
 } ofp_group_info;
 /* Window totals into info, and per-group counts into main_counts /
  * blocked_counts (groups entries each; either may be NULL).  No device work;

@@ -71,8 +71,8 @@ struct ofp_ctx {
     int pair_kc = kMaxKC;  // moduli per pair-pass launch (even, <= 16)
     uint32_t pair_mask = 0;  // bit p: cells p and p-1 run as one pair pass; 0: the library's default
     int pair_variant = 0;    // pair-kernel variant (PairLaunch::variant)
-    int group_pairs = 0;     // group sweeps run pair passes too (off: at 4 moduli they are slower)
-    int group_streams = 2;   // streams a group sweep deals each cell's group launches to (1..4)
+    int64_t group_pair_mask = -1;  // group sweeps: bit p pairs cells p, p-1; -1: default_group_pair_mask
+    int group_streams = 3;   // streams a group sweep deals each step's group launches to (1..4)
     hipStream_t gstream[3] = {};             // the extra streams (created on first use)
     hipEvent_t gfork = nullptr, gjoin[3] = {};
     int cap_percent = -1;  // height caps: cap a cell when it then sweeps <= this % of the blocked words
@@ -467,6 +467,11 @@ bool pair_ready(const ofp_ctx* c, const Sweep& s) { return pairs_apply(c, s.n) &
 // against 12.8 + 13.3 ms at one modulus; the other passes are 1.2-1.5x faster
 // than their two cells).
 uint32_t default_pair_mask(int) { return ~((1u << 4) | (1u << 3)); }
+// Group sweeps (4 moduli, n=27 windows over three streams): pair passes over
+// every pair of cells from 7 up; cells 6..1 one per launch, where the pair
+// pass over a group is no faster than its two cells (6, 5: 781 against 780 ms
+// per share row) or slower (2, 1: 1127 against 957 ms).
+uint32_t default_group_pair_mask() { return ~0x7fu; }
 
 ofp_status enqueue_pairs(ofp_ctx* c, Sweep& s, int p_hi, int p_lo, bool first_reads_blk, int* launches,
                          hipEvent_t* cell_ev = nullptr, int* cell_kind = nullptr) {
@@ -1037,9 +1042,9 @@ ofp_status ofp_time_cells(ofp_ctx* c, float* ms_out, int* kind_out, int capacity
 
 ofp_status ofp_set_pair_passes(ofp_ctx* c, int mode, uint32_t cell_mask) {
     if (!c) return OFP_E_ARG;
-    // mode bits 8..9 select the pair-kernel variant (tuning), bit 10 pair passes in group sweeps
+    // mode bits 8..9 select the pair-kernel variant (tuning)
+    if (mode & ~0x3ff) return fail(c, OFP_E_ARG, "pair passes: unknown mode bits (group sweeps: ofp_set_group_launch)");
     const int variant = (mode >> 8) & 3;
-    c->group_pairs = (mode >> 10) & 1;
     mode &= 0xff;
     if (mode < 0 || mode > kMaxKC || (mode > 1 && (mode & 1)))
         return fail(c, OFP_E_ARG, "pair passes: 0 (off), 1 (on) or an even moduli count per launch up to 16");
@@ -1053,11 +1058,14 @@ ofp_status ofp_set_pair_passes(ofp_ctx* c, int mode, uint32_t cell_mask) {
     return OFP_OK;
 }
 
-ofp_status ofp_set_group_streams(ofp_ctx* c, int streams) {
+ofp_status ofp_set_group_launch(ofp_ctx* c, int streams, long long pair_mask) {
     if (!c) return OFP_E_ARG;
-    if (streams == 0) streams = 2;
-    if (streams < 1 || streams > 4) return fail(c, OFP_E_ARG, "group streams: 1..4 (0 = the default, 2)");
+    if (streams == 0) streams = 3;
+    if (streams < 1 || streams > 4) return fail(c, OFP_E_ARG, "group streams: 1..4 (0 = the default, 3)");
+    if (pair_mask < -1 || pair_mask >= (1ll << 32))
+        return fail(c, OFP_E_ARG, "group pair mask: a 32-bit cell mask, or -1 for the library default");
     c->group_streams = streams;
+    c->group_pair_mask = pair_mask;
     return OFP_OK;
 }
 
@@ -1407,10 +1415,10 @@ ofp_status ofp_group_sweep(ofp_ctx* c, const ofp_group_spec* sp, ofp_group_info*
     // are on and the context's pair settings pair them (the full-state rows'
     // mask), else one cell.
     const int p_hi = sp->high ? L : W - 1, p_lo = sp->high ? 1 : L + 1;
-    const uint32_t pmask = c->pair_mask ? c->pair_mask : default_pair_mask(k);
+    const uint32_t pmask = c->group_pair_mask >= 0 ? (uint32_t)c->group_pair_mask : default_group_pair_mask();
     std::vector<int> steps;  // p of each step; a pair step is marked by p | 0x100
     for (int p = p_hi; p >= p_lo;) {
-        if (c->group_pairs && c->pair_rows && p - 1 >= p_lo && p >= 2 && ((pmask >> p) & 1u)) {
+        if (p - 1 >= p_lo && p >= 2 && ((pmask >> p) & 1u)) {
             steps.push_back(p | 0x100);
             p -= 2;
         } else {
@@ -1510,14 +1518,53 @@ ofp_status ofp_group_sweep(ofp_ctx* c, const ofp_group_spec* sp, ofp_group_info*
             OFP_HIP_R(hipEventCreateWithFlags(&c->gjoin[q], hipEventDisableTiming), "group event");
         }
     }
+    // The groups of a step are independent (each has its own words), so their
+    // launches are dealt to ns streams that run side by side: the many small
+    // groups' launches fill each other's ramp-up and tail.  The streams fork
+    // from and join the sweep's stream at every step (a cell or a pair pass),
+    // which keeps the step boundaries (and per-step times) of the one-stream
+    // order; a group keeps one stream within a step.
+    const int ns = c->group_streams;
+    auto fork = [&]() -> ofp_status {
+        if (ns > 1) {
+            OFP_HIP_R(hipEventRecord(c->gfork, c->stream), "group fork");
+            for (int q = 0; q < ns - 1; ++q) OFP_HIP_R(hipStreamWaitEvent(c->gstream[q], c->gfork, 0), "group fork");
+        }
+        return OFP_OK;
+    };
+    auto join = [&]() -> ofp_status {
+        if (ns > 1)
+            for (int q = 0; q < ns - 1; ++q) {
+                OFP_HIP_R(hipEventRecord(c->gjoin[q], c->gstream[q]), "group join");
+                OFP_HIP_R(hipStreamWaitEvent(c->stream, c->gjoin[q], 0), "group join");
+            }
+        return OFP_OK;
+    };
+    // Groups go to the stream with the least work so far (blocked words), in
+    // group order: group sizes follow the mask bits, so a round-robin deal
+    // would give one stream most of the large groups.
+    std::vector<uint8_t> gsi(G, 0);
+    auto deal = [&]() {
+        uint64_t load[4] = {0, 0, 0, 0};
+        for (uint32_t g = 0; g < G; ++g) {
+            int q = 0;
+            for (int r = 1; r < ns; ++r)
+                if (load[r] < load[q]) q = r;
+            gsi[g] = (uint8_t)q;
+            load[q] += bc[g] + 1;
+        }
+    };
+    deal();
+    auto stream_of = [&](uint32_t g) { return gsi[g] == 0 ? c->stream : c->gstream[gsi[g] - 1]; };
     int launches = 0;
     OFP_HIP_R(hipEventRecord(ev[0], c->stream), "event");
     int i = 0, pj = 0;
     for (int x : steps) {
         const int p = x & 0xff;
         if (x & 0x100) {
-            // pair pass: per window change d, every group (groups are independent;
-            // inside a group the launches keep the order d = -1, 0, 1, -2, 2)
+            // pair pass: per window change d, every group (inside a group the
+            // launches keep the order d = -1, 0, 1, -2, 2 on the group's stream)
+            if (ofp_status f = fork()) return f;
             for (int di = 0; di < 5; ++di)
                 for (uint32_t g = 0; g < G; ++g) {
                     const size_t slot = ((size_t)g * npairs + pj) * 5 + di;
@@ -1541,30 +1588,20 @@ ofp_status ofp_group_sweep(ofp_ctx* c, const ofp_group_spec* sp, ofp_group_info*
                     for (int jj = 0; jj < Kp; ++jj) a.mod[jj] = sp->moduli[jj < k ? jj : 0];
                     a.xcd_shift = piece_shift(c->xcd_shift, cnt, c->threads);
                     a.variant = c->pair_variant;
-                    OFP_HIP_R(launch_pair(a, false, c->threads, grid_blocks(c, cnt), c->stream), "group pair launch");
+                    OFP_HIP_R(launch_pair(a, false, c->threads, grid_blocks(c, cnt), stream_of(g)), "group pair launch");
                     ++launches;
                 }
+            if (ofp_status f = join()) return f;
             OFP_HIP_R(hipEventRecord(ev[i + 1], c->stream), "event");
             OFP_HIP_R(hipEventRecord(ev[i + 2], c->stream), "event");
             i += 2;
             ++pj;
             continue;
         }
-        // The groups of a cell are independent (each has its own words), so their
-        // launches are dealt round-robin to ns streams that run side by side: the
-        // many small groups' launches fill each other's ramp-up and tail.  The
-        // streams fork from and join the sweep's stream at every cell, which keeps
-        // the cell boundaries (and per-cell times) of the single-stream order.
-        const int ns = c->group_streams;
-        if (ns > 1) {
-            OFP_HIP_R(hipEventRecord(c->gfork, c->stream), "group fork");
-            for (int q = 0; q < ns - 1; ++q) OFP_HIP_R(hipStreamWaitEvent(c->gstream[q], c->gfork, 0), "group fork");
-        }
-        uint32_t dealt = 0;
+        if (ofp_status f = fork()) return f;
         for (uint32_t g = 0; g < G; ++g) {
             if (!bc[g]) continue;  // no blocked word: the cell changes nothing in the group
-            const uint32_t q = dealt++ % (uint32_t)ns;
-            hipStream_t gs = q == 0 ? c->stream : c->gstream[q - 1];
+            hipStream_t gs = stream_of(g);
             CellLaunch a{};
             a.main = dm + moff[g] * Kp;
             a.blk = db + boff[g] * Kp;
@@ -1584,11 +1621,7 @@ ofp_status ofp_group_sweep(ofp_ctx* c, const ofp_group_spec* sp, ofp_group_info*
             OFP_HIP_R(launch_cell(a, false, c->threads, grid_blocks(c, bc[g]), gs), "group cell launch");
             ++launches;
         }
-        if (ns > 1)
-            for (int q = 0; q < ns - 1; ++q) {
-                OFP_HIP_R(hipEventRecord(c->gjoin[q], c->gstream[q]), "group join");
-                OFP_HIP_R(hipStreamWaitEvent(c->stream, c->gjoin[q], 0), "group join");
-            }
+        if (ofp_status f = join()) return f;
         OFP_HIP_R(hipEventRecord(ev[i + 1], c->stream), "event");
         ++i;
     }
@@ -1602,6 +1635,7 @@ ofp_status ofp_group_sweep(ofp_ctx* c, const ofp_group_spec* sp, ofp_group_info*
     }
     info->kernel_s = total * 1e-3;
     info->launches = launches;
+    info->steps = (int)steps.size();
     if (!sp->high) {
         ofp_ctx::LowWindow& lw = c->last_low;
         lw.n = sp->n;

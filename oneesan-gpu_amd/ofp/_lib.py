@@ -31,7 +31,7 @@ EXPORTS = (
     "ofp_shard_plan", "ofp_shard_sweep", "ofp_group_plan", "ofp_group_sweep", "ofp_set_height_caps", "ofp_height_cap", "ofp_set_xcd_pieces",
     "ofp_set_sorted_cells", "ofp_time_cells", "ofp_set_index_width", "ofp_set_max_batch",
     "ofp_moduli_groups", "ofp_solve_static", "ofp_set_pair_passes", "ofp_group_exchange",
-    "ofp_set_group_streams",
+    "ofp_set_group_launch",
 )
 
 
@@ -148,6 +148,7 @@ class GroupInfo(ctypes.Structure):
         ("launches", ctypes.c_int),
         ("kernel_s", ctypes.c_double),
         ("cell_ms", ctypes.c_float * OFP_MAX_N),
+        ("steps", ctypes.c_int),
     ]
 
     def as_dict(self) -> dict:
@@ -212,7 +213,7 @@ def load() -> ctypes.CDLL:
         "ofp_set_index_width": (ctypes.c_int32, [vp, ctypes.c_int]),
         "ofp_set_max_batch": (ctypes.c_int32, [vp, ctypes.c_int]),
         "ofp_set_pair_passes": (ctypes.c_int32, [vp, ctypes.c_int, ctypes.c_uint32]),
-        "ofp_set_group_streams": (ctypes.c_int32, [vp, ctypes.c_int]),
+        "ofp_set_group_launch": (ctypes.c_int32, [vp, ctypes.c_int, ctypes.c_longlong]),
         "ofp_solve_static": (ctypes.c_int32, [vp, ctypes.c_int, u32p, ctypes.c_int, u32p]),
         "ofp_moduli_groups": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                              ctypes.c_int]),
@@ -411,13 +412,15 @@ class Context:
         one launch per cell: False/0 off, True/1 on, an even 2..16 = on with at most
         that many moduli per launch; cells p and p-1 pair up where bit p of
         cell_mask is set (0: the library's default).  Bits 8..9 of an int mode pick
-        a kernel variant; bit 10 also runs group sweeps as pair passes."""
+        a kernel variant."""
         m = int(mode) if not isinstance(mode, bool) else (1 if mode else 0)
         self._check(self._lib.ofp_set_pair_passes(self._h, m, cell_mask & 0xFFFFFFFF), "ofp_set_pair_passes")
 
-    def set_group_streams(self, streams: int = 0) -> None:
-        """Streams a group sweep deals each cell's group launches to (1..4, 0 = 2)."""
-        self._check(self._lib.ofp_set_group_streams(self._h, streams), "ofp_set_group_streams")
+    def set_group_launch(self, streams: int = 0, pair_mask: int = -1) -> None:
+        """Group sweeps: streams the group launches of a step are dealt to (1..4,
+        0 = 3); cells p and p-1 as one pass per group where bit p of pair_mask is
+        set (0 = never, -1 = the library's default: from cell 7 up)."""
+        self._check(self._lib.ofp_set_group_launch(self._h, streams, int(pair_mask)), "ofp_set_group_launch")
 
     def set_max_batch(self, max_moduli: int) -> None:
         """At most max_moduli moduli per sweep of solve() (0 = as many as fit HBM)."""

@@ -131,7 +131,8 @@ def measure_share(ctx: _lib.Context, n: int = 27, k: int = 4, shares: int = 8, s
         h = {"placement": (f"HIGH{W - 1 - low} groups (occupancy of positions {low + 1}..{W - 1})" if high
                            else f"LOW{low} groups (occupancy of positions 0..{low - 1})"),
              "cells": f"{low}..1" if high else f"{W - 1}..{low + 1}", "windows": len(windows), "groups": 0,
-             "words": 0, "kernel_s": 0.0, "launches": 0, "max_window_bytes": 0, "cell_ms": [0.0] * cells}
+             "words": 0, "kernel_s": 0.0, "launches": 0, "steps": 0, "max_window_bytes": 0,
+             "cell_ms": [0.0] * cells}
         for g0, g in windows:
             info, _, _ = ctx.group_sweep(n, mods, g0, g, low, seed=seed, high=high)
             if exchange and not high:
@@ -151,12 +152,15 @@ def measure_share(ctx: _lib.Context, n: int = 27, k: int = 4, shares: int = 8, s
             h["words"] += info["main_count"] + info["blocked_count"]
             h["kernel_s"] += info["kernel_s"]
             h["launches"] += info["launches"]
+            h["steps"] = info["steps"]  # passes over each window (a pair pass counts once)
             h["max_window_bytes"] = max(h["max_window_bytes"], info["state_bytes"])
             h["cell_ms"] = [a + b for a, b in zip(h["cell_ms"], info["cell_ms"])]
             if log:
                 log(f"share {share}: {name} cells, groups {g0}..{g0 + g - 1}: {info['state_bytes'] / 1e9:.1f} GB, "
                     f"{info['kernel_s']:.3f} s")
-        declared = 8.0 * k * h["words"] * cells
+        # declared bytes: one read and one write of every residue per pass over
+        # the window (a pair pass, which applies two cells, counts once)
+        declared = 8.0 * k * h["words"] * h["steps"]
         h["declared_bytes"] = declared
         h["transitions_per_s"] = cells * h["words"] * k / h["kernel_s"]
         h["roofline"] = {"bound": "hbm", "achieved": declared / h["kernel_s"] / 1e9, "peak": HBM_PEAK_GBS,

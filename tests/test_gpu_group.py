@@ -14,7 +14,6 @@ from group_masks import group_masks, occupancy_masks, window_order
 from ofp.path_bound import PRIMES
 
 pytestmark = pytest.mark.gpu
-GROUP_PAIRS = 1 << 10  # ofp_set_pair_passes mode bit: group sweeps run pair passes
 
 
 def _inputs(nm: int, mods, seed: int) -> np.ndarray:
@@ -179,7 +178,7 @@ def test_n27_pair_passes_equal_single_cells(high):
     out = {}
     for pairs in (True, False):
         with ofp.Context() as ctx:
-            ctx.set_pair_passes(1 | GROUP_PAIRS if pairs else 1)
+            ctx.set_group_launch(0, 0xFFFFFFFF if pairs else 0)
             print(f"case n=27 high={high} pairs={pairs}", flush=True)
             i, mo, bo = ctx.group_sweep(n, mods, g0, g, low, main_in=main, high=high, blocked_in=blk,
                                         blocked_out=not high)
@@ -206,7 +205,7 @@ def test_group_pair_passes_equal_oracle(oracle, n, low, k, high):
     main = _inputs(nm, mods, 4000 * n + low)
     blk = _inputs(nb, mods, 5000 * n + low) if high else None
     with ofp.Context() as ctx:
-        ctx.set_pair_passes(1 | GROUP_PAIRS)
+        ctx.set_group_launch(0, 0xFFFFFFFF)
         print(f"case pairs n={n} low={low} k={k} high={high}", flush=True)
         info, mo, bo = ctx.group_sweep(n, mods, 0, 1 << fixed, low, main_in=main[om], high=high,
                                        blocked_in=blk[ob] if high else None, blocked_out=not high)
@@ -226,7 +225,8 @@ def test_group_pair_passes_equal_oracle(oracle, n, low, k, high):
 @pytest.mark.parametrize("high", [False, True])
 def test_group_streams_same_results(high):
     """A window of many small and a few large groups (n = 27 shape) swept with its
-    group launches on one, two and four streams: bit-identical outputs."""
+    group launches on one, two and four streams (pair passes, single cells,
+    pair passes): bit-identical outputs."""
     n, low, mods = 27, 14, list(PRIMES[:4])
     first, groups = (2484, 12) if not high else (97, 12)
     ginfo, _, _ = ofp.group_plan(n, 4, first, groups, low, high=high)
@@ -234,7 +234,7 @@ def test_group_streams_same_results(high):
     outs = []
     with ofp.Context() as ctx:
         for ns in (1, 2, 4):
-            ctx.set_group_streams(ns)
+            ctx.set_group_launch(ns, 0 if ns == 2 else -1)
             print(f"case n=27 {'HIGH' if high else 'LOW'} groups {first}+{groups} streams={ns}", flush=True)
             _, mo, bo = ctx.group_sweep(n, mods, first, groups, low, main_in=main, blocked_out=not high, high=high)
             outs.append((mo, bo))
