@@ -47,6 +47,10 @@ struct Sweep {
     bool fixed = false;        // buffers are slices of the library's static device array (no graphs)
     int row = -1;              // rows swept since the initial state; -1: state of unknown origin
     std::vector<hipEvent_t> seg_ev;  // timing events of row segments (ofp_row_sweep)
+    // pair passes: two more streams for the launches of one pass that may run
+    // side by side (created on first use, not for the static sweep)
+    hipStream_t par[2] = {};
+    hipEvent_t pfork = nullptr, pjoin[2] = {};
 };
 
 struct ofp_ctx {
@@ -202,6 +206,14 @@ void free_sweep(Sweep& s) {
     drop_graph(s);
     for (hipEvent_t e : s.seg_ev) (void)hipEventDestroy(e);
     s.seg_ev.clear();
+    for (int q = 0; q < 2; ++q) {
+        if (s.par[q]) (void)hipStreamDestroy(s.par[q]);
+        if (s.pjoin[q]) (void)hipEventDestroy(s.pjoin[q]);
+        s.par[q] = nullptr;
+        s.pjoin[q] = nullptr;
+    }
+    if (s.pfork) (void)hipEventDestroy(s.pfork);
+    s.pfork = nullptr;
     sweep_free(s, s.main);
     sweep_free(s, s.blk);
     sweep_free(s, s.mods_dev);
@@ -473,6 +485,18 @@ uint32_t default_pair_mask(int) { return ~((1u << 4) | (1u << 3)); }
 // per share row) or slower (2, 1: 1127 against 957 ms).
 uint32_t default_group_pair_mask() { return ~0x7fu; }
 
+// The pair passes' extra streams and events of a sweep (created once, before
+// any row is captured into a graph).
+ofp_status pass_streams(ofp_ctx* c, Sweep& s) {
+    if (s.pfork) return OFP_OK;
+    OFP_HIP(c, hipEventCreateWithFlags(&s.pfork, hipEventDisableTiming), "pair pass event");
+    for (int q = 0; q < 2; ++q) {
+        OFP_HIP(c, hipStreamCreateWithFlags(&s.par[q], hipStreamNonBlocking), "pair pass stream");
+        OFP_HIP(c, hipEventCreateWithFlags(&s.pjoin[q], hipEventDisableTiming), "pair pass event");
+    }
+    return OFP_OK;
+}
+
 ofp_status enqueue_pairs(ofp_ctx* c, Sweep& s, int p_hi, int p_lo, bool first_reads_blk, int* launches,
                          hipEvent_t* cell_ev = nullptr, int* cell_kind = nullptr) {
     const int W = s.W;
@@ -492,8 +516,33 @@ ofp_status enqueue_pairs(ofp_ctx* c, Sweep& s, int p_hi, int p_lo, bool first_re
         for (int off = 0; off < s.Kp; off += c->pair_kc) {
             const int kc = std::min(c->pair_kc, s.Kp - off);
             static const int kOrder[7] = {-1, 0, 1, -3, -2, 2, 3};
+            // The readers (d = -1, 0, 1) never read each other's changed words and
+            // the d = -2, 2 classes read only their own, so each of the two groups
+            // of launches runs side by side on three streams (fork before, join
+            // after): their ramp-ups and tails overlap.
+            const bool par = !s.fixed;
+            if (par)
+                if (ofp_status f = pass_streams(c, s)) return f;
+            auto fork = [&]() -> ofp_status {
+                if (!par) return OFP_OK;
+                OFP_HIP(c, hipEventRecord(s.pfork, s.stream), "pair pass fork");
+                for (int q = 0; q < 2; ++q) OFP_HIP(c, hipStreamWaitEvent(s.par[q], s.pfork, 0), "pair pass fork");
+                return OFP_OK;
+            };
+            auto join = [&]() -> ofp_status {
+                if (!par) return OFP_OK;
+                for (int q = 0; q < 2; ++q) {
+                    OFP_HIP(c, hipEventRecord(s.pjoin[q], s.par[q]), "pair pass join");
+                    OFP_HIP(c, hipStreamWaitEvent(s.stream, s.pjoin[q], 0), "pair pass join");
+                }
+                return OFP_OK;
+            };
             for (int di = 0; di < 7; ++di) {
                 const int d = kOrder[di];
+                if (di == 3)
+                    if (ofp_status f = join()) return f;
+                if (di == 0 || di == 4)
+                    if (ofp_status f = fork()) return f;
                 // d = -3 / +3: the class is one word, RRR or LLL, which neither cell
                 // changes (no target at (p, p-1) nor at (p-1, p-2)): nothing to do
                 if (d == -3 || d == 3) continue;
@@ -518,10 +567,12 @@ ofp_status enqueue_pairs(ofp_ctx* c, Sweep& s, int p_hi, int p_lo, bool first_re
                 for (int jj = 0; jj < kc; ++jj) a.mod[jj] = s.mods[off + jj < s.K ? off + jj : 0];
                 a.xcd_shift = piece_shift(c->xcd_shift, cnt, c->threads);
                 a.variant = c->pair_variant;
-                hipError_t e = launch_pair(a, false, c->threads, grid_blocks(c, cnt), s.stream);
+                hipStream_t ls = !par ? s.stream : (d == -1 || d == -2) ? s.par[0] : (d == 1 ? s.par[1] : s.stream);
+                hipError_t e = launch_pair(a, false, c->threads, grid_blocks(c, cnt), ls);
                 if (e != hipSuccess) return hip_fail(c, e, "pair pass launch");
                 if (launches) ++*launches;
             }
+            if (ofp_status f = join()) return f;
         }
         if (cell_ev) {
             OFP_HIP(c, hipEventRecord(cell_ev[2 * slot + 1], s.stream), "event record");
@@ -560,6 +611,8 @@ ofp_status full_rows(ofp_ctx* c, Sweep& s, int rows, int* launches) {
     for (uint32_t m : s.mods) k += "/" + std::to_string(m);
     if (k != s.row_key || !s.row_exec) {
         drop_graph(s);
+        if (pairs)
+            if (ofp_status f = pass_streams(c, s)) return f;
         OFP_HIP(c, hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal), "begin capture");
         int captured = 0;
         ofp_status st = row(&captured);
