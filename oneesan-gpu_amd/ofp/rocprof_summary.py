@@ -70,23 +70,27 @@ def _pair_d(name: str) -> int | None:
 
 
 def _steps(seq):
-    """seq: [(t0, ms, capped, d)] (d None for a one-cell launch) -> steps
+    """seq: [(t0, t1, capped, d)] (ns; d None for a one-cell launch) -> steps
     [(t0, ms, capped, ncells)]: a one-cell launch is a step of one cell, the
-    consecutive pair launches of one pass a step of two."""
+    pair launches of one pass a step of two.  A pass's launches may run side by
+    side (its readers d = -1, 0, 1 on three streams, then d = -2, 2), so a step's
+    time is its wall span, first start to last end, and a new pass starts where a
+    launch's d is already in the current one (every d occurs once per pass)."""
     out = []
-    last = None  # order index of the previous pair launch of the current pass
-    for t0, ms, capped, d in sorted(seq):
+    seen = None  # d's of the current pass
+    for t0, t1, capped, d in sorted(seq):
         if d is None:
-            out.append([t0, ms, capped, 1])
-            last = None
+            out.append([t0, t1, capped, 1])
+            seen = None
             continue
-        o = _PAIR_ORDER.get(d, 0)
-        if last is not None and o > last:
-            out[-1][1] += ms
+        if seen is not None and d not in seen:
+            out[-1][1] = max(out[-1][1], t1)
+            out[-1][2] = out[-1][2] or capped
+            seen.add(d)
         else:
-            out.append([t0, ms, capped, 2])
-        last = o
-    return [tuple(x) for x in out]
+            out.append([t0, t1, capped, 2])
+            seen = {d}
+    return [(a, (b - a) / 1e6, c, n) for a, b, c, n in out]
 
 
 def _step_rows(seq, cells):
@@ -142,7 +146,7 @@ def trace(path: str, cells: int | None) -> None:
         by[name].append((t1 - t0) / 1e6)
         kc = _kc(name)
         if kc is not None:
-            fam[kc].append((t0, (t1 - t0) / 1e6, "capped" in name, _pair_d(name)))
+            fam[kc].append((t0, t1, "capped" in name, _pair_d(name)))
     print(f"{'kernel':48s} {'calls':>6s} {'mean_ms':>10s} {'min_ms':>9s} {'max_ms':>9s}")
     for k, v in sorted(by.items(), key=lambda kv: -sum(kv[1])):
         print(f"{k[:48]:48s} {len(v):6d} {sum(v) / len(v):10.3f} {min(v):9.3f} {max(v):9.3f}")

@@ -52,6 +52,28 @@ def test_trace_rows_with_pair_passes(tmp_path, capsys):
     assert "per step (p: ms): 5,4:5.0 3,2:6.0 1:7.0" in out
 
 
+def test_trace_pair_pass_with_overlapping_launches(tmp_path, capsys):
+    """A pass whose readers run side by side (three streams) is timed by its wall
+    span, not by the sum of its launches' durations."""
+    ms = 1_000_000
+    rows = []
+    for base in (0, 100 * ms):  # two rows of one pass (2 cells) each
+        rows += [(PAIR.format(d=-1), base + 0, base + 10 * ms), (PAIR.format(d=0), base + 0, base + 12 * ms),
+                 (PAIR.format(d=1), base + 1 * ms, base + 9 * ms), (PAIR.format(d=-2), base + 12 * ms,
+                 base + 15 * ms), (PAIR.format(d=2), base + 12 * ms, base + 14 * ms)]
+    p = tmp_path / "t.csv"
+    with open(p, "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=["Kernel_Name", "Start_Timestamp", "End_Timestamp"])
+        w.writeheader()
+        for name, t0, t1 in rows:
+            w.writerow({"Kernel_Name": name, "Start_Timestamp": t0, "End_Timestamp": t1})
+    rs.trace(str(p), 2)
+    out = capsys.readouterr().out
+    assert "KC=14: 2 rows of 2 cells" in out
+    assert "15.0(0) 15.0(0)" in out
+    assert "per step (p: ms): 2,1:15.0" in out
+
+
 def test_counter_calibration(tmp_path, capsys):
     p = tmp_path / "c.csv"
     with open(p, "w", newline="") as f:
