@@ -136,7 +136,7 @@ def _trace_rows(path: str) -> list[dict]:
     return list(csv.DictReader(open(path)))
 
 
-def trace(path: str, cells: int | None) -> None:
+def trace(path: str, cells: int | None, kcs: set[int] | None = None) -> None:
     rows = _trace_rows(path)
     by = collections.defaultdict(list)
     fam = collections.defaultdict(list)
@@ -153,6 +153,8 @@ def trace(path: str, cells: int | None) -> None:
     if not cells:
         return
     for kc, seq in sorted(fam.items(), reverse=True):
+        if kcs and kc not in kcs:
+            continue
         rws = _step_rows(seq, cells)
         if not rws:
             continue
@@ -337,6 +339,8 @@ def main(argv=None) -> int:
     ap.add_argument("--rows", type=int, default=None,
                     help="only the first ROWS rows of each moduli count (one solve's W rows; later "
                          "launches of other sizes are left out)")
+    ap.add_argument("--kc", type=str, default=None,
+                    help="moduli per launch whose rows to list, comma separated (trace; default all)")
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--cus", type=int, default=256, help="compute units (MI355X: 256)")
     ap.add_argument("--xcds", type=int, default=8, help="XCDs GRBM_GUI_ACTIVE is summed over (MI355X: 8)")
@@ -345,7 +349,7 @@ def main(argv=None) -> int:
     global MAX_ROWS
     MAX_ROWS = a.rows
     if a.mode == "trace":
-        trace(a.csv, a.cells)
+        trace(a.csv, a.cells, {int(x) for x in a.kc.split(",")} if a.kc else None)
     elif a.mode == "counter":
         counter(a.csv, a.scale, a.cells)
     elif a.mode == "dram":

@@ -72,6 +72,8 @@ def test_trace_pair_pass_with_overlapping_launches(tmp_path, capsys):
     assert "KC=14: 2 rows of 2 cells" in out
     assert "15.0(0) 15.0(0)" in out
     assert "per step (p: ms): 2,1:15.0" in out
+    rs.trace(str(p), 2, {1, 4})  # rows of other moduli counts only: none here
+    assert "KC=14" not in capsys.readouterr().out
 
 
 def test_counter_calibration(tmp_path, capsys):
