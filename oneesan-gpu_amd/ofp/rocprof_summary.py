@@ -209,6 +209,8 @@ VALU_COUNTERS = ("SQ_ACTIVE_INST_VALU", "SQ_THREAD_CYCLES_VALU", "SQ_INSTS_VALU"
 
 
 def valu(path: str, cells: int, cus: int, xcds: int, words: int | None) -> None:
+    """Per step (a one-cell launch, or the launches of one pair pass, whose
+    counters are summed) of the last full-state row of each moduli count."""
     by = collections.defaultdict(dict)
     meta = {}
     for r in csv.DictReader(open(path)):
@@ -216,32 +218,59 @@ def valu(path: str, cells: int, cus: int, xcds: int, words: int | None) -> None:
         by[d][r["Counter_Name"]] = float(r["Counter_Value"])
         meta[d] = (_short(r["Kernel_Name"]), int(r["Start_Timestamp"]), int(r["End_Timestamp"]))
     fam = collections.defaultdict(list)
-    for d, (name, t0, t1) in meta.items():
+    for d, (name, t0, t1) in sorted(meta.items()):
         kc = _kc(name)
         if kc is not None and all(c in by[d] for c in VALU_COUNTERS):
-            fam[kc].append((d, d, "capped" in name))
+            fam[kc].append((d, "capped" in name, _pair_d(name)))
     for kc, seq in sorted(fam.items(), reverse=True):
-        full = [r for r in _rows(seq, cells) if not any(x[2] for x in r)]
+        # steps in dispatch order: [dispatches, capped, cells]
+        steps, seen = [], None
+        for d, capped, pd in seq:
+            if pd is None:
+                steps.append([[d], capped, 1])
+                seen = None
+            elif seen is not None and pd not in seen:
+                steps[-1][0].append(d)
+                seen.add(pd)
+            else:
+                steps.append([[d], capped, 2])
+                seen = {pd}
+        rows, cur, n = [], [], 0
+        for st in steps:
+            cur.append(st)
+            n += st[2]
+            if n >= cells:
+                rows.append(cur)
+                cur, n = [], 0
+        if MAX_ROWS:
+            rows = rows[:MAX_ROWS]
+        full = [r for r in rows if not any(x[1] for x in r)]
         if not full:
             continue
         row = full[-1]
-        print(f"KC={kc}: last full-state row ({len(full)} full-state rows); per cell p={cells}..1")
-        print(f"{'p':>3s} {'ms':>7s} {'busy%':>6s} {'lanes%':>6s} {'VALU/64w':>9s} {'SALU/64w':>9s} "
+        print(f"KC={kc}: last full-state row ({len(full)} full-state rows); per step, top of the row first")
+        print(f"{'p':>5s} {'ms':>7s} {'busy%':>6s} {'lanes%':>6s} {'VALU/64w':>9s} {'SALU/64w':>9s} "
               f"{'LDS/64w':>8s} {'VMRD/64w':>9s} {'VMWR/64w':>9s}")
         tot_valu = tot_cyc = 0.0
-        for i, x in enumerate(row):
-            d = x[0]
-            b = by[d]
-            _, t0, t1 = meta[d]
-            cyc = b["GRBM_GUI_ACTIVE"] / xcds
-            busy = 100.0 * b["SQ_ACTIVE_INST_VALU"] / cus / cyc
-            lanes = 100.0 * b["SQ_THREAD_CYCLES_VALU"] / (b["SQ_ACTIVE_INST_VALU"] * 64)
-            w64 = (words / 64.0) if words else b["SQ_WAVES"]
-            tot_valu += b["SQ_ACTIVE_INST_VALU"]
+        p = cells
+        for ds, _, nc in row:
+            agg = collections.Counter()
+            ms = 0.0
+            for d in ds:
+                for c in VALU_COUNTERS:
+                    agg[c] += by[d][c]
+                ms += (meta[d][2] - meta[d][1]) / 1e6
+            cyc = agg["GRBM_GUI_ACTIVE"] / xcds
+            busy = 100.0 * agg["SQ_ACTIVE_INST_VALU"] / cus / cyc
+            lanes = 100.0 * agg["SQ_THREAD_CYCLES_VALU"] / (agg["SQ_ACTIVE_INST_VALU"] * 64)
+            w64 = (words / 64.0) if words else agg["SQ_WAVES"]
+            tot_valu += agg["SQ_ACTIVE_INST_VALU"]
             tot_cyc += cyc
-            print(f"{cells - i:3d} {(t1 - t0) / 1e6:7.2f} {busy:6.1f} {lanes:6.1f} "
-                  f"{b['SQ_INSTS_VALU'] / w64:9.0f} {b['SQ_INSTS_SALU'] / w64:9.0f} {b['SQ_INSTS_LDS'] / w64:8.1f} "
-                  f"{b['SQ_INSTS_VMEM_RD'] / w64:9.1f} {b['SQ_INSTS_VMEM_WR'] / w64:9.1f}")
+            label = f"{p},{p - 1}" if nc == 2 else f"{p}"
+            print(f"{label:>5s} {ms:7.2f} {busy:6.1f} {lanes:6.1f} "
+                  f"{agg['SQ_INSTS_VALU'] / w64:9.0f} {agg['SQ_INSTS_SALU'] / w64:9.0f} {agg['SQ_INSTS_LDS'] / w64:8.1f} "
+                  f"{agg['SQ_INSTS_VMEM_RD'] / w64:9.1f} {agg['SQ_INSTS_VMEM_WR'] / w64:9.1f}")
+            p -= nc
         print(f"  row: VALU busy {100.0 * tot_valu / cus / tot_cyc:.1f}% of the CUs' VALU issue cycles")
 
 

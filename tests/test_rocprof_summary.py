@@ -113,6 +113,32 @@ def test_valu_busy_and_lanes(tmp_path, capsys):
     assert "row: VALU busy 25.0%" in out
 
 
+def test_valu_sums_a_pair_pass(tmp_path, capsys):
+    """The launches of one pair pass (d = -1, 0, 1, -2, 2) are one step of two
+    cells, their counters summed."""
+    p = tmp_path / "v.csv"
+    fields = ["Kernel_Name", "Counter_Name", "Counter_Value", "Dispatch_Id", "Start_Timestamp", "End_Timestamp"]
+    with open(p, "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=fields)
+        w.writeheader()
+        did = 0
+        for row in range(2):
+            for d, busy in ((-1, 1000.0), (0, 3000.0), (1, 1000.0), (-2, 1000.0), (2, 2000.0)):
+                vals = {"SQ_ACTIVE_INST_VALU": 256 * busy, "SQ_THREAD_CYCLES_VALU": 256 * busy * 64,
+                        "SQ_INSTS_VALU": 0.0, "SQ_INSTS_SALU": 0.0, "SQ_INSTS_LDS": 0.0,
+                        "SQ_INSTS_VMEM_RD": 0.0, "SQ_INSTS_VMEM_WR": 0.0, "SQ_WAVES": 64.0,
+                        "GRBM_GUI_ACTIVE": 8 * 4000.0}
+                for name, v in vals.items():
+                    w.writerow({"Kernel_Name": PAIR.format(d=d), "Counter_Name": name, "Counter_Value": v,
+                                "Dispatch_Id": did, "Start_Timestamp": 0, "End_Timestamp": 1000000})
+                did += 1
+    rs.valu(str(p), 2, 256, 8, None)
+    out = capsys.readouterr().out
+    assert "(2 full-state rows)" in out
+    # 8000 VALU issue cycles per CU over 5 x 4000 cycles: 40 %, every lane active
+    assert "  2,1    5.00   40.0  100.0" in out
+
+
 SORTED = "void ofp::(anonymous namespace)::cell_sorted_kernel<14, unsigned int>(ofp::CellLaunch)"
 
 
