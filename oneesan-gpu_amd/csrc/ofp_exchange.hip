@@ -221,12 +221,17 @@ __global__ __launch_bounds__(kXThreads) void xchg_write_kernel(XchgLaunch a) {
             if (sh[j] < 0) continue;
             const uint32_t i = r0 + (uint32_t)(j * kXThreads + tid);
             const uint64_t slot = sbase[j * kXWaves + wave][sh[j]] + pre[j];
-            a.out_idx[slot] = wdst[j * kXThreads + tid];
+            // records are written once and not read here again: non-temporal stores
+            __builtin_nontemporal_store(wdst[j * kXThreads + tid], &a.out_idx[slot]);
             const uint32_t* src = a.res + (t.word0 + i) * (uint64_t)a.K;
             uint32_t* out = a.out_res + slot * (uint64_t)a.K;
             if (even) {
-                *reinterpret_cast<uint2*>(out) = v01[j];
-                if (a.K >= 4) *reinterpret_cast<uint2*>(out + 2) = v23[j];
+                __builtin_nontemporal_store(v01[j].x, out);
+                __builtin_nontemporal_store(v01[j].y, out + 1);
+                if (a.K >= 4) {
+                    __builtin_nontemporal_store(v23[j].x, out + 2);
+                    __builtin_nontemporal_store(v23[j].y, out + 3);
+                }
                 for (int k = 4; k < a.K; k += 2)
                     *reinterpret_cast<uint2*>(out + k) = *reinterpret_cast<const uint2*>(src + k);
             } else {
