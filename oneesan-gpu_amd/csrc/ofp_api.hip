@@ -71,6 +71,7 @@ struct ofp_ctx {
     void* dpc32 = nullptr; // packed capped tables, caps 1..kMaxCap (uint32, uint64)
     void* dpc64 = nullptr;
     void* ct32 = nullptr;  // pair-pass class tables, every (p, d) (32-bit ranks)
+    void* ct64 = nullptr;  // the same with 64-bit entries (n >= 23, or 64-bit ranks forced)
     int pair_rows = 1;     // 1: full-state rows run as pair passes (two cells per pass, in place)
     int pair_kc = kMaxKC;  // moduli per pair-pass launch (even, <= 16)
     uint32_t pair_mask = 0;  // bit p: cells p and p-1 run as one pair pass; 0: the library's default
@@ -290,10 +291,9 @@ struct Carve {
 
 // Ensure device buffers of s for (n, moduli) in the context's current layout.
 // Whether n in the context's settings runs its full rows as pair passes: pair
-// mode on, state-major layout, 32-bit ranks (n <= 22), at least two cells.
-bool pairs_apply(const ofp_ctx* c, int n) {
-    return c->pair_rows && c->layout == 1 && n + 1 <= 23 && c->index_bits != 64 && n >= 2;
-}
+// mode on, state-major layout, at least two cells (32-bit ranks for n <= 22,
+// 64-bit above or when forced).
+bool pairs_apply(const ofp_ctx* c, int n) { return c->pair_rows && c->layout == 1 && n >= 2; }
 
 ofp_status ensure_state(ofp_ctx* c, Sweep& s, int n, const uint32_t* moduli, int k) {
     uint64_t nm, nb;
@@ -500,7 +500,8 @@ ofp_status pass_streams(ofp_ctx* c, Sweep& s) {
 ofp_status enqueue_pairs(ofp_ctx* c, Sweep& s, int p_hi, int p_lo, bool first_reads_blk, int* launches,
                          hipEvent_t* cell_ev = nullptr, int* cell_kind = nullptr) {
     const int W = s.W;
-    const size_t tb = class_table_bytes(false);
+    const bool wide = s.W > 23 || c->index_bits == 64;
+    const size_t tb = class_table_bytes(wide);
     const uint32_t mask = c->pair_mask ? c->pair_mask : default_pair_mask(std::min(s.K, c->pair_kc));
     for (int p = p_hi; p >= p_lo;) {
         const int slot = p_hi - p;
@@ -553,8 +554,8 @@ ofp_status enqueue_pairs(ofp_ctx* c, Sweep& s, int p_hi, int p_lo, bool first_re
                 a.blk_in = s.blk;
                 a.main_out = s.main;
                 a.blk_out = s.blk;
-                a.dp = c->dp32;
-                a.ct = static_cast<const char*>(c->ct32) + tb * (size_t)(p * 7 + d + 3);
+                a.dp = wide ? c->dp64 : c->dp32;
+                a.ct = static_cast<const char*>(wide ? c->ct64 : c->ct32) + tb * (size_t)(p * 7 + d + 3);
                 a.classes = cnt;
                 a.W = W;
                 a.p = p;
@@ -568,7 +569,7 @@ ofp_status enqueue_pairs(ofp_ctx* c, Sweep& s, int p_hi, int p_lo, bool first_re
                 a.xcd_shift = piece_shift(c->xcd_shift, cnt, c->threads);
                 a.variant = c->pair_variant;
                 hipStream_t ls = !par ? s.stream : (d == -1 || d == -2) ? s.par[0] : (d == 1 ? s.par[1] : s.stream);
-                hipError_t e = launch_pair(a, false, c->threads, grid_blocks(c, cnt), ls);
+                hipError_t e = launch_pair(a, wide, c->threads, grid_blocks(c, cnt), ls);
                 if (e != hipSuccess) return hip_fail(c, e, "pair pass launch");
                 if (launches) ++*launches;
             }
@@ -871,6 +872,7 @@ ofp_status ofp_create(const ofp_config* cfg, ofp_ctx** out) {
         if (c->dpc32) (void)hipFree(c->dpc32);
         if (c->dpc64) (void)hipFree(c->dpc64);
         if (c->ct32) (void)hipFree(c->ct32);
+        if (c->ct64) (void)hipFree(c->ct64);
         if (c->ev0) (void)hipEventDestroy(c->ev0);
         if (c->ev1) (void)hipEventDestroy(c->ev1);
         if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -921,6 +923,14 @@ ofp_status ofp_create(const ofp_config* cfg, ofp_ctx** out) {
         return OFP_E_RUNTIME;
     }
     if (bad(hipMalloc(&c->ct32, bt32)) || bad(hipMemcpy(c->ct32, t32.data(), bt32, hipMemcpyHostToDevice)))
+        return OFP_E_RUNTIME;
+    std::vector<unsigned char> t64(class_table_bytes(true) * kMaxW * 7);
+    const size_t bt64 = pack_class_tables(true, t64.data(), t64.size());
+    if (!bt64) {
+        (void)bad(hipErrorInvalidValue);
+        return OFP_E_RUNTIME;
+    }
+    if (bad(hipMalloc(&c->ct64, bt64)) || bad(hipMemcpy(c->ct64, t64.data(), bt64, hipMemcpyHostToDevice)))
         return OFP_E_RUNTIME;
     *out = c;
     return OFP_OK;
@@ -2054,6 +2064,7 @@ void ofp_destroy(ofp_ctx* c) {
     if (c->dpc32) (void)hipFree(c->dpc32);
     if (c->dpc64) (void)hipFree(c->dpc64);
     if (c->ct32) (void)hipFree(c->ct32);
+    if (c->ct64) (void)hipFree(c->ct64);
     for (int i = 0; i < 3; ++i) {
         if (c->gstream[i]) (void)hipStreamDestroy(c->gstream[i]);
         if (c->gjoin[i]) (void)hipEventDestroy(c->gjoin[i]);

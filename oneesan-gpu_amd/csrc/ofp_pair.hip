@@ -121,7 +121,20 @@ uint64_t pair_class_count(int W, int p, int d) {
 
 hipError_t launch_pair(const pair::PairLaunch& a, bool wide, int threads, int blocks, hipStream_t s) {
     if (a.p < 2 || a.p > a.W - 1 || a.d < -3 || a.d > 3 || a.kc < 1 || a.off + a.kc > a.K) return hipErrorInvalidValue;
-    if (wide) return hipErrorInvalidValue;  // 64-bit ranks are not built for the pair pass (n <= 22 only)
+    if (wide) {
+        switch (a.kc) {
+            case 1: return launch_gf<1, uint64_t>(a, threads, blocks, s);
+            case 2: return launch_gf<2, uint64_t>(a, threads, blocks, s);
+            case 4: return launch_gf<4, uint64_t>(a, threads, blocks, s);
+            case 6: return launch_gf<6, uint64_t>(a, threads, blocks, s);
+            case 8: return launch_gf<8, uint64_t>(a, threads, blocks, s);
+            case 10: return launch_gf<10, uint64_t>(a, threads, blocks, s);
+            case 12: return launch_gf<12, uint64_t>(a, threads, blocks, s);
+            case 14: return launch_gf<14, uint64_t>(a, threads, blocks, s);
+            case 16: return launch_gf<16, uint64_t>(a, threads, blocks, s);
+            default: return hipErrorInvalidValue;
+        }
+    }
     switch (a.kc) {
         case 1: return launch_gf<1, uint32_t>(a, threads, blocks, s);
         case 2: return launch_gf<2, uint32_t>(a, threads, blocks, s);

@@ -117,3 +117,31 @@ def test_pair_rows_split_over_calls(pair_ctx):
     pair_ctx.row_sweep(7)
     pair_ctx.row_sweep(n + 1 - 10)
     assert pair_ctx.read_main(0, len(mods)) == want
+
+
+@pytest.mark.parametrize("k", [1, 8, 14])
+def test_pair_windows_64bit_ranks(pair_ctx, oracle, k):
+    """The pair pass with 64-bit ranks (the path n >= 23 takes), forced at small n:
+    every pair of cells on random vectors, blocked input on and off, against the
+    oracle's cells."""
+    import ofp
+    mods = ([2] + PRIMES)[:k] if k > 1 else [4294967291]
+    try:
+        pair_ctx.set_index_width(64)
+        for n in (3, 7, 11):
+            nm, nb = ofp.state_counts(n)
+            W = n + 1
+            rng = np.random.default_rng(0x64A1 + 31 * n + k)
+            for p_hi in range(W - 1, 1, -1):
+                for bv in (True, False):
+                    main = np.stack([rng.integers(0, m, nm, dtype=np.uint64) for m in mods], 1).astype(np.uint32)
+                    blk = np.stack([rng.integers(0, m, nb, dtype=np.uint64) for m in mods], 1).astype(np.uint32)
+                    g_main, g_blk = main.copy(), blk.copy()
+                    print(f"case 64-bit pair n={n} k={k} window={p_hi}..{p_hi - 1} blocked={bv}", flush=True)
+                    pair_ctx.apply_cells(n, mods, g_main, g_blk, p_hi, p_hi - 1, bv)
+                    for j in sorted({0, k // 2, k - 1}):
+                        em, eb = _oracle_cells(oracle, n, mods[j], main[:, j], blk[:, j], p_hi, p_hi - 1, bv)
+                        assert np.array_equal(g_main[:, j], em), (n, k, p_hi, bv, j, "main")
+                        assert np.array_equal(g_blk[:, j], eb), (n, k, p_hi, bv, j, "blocked")
+    finally:
+        pair_ctx.set_index_width(0)
