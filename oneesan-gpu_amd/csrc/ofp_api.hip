@@ -1933,13 +1933,25 @@ ofp_status ofp_group_exchange(ofp_ctx* c, const ofp_group_spec* sp, int shares, 
         return a;
     };
     const XchgLaunch am = launch_of(true), ab = launch_of(false);
+    // The main and blocked words' launches are independent (own tiles, counts,
+    // totals and buckets): the blocked ones run on a second stream beside them
+    // where the group sweeps' streams exist.
+    hipStream_t xs = c->gfork ? c->gstream[0] : c->stream;
     OFP_HIP(c, hipEventRecord(c->ev0, c->stream), "event");
+    if (xs != c->stream) {
+        OFP_HIP(c, hipEventRecord(c->gfork, c->stream), "exchange fork");
+        OFP_HIP(c, hipStreamWaitEvent(xs, c->gfork, 0), "exchange fork");
+    }
+    OFP_HIP(c, launch_exchange_count(ab, xs), "exchange count");
+    OFP_HIP(c, launch_exchange_scan(ab, xs), "exchange scan");
+    OFP_HIP(c, launch_exchange_write(ab, xs), "exchange write");
     OFP_HIP(c, launch_exchange_count(am, c->stream), "exchange count");
-    OFP_HIP(c, launch_exchange_count(ab, c->stream), "exchange count");
     OFP_HIP(c, launch_exchange_scan(am, c->stream), "exchange scan");
-    OFP_HIP(c, launch_exchange_scan(ab, c->stream), "exchange scan");
     OFP_HIP(c, launch_exchange_write(am, c->stream), "exchange write");
-    OFP_HIP(c, launch_exchange_write(ab, c->stream), "exchange write");
+    if (xs != c->stream) {
+        OFP_HIP(c, hipEventRecord(c->gjoin[0], xs), "exchange join");
+        OFP_HIP(c, hipStreamWaitEvent(c->stream, c->gjoin[0], 0), "exchange join");
+    }
     OFP_HIP(c, hipEventRecord(c->ev1, c->stream), "event");
     OFP_HIP(c, hipEventSynchronize(c->ev1), "exchange sync");
     float ms = 0;
