@@ -1474,9 +1474,8 @@ ofp_status ofp_group_sweep(ofp_ctx* c, const ofp_group_spec* sp, ofp_group_info*
     OFP_HIP(c, hipSetDevice(c->device), "hipSetDevice");
     const size_t mbytes = std::max<uint64_t>(M, 1) * Kp * 4, bbytes = std::max<uint64_t>(B, 1) * Kp * 4;
     const size_t tbytes = (size_t)G * kGroupTableBytes;
-    // The half's steps: cells p and p-1 as one pair pass where group pair passes
-    // are on and the context's pair settings pair them (the full-state rows'
-    // mask), else one cell.
+    // The half's steps: cells p and p-1 as one pair pass where the group pair
+    // mask (ofp_set_group_launch) pairs them, else one cell.
     const int p_hi = sp->high ? L : W - 1, p_lo = sp->high ? 1 : L + 1;
     const uint32_t pmask = c->group_pair_mask >= 0 ? (uint32_t)c->group_pair_mask : default_group_pair_mask();
     std::vector<int> steps;  // p of each step; a pair step is marked by p | 0x100
