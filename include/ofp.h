@@ -154,10 +154,12 @@ ofp_status ofp_time_cells(ofp_ctx* ctx, float* ms_out, int* kind_out, int capaci
 /* Pair passes (default) or one launch per cell for the whole-state rows
  * of the context's own solves (ofp_solve, ofp_prepare + ofp_row_sweep,
  * ofp_apply_cells, ofp_time_cells).  A pair pass applies cells p and p-1 in
- * place, in seven launches (one per window height change), so every word is
- * read and written once per two cells instead of once per cell.  Rows with
- * height-capped cells still run one launch per cell.  32-bit ranks (n <= 22)
- * only; elsewhere the setting is ignored.  Results do not depend on it.
+ * place, in up to five launches (one per window height change that has
+ * classes; the three that read other classes side by side, then the other
+ * two), so every word is read and written once per two cells instead of once
+ * per cell.  Height-capped cells still run one launch per cell.  32-bit and
+ * 64-bit ranks (n >= 23) alike; the static solve (ofp_solve_static) runs the
+ * launches one after another.  Results do not depend on it.
  * mode: 0 = off, 1 = on (the default), 2..16 (even) = on with at most that
  * many moduli per launch (default 16).  Walking down a row, cells p and p-1 run
  * as one pair pass when bit p of cell_mask is set (~0u: everywhere), else cell
