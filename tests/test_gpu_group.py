@@ -238,6 +238,7 @@ def test_group_streams_same_results(high):
             print(f"case n=27 {'HIGH' if high else 'LOW'} groups {first}+{groups} streams={ns}", flush=True)
             _, mo, bo = ctx.group_sweep(n, mods, first, groups, low, main_in=main, blocked_out=not high, high=high)
             outs.append((mo, bo))
+    assert not np.array_equal(outs[0][0], main)  # the sweep changed the window
     for mo, bo in outs[1:]:
         assert np.array_equal(mo, outs[0][0])
         if not high:
