@@ -488,12 +488,12 @@ uint32_t default_group_pair_mask() { return ~0x7fu; }
 // The pair passes' extra streams and events of a sweep (created once, before
 // any row is captured into a graph).
 ofp_status pass_streams(ofp_ctx* c, Sweep& s) {
-    if (s.pfork) return OFP_OK;
-    OFP_HIP(c, hipEventCreateWithFlags(&s.pfork, hipEventDisableTiming), "pair pass event");
+    if (s.pfork) return OFP_OK;  // set last: all of them exist
     for (int q = 0; q < 2; ++q) {
-        OFP_HIP(c, hipStreamCreateWithFlags(&s.par[q], hipStreamNonBlocking), "pair pass stream");
-        OFP_HIP(c, hipEventCreateWithFlags(&s.pjoin[q], hipEventDisableTiming), "pair pass event");
+        if (!s.par[q]) OFP_HIP(c, hipStreamCreateWithFlags(&s.par[q], hipStreamNonBlocking), "pair pass stream");
+        if (!s.pjoin[q]) OFP_HIP(c, hipEventCreateWithFlags(&s.pjoin[q], hipEventDisableTiming), "pair pass event");
     }
+    OFP_HIP(c, hipEventCreateWithFlags(&s.pfork, hipEventDisableTiming), "pair pass event");
     return OFP_OK;
 }
 
@@ -1573,12 +1573,13 @@ ofp_status ofp_group_sweep(ofp_ctx* c, const ofp_group_spec* sp, ofp_group_info*
         OFP_HIP_R(launch_fill_remote(dm, dom, fillM, db, dom, fillB, Kp, k, dmods, sp->seed, c->stream),
                   "fill group window");
     const int sf = c->sorted_from >= 0 ? c->sorted_from : (k >= 8 ? 6 : 0);
-    if (c->group_streams > 1 && !c->gfork) {
-        OFP_HIP_R(hipEventCreateWithFlags(&c->gfork, hipEventDisableTiming), "group event");
+    if (c->group_streams > 1 && !c->gfork) {  // gfork is created last: all of them exist
         for (int q = 0; q < 3; ++q) {
-            OFP_HIP_R(hipStreamCreateWithFlags(&c->gstream[q], hipStreamNonBlocking), "group stream");
-            OFP_HIP_R(hipEventCreateWithFlags(&c->gjoin[q], hipEventDisableTiming), "group event");
+            if (!c->gstream[q])
+                OFP_HIP_R(hipStreamCreateWithFlags(&c->gstream[q], hipStreamNonBlocking), "group stream");
+            if (!c->gjoin[q]) OFP_HIP_R(hipEventCreateWithFlags(&c->gjoin[q], hipEventDisableTiming), "group event");
         }
+        OFP_HIP_R(hipEventCreateWithFlags(&c->gfork, hipEventDisableTiming), "group event");
     }
     // The groups of a step are independent (each has its own words), so their
     // launches are dealt to ns streams that run side by side: the many small
