@@ -114,7 +114,7 @@ struct XTile {
     uint32_t group, rank0, len, pad;
 };
 constexpr int kMaxShares = 16;
-constexpr uint32_t kXTileWords = 8192;  // words per tile (at most)
+constexpr uint32_t kXTileWords = 32768;  // words per tile (at most)
 struct XchgLaunch {
     const uint32_t* res;     // the window's residues of this kind (main or blocked), stride K
     const char* gtab;        // the window's packed 32-bit group tables (kDpRows x kDpCols quads each)
