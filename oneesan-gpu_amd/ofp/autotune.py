@@ -78,6 +78,25 @@ def lookup(n: int, k: int, path: Path = DEFAULT_CACHE) -> dict | None:
     return min(cands, key=lambda t: t[0])[1] if cands else None
 
 
+def pair_mask_for(n: int, k: int, path: Path = DEFAULT_CACHE) -> int:
+    """The pair mask to run (n, k) with: the one tuned for (n, k), else for the
+    same n and the nearest moduli count (one modulus or several first), else 0
+    (the library's default).  A mask names cell positions, which shift with n, so
+    one tuned at another n does not carry over."""
+    key = _key(n, k)
+    tables = (load_cache(path), load_cache(PACKAGED))
+    for table in tables:
+        if key in table:
+            return int(table[key].get("pair_mask", 0))
+    for table in tables:
+        same_n = [(((kk > 1) != (k > 1)), abs(kk - k), kk, v) for kk, v in
+                  ((int(kk_.split(",")[1][2:]), v_) for kk_, v_ in table.items()
+                   if int(kk_.split(",")[0][2:]) == n)]
+        if same_n:
+            return int(min(same_n, key=lambda t: t[:3])[3].get("pair_mask", 0))
+    return 0
+
+
 def apply(ctx: "_lib.Context", n: int, k: int) -> dict | None:
     """Set the best known launch shape and layout on ctx (before prepare/solve)."""
     ent = lookup(n, k)
@@ -89,7 +108,7 @@ def apply(ctx: "_lib.Context", n: int, k: int) -> dict | None:
         ctx.set_height_caps(int(ent.get("cap_percent", _lib.DEFAULT_CAP_PERCENT)))
         ctx.set_xcd_pieces(int(ent.get("xcd_shift", 5)))
         ctx.set_sorted_cells(int(ent.get("sorted_from", -1)))
-        ctx.set_pair_passes(int(ent.get("pair_mode", 1)), int(ent.get("pair_mask", 0)))
+        ctx.set_pair_passes(int(ent.get("pair_mode", 1)), pair_mask_for(n, k))
     return ent
 
 

@@ -61,6 +61,21 @@ def test_lookup_ties_in_n_go_to_nearest_k(tmp_path):
         autotune.PACKAGED = old
 
 
+def test_pair_mask_only_from_an_entry_of_the_same_n(tmp_path):
+    """A tuned pair mask names cell positions: it is used for its own (n, k) and not
+    carried to another n by the nearest-entry fallback (0 = the library default)."""
+    from ofp import autotune
+    path = tmp_path / "user.json"
+    autotune.save_cache({"n=19,k=3": {"threads": 256, "pair_mask": 0x1234}}, path)
+    assert autotune.pair_mask_for(19, 3, path) == 0x1234
+    packaged = autotune.load_cache(autotune.PACKAGED)
+    assert autotune.pair_mask_for(22, 14, path) == int(packaged["n=22,k=14"].get("pair_mask", 0))
+    assert autotune.lookup(23, 8, path) is not None  # launch settings fall back to n=22 ...
+    assert autotune.pair_mask_for(23, 8, path) == 0  # ... the pair mask does not
+    assert autotune.pair_mask_for(27, 4, path) == 0
+    assert autotune.pair_mask_for(19, 5, path) == 0x1234  # same n, another moduli count
+
+
 @pytest.mark.gpu
 def test_tiny_tune_keeps_residues(gpu_ctx, tmp_path):
     """A reduced tuning run at n=12 (every stage, a few candidates each) writes its
